@@ -1,0 +1,185 @@
+"""Benchmark: committed field elements/s for coset LDE + Poseidon2 Merkle commit.
+
+Default workload (one MI355X): BabyBear trace of 2^20 rows x 256 columns,
+log_blowup 1 (2^21 x 256 LDE, 2 GiB), shift 31, Poseidon2 width 16 / x^7 /
+8 full + 13 partial rounds with seeded constants, padding-free sponge leaves
+(rate 8, digest 8), 2-to-1 truncated-permutation tree, cap 1 root.
+A step = coset LDE + leaf hashing + tree compression of one resident trace,
+ending with the root copied to the host.  Inputs are resident in HBM before
+the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME]
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+BB_P = 0x78000001
+GL_P = 0xFFFFFFFF00000001
+
+# Published MI355X peaks used for roofline fractions.
+HBM_PEAK_GBS = 8000.0
+# 32-bit VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (full-rate ops).
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+WORKLOADS = {
+    # name: (field, log_n, width, log_blowup, poseidon2 rounds_p)
+    "bb_2^20x256_b1": ("bb", 20, 256, 1, 13),
+    "bb_2^22x64_b1": ("bb", 22, 64, 1, 13),
+    "bb_2^16x32_b1": ("bb", 16, 32, 1, 13),
+    "gl_2^22x64_b2": ("gl", 22, 64, 2, 22),
+}
+DEFAULT = "bb_2^20x256_b1"
+
+
+def load_p3hip():
+    spec = importlib.util.spec_from_file_location("p3hip", os.path.join(ROOT, "plonky3-gpu_amd", "p3hip.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["p3hip"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def p2_constants(field: str, rp: int, seed: int = 1) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    if field == "bb":
+        return rng.integers(0, BB_P, size=8 * 16 + rp, dtype=np.uint32)
+    return rng.integers(0, GL_P, size=8 * 8 + rp, dtype=np.uint64)
+
+
+def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
+    """The reference's own CPU path (oracle/_ref, compiled from the reference
+    sources in the build container) on a bounded sample of the same workload:
+    2^15 rows x width, same blowup, LDE + single-matrix Poseidon2 commit, one
+    thread.  Falls back to the oracle restatement when the reference build is
+    absent."""
+    if field != "bb":
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import native  # test infrastructure; used here only as the CPU baseline
+
+    lib = native.reference()
+    kind = "reference"
+    if lib is None:
+        lib, kind = native.oracle(), "port"
+    log_n = 15
+    n = 1 << log_n
+    rng = np.random.default_rng(3)
+    trace = rng.integers(0, BB_P, size=(n, width), dtype=np.uint32)
+    consts = p2_constants("bb", rp)
+    t0 = time.perf_counter()
+    lde = lib.lde("bb", trace, log_blowup, 31)
+    lib.flat_commit("bb", consts, 8, rp, lde)
+    dt = time.perf_counter() - t0
+    elems = (n << log_blowup) * width
+    return {"value": elems / dt, "unit": "committed field elems/s", "cores": 1, "kind": kind,
+            "sample": f"BabyBear 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
+                      f"{dt:.2f} s on 1 host thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    p3hip = load_p3hip()
+    field, log_n, width, lb, rp = WORKLOADS[args.workload]
+    fid = p3hip.BABYBEAR if field == "bb" else p3hip.GOLDILOCKS
+    n, N = 1 << log_n, 1 << (log_n + lb)
+    shift = 31 if field == "bb" else 7
+    ctx = p3hip.Context(0)
+    ctx.set_poseidon2(fid, p2_constants(field, rp), 8, rp)
+    rng = np.random.default_rng(2024)
+    p = BB_P if field == "bb" else GL_P
+    dtype = np.uint32 if field == "bb" else np.uint64
+    trace = rng.integers(0, p, size=(n, width), dtype=dtype)
+    m = ctx.upload(trace)
+    del trace
+
+    def step():
+        lde = ctx.coset_lde(m, lb, shift)
+        tree = ctx.mmcs_commit([lde], 0)
+        root = tree.root()
+        tree.free()
+        lde.free()
+        return root
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    roots = [step() for _ in range(args.steps)]
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    phases = {ph: ctx.phase_stats(ph) for ph in ("lde", "leaf_hash", "compress")}
+    ctx.set_profiling(False)
+    assert all((r == roots[0]).all() for r in roots), "non-deterministic root"
+
+    ms_per_step = dt * 1e3 / args.steps
+    elems = N * width
+    value = elems / (ms_per_step / 1e3)
+    eb = 4 if field == "bb" else 8
+    rate, dig = (8, 8) if field == "bb" else (4, 4)
+    lde_ms = phases["lde"][1] / max(1, phases["lde"][0])
+    hash_ms = phases["leaf_hash"][1] / max(1, phases["leaf_hash"][0])
+    comp_ms = phases["compress"][1] / max(1, args.steps)
+    # LDE phase: algorithmic bytes = eb * w * (3n + N)
+    lde_bytes = eb * width * (3 * n + N)
+    lde_gbs = lde_bytes / (lde_ms / 1e3) / 1e9
+    # leaf hashing: perms per launch; VALU slots per permutation from the gfx950 ISA
+    perms = N * ((width + rate - 1) // rate)
+    out = {
+        "metric": "committed field elems/s (LDE+Poseidon2 Merkle)",
+        "value": value,
+        "unit": "field elems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32" if field == "bb" else "u64",
+        "data": "synthetic (seeded uniform field elements, seeded Poseidon2 constants)",
+        "config": {"workload": f"{'BabyBear' if field == 'bb' else 'Goldilocks'} trace 2^{log_n} x {width}, "
+                               f"log_blowup {lb}, Poseidon2 W{16 if field == 'bb' else 8} RP{rp}, "
+                               f"single-matrix MMCS root",
+                   "rows": n, "width": width, "log_blowup": lb, "lde_rows": N},
+        "phases_ms": {"lde": lde_ms, "leaf_hash": hash_ms, "compress": comp_ms},
+        "roofline": {"bound": "valu", "kernel": "hash_rows_dense (Poseidon2 leaf sponge)",
+                     "achieved": perms / (hash_ms / 1e3) / 1e9, "unit": "Gperm/s",
+                     "peak": None, "frac": None, "traffic": None},
+        "roofline_lde": {"bound": "hbm", "achieved": lde_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": lde_gbs / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes": lde_bytes},
+        "root": [int(x) for x in roots[0]],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(field, width, lb, rp)
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

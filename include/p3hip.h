@@ -1,0 +1,172 @@
+/*
+ * libp3hip -- MI355X (gfx950) STARK trace-commit hot path behind a C ABI.
+ *
+ * Scope: coset low-degree extension of BabyBear / Goldilocks columns, the
+ * Poseidon2 Merkle commitment over the extended rows (multi-height MMCS and the
+ * single-matrix MMCS used by the PCS and FRI), the FRI commit phase over
+ * BabyBear^4, batched commits of many traces and a serving queue.
+ *
+ * Each entry point replaces a piece of the reference's C++ API
+ * (dloghin/plonky3-gpu, src/include/), cited beside it.  The reference passes
+ * RowMajorMatrix<F> by value and throws std::runtime_error /
+ * std::invalid_argument; here every call returns an int status (P3H_OK = 0)
+ * and p3h_last_error() holds the message, which the C++ adapters in
+ * plonky3-gpu_amd/cpp/p3hip.hpp turn back into the same exception types.
+ *
+ * Data: canonical field elements only -- uint32 in [0, p) for BabyBear
+ * (p = 2^31 - 2^27 + 1), uint64 in [0, p) for Goldilocks (p = 2^64 - 2^32 + 1),
+ * row-major.  BabyBear^4 values are 4 consecutive uint32 coefficients.
+ * Digests: 8 x uint32 (BabyBear, Poseidon2 width 16, rate 8) or 4 x uint64
+ * (Goldilocks, Poseidon2 width 8, rate 4).
+ *
+ * Threading: a context owns one HIP stream and is not thread-safe (like the
+ * reference's NttCuda, ntt_cuda.hpp:301-302); separate contexts may be used
+ * from separate threads.  Handles belong to the context that made them.
+ */
+#ifndef P3HIP_H
+#define P3HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct p3h_ctx p3h_ctx;
+typedef struct p3h_mat p3h_mat;   /* device-resident matrix */
+typedef struct p3h_tree p3h_tree; /* device-resident Merkle tree (prover data) */
+typedef struct p3h_fri p3h_fri;   /* device-resident FRI commit-phase data */
+typedef struct p3h_serve p3h_serve; /* serving queue */
+
+enum { P3H_BABYBEAR = 0, P3H_GOLDILOCKS = 1 };
+
+enum {
+    P3H_OK = 0,
+    P3H_EINVAL = 1,  /* bad argument or shape (reference: std::invalid_argument) */
+    P3H_ENOMEM = 2,  /* device allocation failed */
+    P3H_EDEVICE = 3, /* HIP runtime error (reference: P3_CUDA_CHECK -> runtime_error) */
+    P3H_EBUDGET = 4, /* request exceeds the context's HBM budget */
+    P3H_ESTATE = 5   /* missing setup (e.g. Poseidon2 constants not set) */
+};
+
+/* ---------------------------------------------------------------- context */
+/* hbm_budget_bytes = 0 means "all device memory". */
+int p3h_ctx_create(int device, uint64_t hbm_budget_bytes, p3h_ctx** out);
+int p3h_ctx_destroy(p3h_ctx* ctx);
+const char* p3h_last_error(const p3h_ctx* ctx);
+int p3h_ctx_sync(p3h_ctx* ctx);
+/* bytes held by live handles, the budget, and bytes parked in the allocator cache */
+int p3h_ctx_memory(const p3h_ctx* ctx, uint64_t* in_use, uint64_t* budget, uint64_t* cached);
+/* release cached device blocks back to HIP */
+int p3h_ctx_trim(p3h_ctx* ctx);
+/* per-phase HIP-event timing on the context stream (off by default) */
+int p3h_ctx_set_profiling(p3h_ctx* ctx, int on);
+/* phase: "lde", "leaf_hash", "compress", "fold"; totals since profiling was switched on */
+int p3h_phase_stats(const p3h_ctx* ctx, const char* phase, uint64_t* launches, double* total_ms);
+/* NTT pass shape: log2 columns per LDS tile and max radix-2 stages per pass (0 = default) */
+int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages);
+
+/* ------------------------------------------------------------- Poseidon2 */
+/* Replaces the POD Poseidon2Cuda<F,W,RF,RP,D> (poseidon2_cuda.hpp:37-46) and the
+ * generic Poseidon2 constructor create_poseidon2 (poseidon2.hpp:195-214).
+ * consts: initial[rounds_f/2][width], terminal[rounds_f/2][width], internal[rounds_p],
+ * canonical uint32 (BabyBear, width 16) or uint64 (Goldilocks, width 8).
+ * S-box x^7, linear layers of the reference's generic Poseidon2.
+ * Limits: rounds_f <= 8 (even), rounds_p <= 32. */
+int p3h_poseidon2_set(p3h_ctx* ctx, int field, uint32_t width, uint32_t rounds_f, uint32_t rounds_p,
+                      const void* consts);
+/* Batch permutation of n states in place (host buffers); the reference's
+ * poseidon2_permute_kernel (poseidon2_cuda.hpp:155-166). */
+int p3h_poseidon2_permute(p3h_ctx* ctx, int field, void* states, uint64_t n);
+
+/* -------------------------------------------------------------- matrices */
+int p3h_mat_upload(p3h_ctx* ctx, int field, const void* host, uint64_t height, uint32_t width,
+                   p3h_mat** out);
+int p3h_mat_alloc(p3h_ctx* ctx, int field, uint64_t height, uint32_t width, p3h_mat** out);
+/* copy out in natural row order (RowMajorMatrix layout) */
+int p3h_mat_download(p3h_ctx* ctx, const p3h_mat* m, void* host);
+int p3h_mat_info(const p3h_mat* m, int* field, uint64_t* height, uint32_t* width);
+/* download rows [row0, row0 + nrows) in natural order */
+int p3h_mat_download_rows(p3h_ctx* ctx, const p3h_mat* m, uint64_t row0, uint64_t nrows, void* host);
+int p3h_mat_free(p3h_mat* m);
+
+/* ------------------------------------------------------------------- DFT */
+/* Replaces NttCuda<F> / Radix2Dit<F> (ntt_cuda.hpp:310-378, radix2_dit.hpp:82-233):
+ * kind 0 dft_batch, 1 idft_batch, 2 coset_dft_batch(shift), 3 coset_idft_batch(shift).
+ * Height must be a power of two (log2_strict_usize, p3_util/util.hpp:17-23). */
+int p3h_dft(p3h_ctx* ctx, int kind, const p3h_mat* in, uint64_t shift, p3h_mat** out);
+/* Replaces coset_lde_batch(mat, added_bits, shift) (radix2_dit.hpp:221-233,
+ * ntt_cuda.hpp:369-378).  The result stays on the device; p3h_mat_download
+ * returns it in the reference's natural row order. */
+int p3h_coset_lde(p3h_ctx* ctx, const p3h_mat* in, uint32_t added_bits, uint64_t shift, p3h_mat** out);
+
+/* ------------------------------------------------------------------ MMCS */
+/* Multi-height MMCS commit: p3_merkle_tree::MerkleTreeMmcs::commit
+ * (merkle_tree_mmcs.hpp:100-107, build_merkle_tree merkle_tree.hpp:82-185) and
+ * its GPU twin build_merkle_tree_cuda (merkle_tree_cuda.hpp:358-621).
+ * All heights must be powers of two; the tree keeps references to the
+ * matrices for openings.  cap: 2^cap_height digests. */
+int p3h_mmcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n_mats, uint32_t cap_height,
+                    p3h_tree** out);
+/* Single-matrix MMCS: p3_merkle::MerkleTreeMmcs::commit_matrix
+ * (fri_merkle_tree_mmcs.hpp:55-102): odd layers padded with a zero digest,
+ * commitment = root. */
+int p3h_flat_commit(p3h_ctx* ctx, const p3h_mat* m, p3h_tree** out);
+/* cap (or root) digests; cap_len receives the number of digests */
+int p3h_tree_cap(p3h_ctx* ctx, const p3h_tree* t, void* cap_out, uint32_t* cap_len);
+int p3h_tree_info(const p3h_tree* t, uint32_t* n_layers, uint64_t* n_leaves, uint32_t* total_width);
+/* copy digest layer `layer` (0 = leaves) to the host */
+int p3h_tree_layer(p3h_ctx* ctx, const p3h_tree* t, uint32_t layer, void* out);
+/* Openings for n_idx row indices at once: open_batch (merkle_tree_mmcs.hpp:120-152)
+ * for multi-height trees, open_row (fri_merkle_tree_mmcs.hpp:129-141) for flat
+ * trees.  values_out: per index, the opened rows of every matrix in commit
+ * order, concatenated (total_width elements); proof_out: per index,
+ * (n_layers - 1) sibling digests, leaf level first. */
+int p3h_tree_open(p3h_ctx* ctx, const p3h_tree* t, const uint64_t* indices, uint32_t n_idx,
+                  void* values_out, void* proof_out);
+int p3h_tree_free(p3h_tree* t);
+
+/* ------------------------------------------------------------------- FRI */
+/* fold_matrix_cuda (fri_fold_cuda.hpp:306-348) / TwoAdicFriFolding::fold_matrix
+ * (fri_folding.hpp:200-222): `cur` is a (2^log_height x 4) BabyBear matrix
+ * holding BabyBear^4 values; beta = 4 canonical coefficients; log_arity 1..3. */
+int p3h_fri_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uint32_t* beta,
+                 p3h_mat** out);
+/* prove_fri commit loop (fri_prover.hpp:83-178) for one input codeword of
+ * BabyBear^4 (2^log_len x 4 matrix), with the Fiat-Shamir challenges supplied
+ * by the caller (betas: 4 words per round).  Each round commits the current
+ * vector as a (len/arity) x (4*arity) matrix (ExtensionMmcs,
+ * fri_extension_mmcs.hpp:49-75) and folds.  roots_out: 8 words per round;
+ * final_poly_out: 4 * 2^log_final_poly_len words; keep (optional) receives the
+ * layer trees for later queries. */
+int p3h_fri_commit_phase(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup,
+                         uint32_t log_final_poly_len, uint32_t max_log_arity, const uint32_t* betas,
+                         uint32_t* roots_out, uint32_t* final_poly_out, uint32_t* n_rounds,
+                         p3h_fri** keep);
+/* query the kept FRI layers: layer r tree (borrowed pointer, valid until p3h_fri_free) */
+int p3h_fri_layer_tree(const p3h_fri* f, uint32_t round, const p3h_tree** out);
+int p3h_fri_free(p3h_fri* f);
+
+/* ------------------------------------------------------------------- PCS */
+/* TwoAdicFriPcs::commit (two_adic_fri_pcs.hpp:207-266): LDE every matrix with
+ * log_blowup and its shift, concatenate rows cyclically (row r of matrix m is
+ * LDE row r mod height_m), zero-pad the width to a power of two, and commit
+ * the result with the single-matrix MMCS.  BabyBear only. */
+int p3h_pcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, const uint64_t* shifts, uint32_t n_mats,
+                   uint32_t log_blowup, p3h_tree** out);
+
+/* --------------------------------------------------------------- batched */
+/* Commit many BabyBear traces as one multi-height MMCS: every trace gets its
+ * own LDE (added bits per trace, common shift) and the LDEs are committed
+ * together (multi-height tree, cap_height).  No counterpart in the reference,
+ * which commits one matrix per call; the result equals
+ * p3h_mmcs_commit over the individual p3h_coset_lde outputs. */
+int p3h_batch_commit(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits,
+                     uint32_t n_traces, uint64_t shift, uint32_t cap_height, p3h_tree** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* P3HIP_H */

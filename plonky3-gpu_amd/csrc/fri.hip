@@ -1,0 +1,180 @@
+// FRI commit-phase folding over BabyBear^4 = F_p[a]/(a^4 - 11).
+//
+// Reference: TwoAdicFriFolding::fold_row / fold_matrix (fri_folding.hpp:74-222),
+// GPU twin fri_fold_arity2_kernel (fri_fold_cuda.hpp:80-109).
+// Arity 2: out[i] = (e0 + e1)/2 + beta * (e0 - e1) / (2x) with
+// x = omega_{2^log_h}^bitrev(i, log_h - 1).  The factor 1/(2x) comes from one
+// table in bit-reversed order, tinv[i] = omega_{2^LH}^-bitrev(i, LH-1) / 2,
+// which serves every layer of height <= 2^LH (its first 2^(lh-1) entries are
+// exactly the layer-lh values).  Arity 4 and 8 follow the reference's weight
+// formula literally (see fold_general_kernel).
+
+#include "fri.h"
+
+namespace p3h {
+
+namespace {
+
+using BB = BabyBear;
+
+struct E4 {
+    uint32_t c[4];
+};
+
+__device__ __forceinline__ E4 e4_add(const E4& a, const E4& b) {
+    E4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.c[i] = BB::add(a.c[i], b.c[i]);
+    return r;
+}
+__device__ __forceinline__ E4 e4_sub(const E4& a, const E4& b) {
+    E4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.c[i] = BB::sub(a.c[i], b.c[i]);
+    return r;
+}
+// canonical a times b given in Montgomery form -> canonical a*b
+__device__ __forceinline__ E4 e4_mul_m(const E4& a, const E4& bm) {
+    constexpr uint32_t ELEVEN_M = 0x37FFFFE9u;  // 11 * 2^32 mod p
+    uint32_t c0 = BB::mul_c(a.c[0], bm.c[0]);
+    uint32_t c1 = BB::add(BB::mul_c(a.c[0], bm.c[1]), BB::mul_c(a.c[1], bm.c[0]));
+    uint32_t c2 = BB::add(BB::add(BB::mul_c(a.c[0], bm.c[2]), BB::mul_c(a.c[1], bm.c[1])), BB::mul_c(a.c[2], bm.c[0]));
+    uint32_t c3 = BB::add(BB::add(BB::mul_c(a.c[0], bm.c[3]), BB::mul_c(a.c[1], bm.c[2])),
+                          BB::add(BB::mul_c(a.c[2], bm.c[1]), BB::mul_c(a.c[3], bm.c[0])));
+    uint32_t h0 = BB::add(BB::add(BB::mul_c(a.c[1], bm.c[3]), BB::mul_c(a.c[2], bm.c[2])), BB::mul_c(a.c[3], bm.c[1]));
+    uint32_t h1 = BB::add(BB::mul_c(a.c[2], bm.c[3]), BB::mul_c(a.c[3], bm.c[2]));
+    uint32_t h2 = BB::mul_c(a.c[3], bm.c[3]);
+    E4 r;
+    r.c[0] = BB::add(c0, BB::mul_c(h0, ELEVEN_M));
+    r.c[1] = BB::add(c1, BB::mul_c(h1, ELEVEN_M));
+    r.c[2] = BB::add(c2, BB::mul_c(h2, ELEVEN_M));
+    r.c[3] = c3;
+    return r;
+}
+__device__ __forceinline__ E4 e4_to_m(const E4& a) {
+    E4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.c[i] = BB::to_mont(a.c[i]);
+    return r;
+}
+__device__ __forceinline__ E4 e4_scale_m(const E4& a, uint32_t sm) {
+    E4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r.c[i] = BB::mul_c(a.c[i], sm);
+    return r;
+}
+__device__ __forceinline__ uint32_t bb_pow_m(uint32_t base_m, uint64_t e) {
+    uint32_t acc = BB::R_MOD_P;
+    while (e) {
+        if (e & 1) acc = BB::mont(acc, base_m);
+        base_m = BB::mont(base_m, base_m);
+        e >>= 1;
+    }
+    return acc;
+}
+// canonical a -> canonical a^-1 (a != 0), Fermat
+__device__ __forceinline__ uint32_t bb_inv(uint32_t a) {
+    return BB::from_mont(bb_pow_m(BB::to_mont(a), BB::P - 2));
+}
+// canonical a -> canonical a^-1 in BabyBear^4: the product t of the three
+// Frobenius conjugates satisfies a * t = N(a) in F_p, so a^-1 = t / N(a).
+// Frobenius: phi(a)_i = a_i * z^i with z = 11^((p-1)/4).
+__device__ __forceinline__ E4 e4_inv(const E4& a) {
+    constexpr uint32_t Z1 = 1728404513u;  // 11^((p-1)/4)
+    constexpr uint32_t Z2 = 2013265920u;  // z^2 = -1
+    constexpr uint32_t Z3 = 284861408u;   // z^3
+    // phi^k(a)_i = a_i * z^(i k)
+    const uint32_t zpow[4] = {1u, Z1, Z2, Z3};
+    E4 t = {{1, 0, 0, 0}};
+    for (int k = 1; k < 4; ++k) {
+        E4 c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c.c[i] = BB::mul_c(a.c[i], BB::to_mont(zpow[(i * k) & 3]));
+        t = e4_mul_m(t, e4_to_m(c));
+    }
+    E4 n = e4_mul_m(a, e4_to_m(t));  // lies in F_p: n.c[1..3] == 0
+    uint32_t ninv = bb_inv(n.c[0]);
+    return e4_scale_m(t, BB::to_mont(ninv));
+}
+
+__global__ void fold_arity2_kernel(const uint32_t* in, uint32_t* out, uint64_t n_out, E4 beta_m,
+                                   const uint32_t* tinv_m) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    constexpr uint32_t HALF_M = 0x07FFFFFFu;  // 2^-1 * 2^32 mod p
+    const uint4* src = reinterpret_cast<const uint4*>(in) + 2 * i;
+    uint4 a = src[0], b = src[1];
+    E4 e0 = {{a.x, a.y, a.z, a.w}}, e1 = {{b.x, b.y, b.z, b.w}};
+    E4 s = e4_scale_m(e4_add(e0, e1), HALF_M);
+    E4 d = e4_scale_m(e4_sub(e0, e1), tinv_m[i]);
+    E4 r = e4_add(s, e4_mul_m(d, beta_m));
+    reinterpret_cast<uint4*>(out)[i] = make_uint4(r.c[0], r.c[1], r.c[2], r.c[3]);
+}
+
+__device__ __forceinline__ uint32_t dbitrev(uint32_t x, uint32_t bits) {
+    return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
+}
+
+// General arity, reference weights: t_j = x * w^bitrev(j),
+// L_j = prod_k (beta - t_k) / ((beta - t_j) * x^(a-1) * w^(-j) * a);
+// if beta equals some t_j the row returns e_j.
+__global__ void fold_general_kernel(const uint32_t* in, uint32_t* out, uint64_t n_out, uint32_t log_h,
+                                    uint32_t log_arity, E4 beta, uint32_t omega_h_m, uint32_t w_m,
+                                    uint32_t arity_inv_m) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const uint32_t arity = 1u << log_arity;
+    const uint32_t* ev = in + 4 * uint64_t(i) * arity;
+    uint32_t x_m = bb_pow_m(omega_h_m, dbitrev(uint32_t(i), log_h - log_arity));
+    uint32_t t[8];
+    for (uint32_t j = 0; j < arity; ++j)
+        t[j] = BB::from_mont(BB::mont(x_m, bb_pow_m(w_m, dbitrev(j, log_arity))));
+    for (uint32_t j = 0; j < arity; ++j) {
+        if (beta.c[0] == t[j] && beta.c[1] == 0 && beta.c[2] == 0 && beta.c[3] == 0) {
+            for (int q = 0; q < 4; ++q) out[4 * i + q] = ev[4 * j + q];
+            return;
+        }
+    }
+    E4 full = {{1, 0, 0, 0}};
+    for (uint32_t k = 0; k < arity; ++k) {
+        E4 d = beta;
+        d.c[0] = BB::sub(d.c[0], t[k]);
+        full = e4_mul_m(full, e4_to_m(d));
+    }
+    full = e4_scale_m(full, arity_inv_m);
+    uint32_t x_am1_m = bb_pow_m(x_m, arity - 1);
+    uint32_t w_inv_m = bb_pow_m(w_m, arity - 1);
+    E4 res = {{0, 0, 0, 0}};
+    for (uint32_t j = 0; j < arity; ++j) {
+        uint32_t base_m = BB::mont(x_am1_m, bb_pow_m(w_inv_m, j));
+        E4 d = beta;
+        d.c[0] = BB::sub(d.c[0], t[j]);
+        E4 den = e4_scale_m(d, base_m);
+        E4 lj = e4_mul_m(full, e4_to_m(e4_inv(den)));
+        E4 e = {{ev[4 * j], ev[4 * j + 1], ev[4 * j + 2], ev[4 * j + 3]}};
+        res = e4_add(res, e4_mul_m(e, e4_to_m(lj)));
+    }
+    for (int q = 0; q < 4; ++q) out[4 * i + q] = res.c[q];
+}
+
+}  // namespace
+
+int fold_device(const uint32_t* in, uint32_t* out, uint64_t n_out, uint32_t log_h, uint32_t log_arity,
+                const uint32_t* beta, const uint32_t* tinv_m, uint32_t omega_h_m, uint32_t w_m,
+                uint32_t arity_inv_m, hipStream_t st) {
+    if (n_out == 0) return 0;
+    uint32_t blocks = uint32_t((n_out + 255) / 256);
+    if (log_arity == 1 && tinv_m) {
+        E4 bm;
+        for (int q = 0; q < 4; ++q) bm.c[q] = uint32_t((uint64_t(beta[q]) << 32) % BabyBear::P);
+        hipLaunchKernelGGL(fold_arity2_kernel, dim3(blocks), dim3(256), 0, st, in, out, n_out, bm, tinv_m);
+    } else {
+        E4 b;
+        for (int q = 0; q < 4; ++q) b.c[q] = beta[q];
+        hipLaunchKernelGGL(fold_general_kernel, dim3(blocks), dim3(256), 0, st, in, out, n_out, log_h, log_arity, b,
+                           omega_h_m, w_m, arity_inv_m);
+    }
+    return int(hipGetLastError());
+}
+
+}  // namespace p3h

@@ -1,0 +1,51 @@
+// Host-side launchers for Poseidon2 leaf hashing and tree compression (merkle.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "poseidon2.h"
+
+namespace p3h {
+
+// One matrix taking part in a leaf row.  Logical row j is first wrapped to
+// j & hmask (cyclic repetition of a shorter matrix; hmask = ~0 for none), then
+// mapped to stored row ((j & (2^k - 1)) << L) | (j >> k) of `ptr` (k = 0:
+// natural order; k > 0: a coset-major LDE of 2^k cosets of height 2^L).
+// ptr == nullptr stands for w zero columns.
+template <class T>
+struct RowSeg {
+    const T* ptr;
+    uint64_t hmask;
+    uint32_t w;
+    uint32_t k;
+    uint32_t L;
+    uint32_t pad;
+};
+
+// out[j] = sponge(concatenation over segs of row j), j < rows.
+// segs lives in device memory.
+template <class D>
+int hash_rows_device(const RowSeg<typename D::T>* segs, uint32_t nsegs, uint32_t total_w,
+                     const RowSeg<typename D::T>& seg0_host, uint64_t rows, typename D::T* out,
+                     const P2Params<typename D::T, D::W>& p, hipStream_t st);
+
+// next[i] = compress(prev[2i], prev[2i+1] or zero digest if 2i+1 >= prev_count);
+// when ninj > 0 additionally next[i] = compress(next[i], sponge(rows i of inj)).
+template <class D>
+int compress_layer_device(const typename D::T* prev, uint64_t prev_count, typename D::T* next,
+                          uint64_t next_count, const RowSeg<typename D::T>* inj, uint32_t ninj,
+                          uint32_t inj_w, const P2Params<typename D::T, D::W>& p, hipStream_t st);
+
+// Batch permutation of n states (in place), canonical in and out.
+template <class D>
+int permute_device(typename D::T* states, uint64_t n, const P2Params<typename D::T, D::W>& p,
+                   hipStream_t st);
+
+// Gather opening data: for each of nq query indices, write the sibling digest of
+// each layer (layers given by device pointer table + counts).
+template <class T>
+int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows,
+                       uint32_t nq, T* out, hipStream_t st);
+
+}  // namespace p3h

@@ -1,0 +1,225 @@
+// Poseidon2 Merkle kernels: padding-free sponge leaf hashing over one or more
+// row-concatenated matrices, 2-to-1 truncated-permutation compression with
+// optional injection of shorter matrices, batch permutation and row gathers
+// for openings.
+//
+// Sponge (reference: PaddingFreeSponge::hash_iter, padding_free_sponge.hpp:63-87):
+// zero state; every chunk of up to RATE inputs overwrites the first slots;
+// a permutation follows every non-empty chunk; digest = first DIG lanes.
+// Compression (TruncatedPermutation::compress, truncated_permutation.hpp:61-76):
+// state = [left | right | 0...], permute, keep the first DIG lanes.
+// Injection (build_merkle_tree, merkle_tree.hpp:163-177): node =
+// compress(node, sponge(rows of the matrices whose height is this level)).
+
+#include "merkle.h"
+
+namespace p3h {
+
+namespace {
+
+constexpr int HASH_THREADS = 256;
+
+template <class T>
+__device__ __forceinline__ uint64_t stored_row(const RowSeg<T>& s, uint64_t j) {
+    j &= s.hmask;
+    if (s.k == 0) return j;
+    return ((j & ((1ull << s.k) - 1)) << s.L) | (j >> s.k);
+}
+
+// Absorb the concatenated row j of `segs` into state s (domain form).
+template <class D>
+__device__ __forceinline__ void sponge_row(typename D::T* s, const RowSeg<typename D::T>* segs, uint32_t nsegs,
+                                           uint32_t total_w, uint64_t j, const P2Params<typename D::T, D::W>& p) {
+    using T = typename D::T;
+#pragma unroll
+    for (int i = 0; i < D::W; ++i) s[i] = 0;
+    uint32_t seg = 0, off = 0;
+    RowSeg<T> cur = segs[0];
+    const T* rowp = cur.ptr + stored_row(cur, j) * cur.w;
+    for (uint32_t start = 0; start < total_w; start += D::RATE) {
+#pragma unroll
+        for (int q = 0; q < D::RATE; ++q) {
+            if (start + q < total_w) {
+                while (off >= cur.w) {
+                    off = 0;
+                    ++seg;
+                    cur = segs[seg];
+                    rowp = cur.ptr + stored_row(cur, j) * cur.w;
+                }
+                s[q] = cur.ptr ? D::to_dom(rowp[off]) : T(0);
+                ++off;
+            }
+        }
+        Poseidon2<D>::permute(s, p);
+    }
+    (void)nsegs;
+}
+
+template <class D>
+__global__ __launch_bounds__(HASH_THREADS) void hash_rows_kernel(const RowSeg<typename D::T>* segs,
+                                                                 uint32_t nsegs, uint32_t total_w,
+                                                                 uint64_t rows, typename D::T* out,
+                                                                 P2Params<typename D::T, D::W> p) {
+    using T = typename D::T;
+    uint64_t j = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (j >= rows) return;
+    T s[D::W];
+    sponge_row<D>(s, segs, nsegs, total_w, j, p);
+#pragma unroll
+    for (int i = 0; i < D::DIG; ++i) out[j * D::DIG + i] = D::from_dom(s[i]);
+}
+
+// Single matrix whose width is a multiple of RATE: each chunk is RATE
+// contiguous elements, fetched with 16-byte loads.
+template <class D>
+__global__ __launch_bounds__(HASH_THREADS) void hash_rows_dense_kernel(RowSeg<typename D::T> seg, uint64_t rows,
+                                                                       typename D::T* out,
+                                                                       P2Params<typename D::T, D::W> p) {
+    using T = typename D::T;
+    constexpr int PER16 = 16 / sizeof(T);
+    uint64_t j = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (j >= rows) return;
+    T s[D::W];
+#pragma unroll
+    for (int i = 0; i < D::W; ++i) s[i] = 0;
+    const T* rowp = seg.ptr + stored_row(seg, j) * seg.w;
+    for (uint32_t start = 0; start < seg.w; start += D::RATE) {
+        const uint4* v = reinterpret_cast<const uint4*>(rowp + start);
+#pragma unroll
+        for (int q = 0; q < D::RATE / PER16; ++q) {
+            uint4 x = v[q];
+            T tmp[PER16];
+            __builtin_memcpy(tmp, &x, 16);
+#pragma unroll
+            for (int e = 0; e < PER16; ++e) s[q * PER16 + e] = D::to_dom(tmp[e]);
+        }
+        Poseidon2<D>::permute(s, p);
+    }
+#pragma unroll
+    for (int i = 0; i < D::DIG; ++i) out[j * D::DIG + i] = D::from_dom(s[i]);
+}
+
+template <class D>
+__global__ __launch_bounds__(HASH_THREADS) void compress_kernel(const typename D::T* prev, uint64_t prev_count,
+                                                                typename D::T* next, uint64_t next_count,
+                                                                const RowSeg<typename D::T>* inj, uint32_t ninj,
+                                                                uint32_t inj_w, P2Params<typename D::T, D::W> p) {
+    using T = typename D::T;
+    uint64_t i = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (i >= next_count) return;
+    T s[D::W];
+#pragma unroll
+    for (int q = 0; q < D::W; ++q) s[q] = 0;
+    const T* l = prev + 2 * i * D::DIG;
+#pragma unroll
+    for (int q = 0; q < D::DIG; ++q) s[q] = D::to_dom(l[q]);
+    if (2 * i + 1 < prev_count) {
+#pragma unroll
+        for (int q = 0; q < D::DIG; ++q) s[D::DIG + q] = D::to_dom(l[D::DIG + q]);
+    }
+    Poseidon2<D>::permute(s, p);
+    if (ninj) {
+        T h[D::W];
+        sponge_row<D>(h, inj, ninj, inj_w, i, p);
+#pragma unroll
+        for (int q = 0; q < D::DIG; ++q) s[D::DIG + q] = h[q];
+#pragma unroll
+        for (int q = 2 * D::DIG; q < D::W; ++q) s[q] = 0;
+        Poseidon2<D>::permute(s, p);
+    }
+#pragma unroll
+    for (int q = 0; q < D::DIG; ++q) next[i * D::DIG + q] = D::from_dom(s[q]);
+}
+
+template <class D>
+__global__ __launch_bounds__(HASH_THREADS) void permute_kernel(typename D::T* st, uint64_t n,
+                                                               P2Params<typename D::T, D::W> p) {
+    using T = typename D::T;
+    uint64_t i = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (i >= n) return;
+    T s[D::W];
+#pragma unroll
+    for (int q = 0; q < D::W; ++q) s[q] = D::to_dom(st[i * D::W + q]);
+    Poseidon2<D>::permute(s, p);
+#pragma unroll
+    for (int q = 0; q < D::W; ++q) st[i * D::W + q] = D::from_dom(s[q]);
+}
+
+// out[q * total_w + c] = concatenated row rows[q], column c
+template <class T>
+__global__ void gather_rows_kernel(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows,
+                                   uint32_t nq, T* out) {
+    uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= uint64_t(nq) * total_w) return;
+    uint32_t q = uint32_t(idx / total_w), c = uint32_t(idx % total_w);
+    uint32_t seg = 0;
+    while (seg + 1 < nsegs && c >= segs[seg].w) {
+        c -= segs[seg].w;
+        ++seg;
+    }
+    const RowSeg<T>& s = segs[seg];
+    out[idx] = s.ptr ? s.ptr[stored_row(s, rows[q]) * s.w + c] : T(0);
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t threads) { return uint32_t((n + threads - 1) / threads); }
+
+}  // namespace
+
+template <class D>
+int hash_rows_device(const RowSeg<typename D::T>* segs, uint32_t nsegs, uint32_t total_w,
+                     const RowSeg<typename D::T>& seg0, uint64_t rows, typename D::T* out,
+                     const P2Params<typename D::T, D::W>& p, hipStream_t st) {
+    if (rows == 0) return 0;
+    bool dense = nsegs == 1 && seg0.ptr && seg0.w > 0 && seg0.w % D::RATE == 0 &&
+                 (reinterpret_cast<uintptr_t>(seg0.ptr) & 15) == 0 && (seg0.w * sizeof(typename D::T)) % 16 == 0;
+    if (dense) {
+        hipLaunchKernelGGL((hash_rows_dense_kernel<D>), dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0,
+                           st, seg0, rows, out, p);
+    } else {
+        hipLaunchKernelGGL((hash_rows_kernel<D>), dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st,
+                           segs, nsegs, total_w, rows, out, p);
+    }
+    return int(hipGetLastError());
+}
+
+template <class D>
+int compress_layer_device(const typename D::T* prev, uint64_t prev_count, typename D::T* next, uint64_t next_count,
+                          const RowSeg<typename D::T>* inj, uint32_t ninj, uint32_t inj_w,
+                          const P2Params<typename D::T, D::W>& p, hipStream_t st) {
+    if (next_count == 0) return 0;
+    hipLaunchKernelGGL((compress_kernel<D>), dim3(grid_for(next_count, HASH_THREADS)), dim3(HASH_THREADS), 0, st,
+                       prev, prev_count, next, next_count, inj, ninj, inj_w, p);
+    return int(hipGetLastError());
+}
+
+template <class D>
+int permute_device(typename D::T* states, uint64_t n, const P2Params<typename D::T, D::W>& p, hipStream_t st) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL((permute_kernel<D>), dim3(grid_for(n, HASH_THREADS)), dim3(HASH_THREADS), 0, st, states, n,
+                       p);
+    return int(hipGetLastError());
+}
+
+template <class T>
+int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows, uint32_t nq,
+                       T* out, hipStream_t st) {
+    uint64_t total = uint64_t(nq) * total_w;
+    if (total == 0) return 0;
+    hipLaunchKernelGGL((gather_rows_kernel<T>), dim3(grid_for(total, 256)), dim3(256), 0, st, segs, nsegs, total_w,
+                       rows, nq, out);
+    return int(hipGetLastError());
+}
+
+#define P3H_INST(D)                                                                                             \
+    template int hash_rows_device<D>(const RowSeg<D::T>*, uint32_t, uint32_t, const RowSeg<D::T>&, uint64_t,    \
+                                     D::T*, const P2Params<D::T, D::W>&, hipStream_t);                          \
+    template int compress_layer_device<D>(const D::T*, uint64_t, D::T*, uint64_t, const RowSeg<D::T>*, uint32_t, \
+                                          uint32_t, const P2Params<D::T, D::W>&, hipStream_t);                  \
+    template int permute_device<D>(D::T*, uint64_t, const P2Params<D::T, D::W>&, hipStream_t);                  \
+    template int gather_rows_device<D::T>(const RowSeg<D::T>*, uint32_t, uint32_t, const uint64_t*, uint32_t,   \
+                                          D::T*, hipStream_t);
+P3H_INST(BBDom)
+P3H_INST(GLDom)
+#undef P3H_INST
+
+}  // namespace p3h

@@ -1,0 +1,48 @@
+// Host-side launchers for the NTT / LDE kernels (ntt.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "field.h"
+
+namespace p3h {
+
+// Pass-shape knobs picked by the autotuner (0 = built-in default).
+struct NttTuning {
+    uint32_t log_cols;    // log2 columns per tile (default 4)
+    uint32_t max_stages;  // cap on radix-2 stages per LDS pass (default: tile-limited)
+};
+
+// Coset LDE, coset-major output: out[(r * n + i) * w + c] = LDE row (i << k | r).
+// tw_inv_fwd: [omega_n^-e for e < n/2][omega_n^e for e < n/2] (multiply-constant form);
+// twist: per coset r and bit-reversed position p, (shift * omega_N^r)^bitrev(p) / n.
+// `in` is read only; out has (n << k) rows.
+template <class F>
+int lde_device(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t w, uint32_t k,
+               const typename F::T* tw_inv_fwd, const typename F::T* twist, const NttTuning& tune,
+               hipStream_t st);
+
+// kind 0 dft, 1 idft, 2 coset_dft, 3 coset_idft; natural order in and out.
+// row_scale: forward kinds -> multiplier of input row i (shift^i) or nullptr;
+// inverse kinds -> multiplier of output row j (n^-1 * shift^-j), required.
+// tmp: n x w scratch used by inverse kinds when more than one pass is needed.
+template <class F>
+int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F::T* tmp, uint32_t L,
+               uint32_t w, const typename F::T* tw_inv_fwd, const typename F::T* row_scale,
+               const NttTuning& tune, hipStream_t st);
+
+// out[i] = base^i * mult for i < count, all in multiply-constant form.
+template <class F>
+int powers_device(typename F::T* out, uint64_t count, typename F::T base_c, typename F::T mult_c,
+                  typename F::T one_c, hipStream_t st);
+
+// out[r * n + p] = coset_base[r]^bitrev(p) * inv_n (multiply-constant form).
+template <class F>
+int lde_twist_device(typename F::T* out, uint32_t L, uint32_t n_cosets, const typename F::T* coset_base_c,
+                     typename F::T inv_n_c, typename F::T one_c, hipStream_t st);
+
+template <class T>
+int coset_to_natural_device(const T* in, T* out, uint32_t L, uint32_t k, uint32_t w, hipStream_t st);
+
+}  // namespace p3h

@@ -1,0 +1,1198 @@
+// libp3hip runtime: contexts, the caching device allocator with an HBM
+// budget, device-resident matrices and trees, table caches, and every C-ABI
+// entry point of include/p3hip.h.  All device work of a context is ordered on
+// its one HIP stream; host copies synchronise that stream.
+
+#include "p3hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fri.h"
+#include "merkle.h"
+#include "ntt.h"
+
+using namespace p3h;
+
+namespace {
+
+// ------------------------------------------------------------- host math ---
+constexpr uint32_t BB_P = BabyBear::P;
+constexpr uint64_t GL_P = Goldilocks::P;
+
+uint32_t bb_mul(uint32_t a, uint32_t b) { return uint32_t(uint64_t(a) * b % BB_P); }
+uint32_t bb_pow(uint32_t a, uint64_t e) {
+    uint32_t r = 1;
+    while (e) {
+        if (e & 1) r = bb_mul(r, a);
+        a = bb_mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+uint32_t bb_inv(uint32_t a) { return bb_pow(a, BB_P - 2); }
+uint32_t bb_mont(uint32_t a) { return uint32_t((uint64_t(a) << 32) % BB_P); }  // a * 2^32 mod p
+uint32_t bb_root(uint32_t bits) { return bb_pow(440564289u, uint64_t(1) << (27 - bits)); }
+
+uint64_t gl_mul(uint64_t a, uint64_t b) { return uint64_t((unsigned __int128)a * b % GL_P); }
+uint64_t gl_pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = gl_mul(r, a);
+        a = gl_mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+uint64_t gl_inv(uint64_t a) { return gl_pow(a, GL_P - 2); }
+// Goldilocks two-adic structure (not defined by the reference): generator 7,
+// omega_{2^32} = 7^((p-1)/2^32).
+uint64_t gl_root(uint32_t bits) { return gl_pow(gl_pow(7, (GL_P - 1) >> 32), uint64_t(1) << (32 - bits)); }
+
+// Per-field constant helpers: "c" = multiply-constant form used by kernels.
+struct HBB {
+    using T = uint32_t;
+    using F = BabyBear;
+    using D = BBDom;
+    static constexpr int FIELD = P3H_BABYBEAR;
+    static constexpr uint32_t MAX_BITS = 27;
+    static T c(T x) { return bb_mont(x); }
+    static T mul(T a, T b) { return bb_mul(a, b); }
+    static T pow(T a, uint64_t e) { return bb_pow(a, e); }
+    static T inv(T a) { return bb_inv(a); }
+    static T root(uint32_t bits) { return bb_root(bits); }
+    static bool canonical(uint64_t x) { return x < BB_P; }
+};
+struct HGL {
+    using T = uint64_t;
+    using F = Goldilocks;
+    using D = GLDom;
+    static constexpr int FIELD = P3H_GOLDILOCKS;
+    static constexpr uint32_t MAX_BITS = 32;
+    static T c(T x) { return x; }
+    static T mul(T a, T b) { return gl_mul(a, b); }
+    static T pow(T a, uint64_t e) { return gl_pow(a, e); }
+    static T inv(T a) { return gl_inv(a); }
+    static T root(uint32_t bits) { return gl_root(bits); }
+    static bool canonical(uint64_t x) { return x < GL_P; }
+};
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+size_t elem_size(int field) { return field == P3H_GOLDILOCKS ? 8 : 4; }
+uint32_t log2_exact(uint64_t n) {
+    if (n == 0 || (n & (n - 1))) throw Err{P3H_EINVAL, "height must be a non-zero power of two"};
+    return uint32_t(__builtin_ctzll(n));
+}
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Err{P3H_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+void launch_check(int e, const char* what) { hip_check(hipError_t(e), what); }
+
+}  // namespace
+
+// ======================================================= context & memory ===
+
+struct DevBlock {
+    p3h_ctx* ctx = nullptr;
+    void* ptr = nullptr;
+    size_t cls = 0;
+    DevBlock() = default;
+    DevBlock(const DevBlock&) = delete;
+    DevBlock& operator=(const DevBlock&) = delete;
+    ~DevBlock();
+};
+using DevPtr = std::shared_ptr<DevBlock>;
+
+struct PhaseStat {
+    uint64_t launches = 0;
+    double ms = 0;
+};
+
+struct p3h_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, in_use = 0, cached = 0;
+    std::multimap<size_t, void*> free_blocks;
+    std::string err;
+    bool p2_set[2] = {false, false};
+    P2Params<uint32_t, 16> bbp{};
+    P2Params<uint64_t, 8> glp{};
+    std::map<std::tuple<int, uint32_t>, DevPtr> twiddles;                      // (field, L)
+    std::map<std::tuple<int, uint32_t, uint32_t, uint64_t>, DevPtr> twists;    // (field, L, k, shift)
+    std::map<std::tuple<int, uint32_t, int, uint64_t>, DevPtr> row_scales;     // (field, L, kind, shift)
+    std::map<uint32_t, DevPtr> fri_tinv;                                        // LH
+    NttTuning tune{};
+    bool profiling = false;
+    std::map<std::string, PhaseStat> stats;
+    struct Pending {
+        std::string phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+};
+
+DevBlock::~DevBlock() {
+    if (ctx && ptr) {
+        ctx->in_use -= cls;
+        ctx->cached += cls;
+        ctx->free_blocks.emplace(cls, ptr);
+    }
+}
+
+namespace {
+
+size_t size_class(size_t bytes) {
+    if (bytes == 0) bytes = 1;
+    if (bytes <= (size_t(2) << 20)) {
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    size_t g = size_t(2) << 20;
+    return (bytes + g - 1) / g * g;
+}
+
+void trim(p3h_ctx* ctx) {
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->free_blocks) hipFree(kv.second);
+    ctx->free_blocks.clear();
+    ctx->cached = 0;
+}
+
+DevPtr dalloc(p3h_ctx* ctx, size_t bytes) {
+    size_t cls = size_class(bytes);
+    if (ctx->in_use + cls > ctx->budget)
+        throw Err{P3H_EBUDGET, "request of " + std::to_string(cls) + " bytes exceeds the HBM budget (" +
+                                   std::to_string(ctx->in_use) + " of " + std::to_string(ctx->budget) + " in use)"};
+    auto blk = std::make_shared<DevBlock>();
+    auto it = ctx->free_blocks.find(cls);
+    if (it != ctx->free_blocks.end()) {
+        blk->ptr = it->second;
+        ctx->free_blocks.erase(it);
+        ctx->cached -= cls;
+    } else {
+        void* p = nullptr;
+        if (hipMalloc(&p, cls) != hipSuccess) {
+            (void)hipGetLastError();
+            trim(ctx);
+            if (hipMalloc(&p, cls) != hipSuccess) {
+                (void)hipGetLastError();
+                throw Err{P3H_ENOMEM, "hipMalloc of " + std::to_string(cls) + " bytes failed"};
+            }
+        }
+        blk->ptr = p;
+    }
+    blk->ctx = ctx;
+    blk->cls = cls;
+    ctx->in_use += cls;
+    return blk;
+}
+
+template <class T>
+T* dptr(const DevPtr& p) {
+    return static_cast<T*>(p->ptr);
+}
+
+void h2d(p3h_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync H2D");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+void d2h(p3h_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+// RAII phase timer: HIP events around the phase's launches on the context stream.
+struct Phase {
+    p3h_ctx* ctx;
+    std::string name;
+    hipEvent_t a = nullptr;
+    Phase(p3h_ctx* c, const char* n) : ctx(c), name(n) {
+        if (ctx->profiling) {
+            hipEventCreate(&a);
+            hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~Phase() {
+        if (ctx->profiling) {
+            hipEvent_t b;
+            hipEventCreate(&b);
+            hipEventRecord(b, ctx->stream);
+            ctx->pending.push_back({name, a, b});
+        }
+    }
+};
+
+void resolve_pending(p3h_ctx* ctx) {
+    for (auto& p : ctx->pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        auto& s = ctx->stats[p.phase];
+        s.launches++;
+        s.ms += ms;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    ctx->pending.clear();
+}
+
+}  // namespace
+
+// ============================================================= handles ===
+
+struct p3h_mat {
+    p3h_ctx* ctx;
+    int field;
+    uint64_t height;
+    uint32_t width;
+    DevPtr buf;
+    uint64_t offset_elems = 0;  // view offset into buf
+    uint32_t k = 0;             // coset-major layout: 2^k cosets of height 2^L
+    uint32_t L = 0;
+    std::atomic<int> refs{1};
+    template <class T>
+    T* data() const {
+        return static_cast<T*>(buf->ptr) + offset_elems;
+    }
+};
+
+struct p3h_tree {
+    p3h_ctx* ctx;
+    int field;
+    bool flat;
+    uint32_t cap_height = 0;
+    std::vector<DevPtr> layers;
+    std::vector<uint64_t> counts;
+    std::vector<p3h_mat*> mats;  // commit order, referenced
+    std::vector<uint64_t> heights;
+    std::vector<uint32_t> widths;
+    std::vector<DevPtr> seg_tables;  // keep device segment tables alive
+    // flat trees built over a virtual concatenation (PCS): the segments
+    std::vector<char> flat_segs_host;  // RowSeg<T> array (host copy)
+    uint32_t total_width = 0;
+};
+
+struct p3h_fri {
+    p3h_ctx* ctx;
+    std::vector<p3h_tree*> trees;
+    std::vector<p3h_mat*> layers;
+};
+
+namespace {
+
+void mat_ref(p3h_mat* m) { m->refs.fetch_add(1); }
+void mat_unref(p3h_mat* m) {
+    if (m && m->refs.fetch_sub(1) == 1) delete m;
+}
+
+p3h_mat* new_mat(p3h_ctx* ctx, int field, uint64_t h, uint32_t w) {
+    auto* m = new p3h_mat{};
+    m->ctx = ctx;
+    m->field = field;
+    m->height = h;
+    m->width = w;
+    try {
+        m->buf = dalloc(ctx, std::max<size_t>(1, h * w * elem_size(field)));
+    } catch (...) {
+        delete m;
+        throw;
+    }
+    return m;
+}
+
+template <class H>
+RowSeg<typename H::T> seg_of(const p3h_mat* m) {
+    RowSeg<typename H::T> s{};
+    s.ptr = m->data<typename H::T>();
+    s.hmask = ~0ull;
+    s.w = m->width;
+    s.k = m->k;
+    s.L = m->L;
+    return s;
+}
+
+template <class H>
+const P2Params<typename H::T, H::D::W>& p2(p3h_ctx* ctx);
+template <>
+const P2Params<uint32_t, 16>& p2<HBB>(p3h_ctx* ctx) {
+    if (!ctx->p2_set[P3H_BABYBEAR]) throw Err{P3H_ESTATE, "Poseidon2 constants for BabyBear not set"};
+    return ctx->bbp;
+}
+template <>
+const P2Params<uint64_t, 8>& p2<HGL>(p3h_ctx* ctx) {
+    if (!ctx->p2_set[P3H_GOLDILOCKS]) throw Err{P3H_ESTATE, "Poseidon2 constants for Goldilocks not set"};
+    return ctx->glp;
+}
+
+// [omega^-e][omega^e], e < n/2, multiply-constant form
+template <class H>
+const typename H::T* twiddles(p3h_ctx* ctx, uint32_t L) {
+    using T = typename H::T;
+    auto key = std::make_tuple(H::FIELD, L);
+    auto it = ctx->twiddles.find(key);
+    if (it != ctx->twiddles.end()) return dptr<T>(it->second);
+    uint64_t half = L ? (1ull << (L - 1)) : 0;
+    DevPtr buf = dalloc(ctx, std::max<uint64_t>(1, 2 * half) * sizeof(T));
+    if (half) {
+        T w = H::root(L);
+        launch_check(powers_device<typename H::F>(dptr<T>(buf), half, H::c(H::inv(w)), H::c(1), H::c(1), ctx->stream),
+                     "twiddles");
+        launch_check(powers_device<typename H::F>(dptr<T>(buf) + half, half, H::c(w), H::c(1), H::c(1), ctx->stream),
+                     "twiddles");
+    }
+    ctx->twiddles[key] = buf;
+    return dptr<T>(buf);
+}
+
+template <class H>
+const typename H::T* lde_twist(p3h_ctx* ctx, uint32_t L, uint32_t k, typename H::T shift) {
+    using T = typename H::T;
+    auto key = std::make_tuple(H::FIELD, L, k, uint64_t(shift));
+    auto it = ctx->twists.find(key);
+    if (it != ctx->twists.end()) return dptr<T>(it->second);
+    uint32_t nc = 1u << k;
+    std::vector<T> bases(nc);
+    T wN = H::root(L + k), cur = shift;
+    for (uint32_t r = 0; r < nc; ++r) {
+        bases[r] = H::c(cur);
+        cur = H::mul(cur, wN);
+    }
+    DevPtr bb = dalloc(ctx, nc * sizeof(T));
+    h2d(ctx, bb->ptr, bases.data(), nc * sizeof(T));
+    DevPtr buf = dalloc(ctx, (uint64_t(nc) << L) * sizeof(T));
+    T inv_n = H::inv(H::pow(2, L));
+    launch_check(lde_twist_device<typename H::F>(dptr<T>(buf), L, nc, dptr<T>(bb), H::c(inv_n), H::c(1), ctx->stream),
+                 "lde twist");
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    ctx->twists[key] = buf;
+    return dptr<T>(buf);
+}
+
+// kind 2: shift^i; kind 1: 1/n; kind 3: shift^-i / n
+template <class H>
+const typename H::T* row_scale(p3h_ctx* ctx, uint32_t L, int kind, typename H::T shift) {
+    using T = typename H::T;
+    if (kind == 0) return nullptr;
+    auto key = std::make_tuple(H::FIELD, L, kind, uint64_t(shift));
+    auto it = ctx->row_scales.find(key);
+    if (it != ctx->row_scales.end()) return dptr<T>(it->second);
+    uint64_t n = 1ull << L;
+    DevPtr buf = dalloc(ctx, n * sizeof(T));
+    T inv_n = H::inv(H::pow(2, L));
+    T base = 1, mult = 1;
+    if (kind == 2) base = shift;
+    if (kind == 1) mult = inv_n;
+    if (kind == 3) {
+        base = H::inv(shift);
+        mult = inv_n;
+    }
+    launch_check(powers_device<typename H::F>(dptr<T>(buf), n, H::c(base), H::c(mult), H::c(1), ctx->stream),
+                 "row scale");
+    ctx->row_scales[key] = buf;
+    return dptr<T>(buf);
+}
+
+// natural-order device copy of m (m itself when already natural)
+template <class H>
+const typename H::T* natural_data(p3h_ctx* ctx, const p3h_mat* m, DevPtr& tmp) {
+    using T = typename H::T;
+    if (m->k == 0) return m->data<T>();
+    tmp = dalloc(ctx, m->height * m->width * sizeof(T));
+    launch_check(coset_to_natural_device<T>(m->data<T>(), dptr<T>(tmp), m->L, m->k, m->width, ctx->stream),
+                 "coset_to_natural");
+    return dptr<T>(tmp);
+}
+
+// ------------------------------------------------------------------ LDE ---
+template <class H>
+p3h_mat* do_lde(p3h_ctx* ctx, const p3h_mat* in, uint32_t k, uint64_t shift64) {
+    using T = typename H::T;
+    if (!H::canonical(shift64)) throw Err{P3H_EINVAL, "shift is not a canonical field element"};
+    uint32_t L = log2_exact(in->height);
+    if (L + k > H::MAX_BITS) throw Err{P3H_EINVAL, "LDE height exceeds the field's two-adicity"};
+    if (in->width == 0) throw Err{P3H_EINVAL, "matrix width is zero"};
+    DevPtr tmp;
+    const T* src = natural_data<H>(ctx, in, tmp);
+    const T* tw = twiddles<H>(ctx, L);
+    const T* twist = lde_twist<H>(ctx, L, k, T(shift64));
+    p3h_mat* out = new_mat(ctx, H::FIELD, in->height << k, in->width);
+    out->k = k;
+    out->L = L;
+    try {
+        Phase ph(ctx, "lde");
+        launch_check(lde_device<typename H::F>(src, out->data<T>(), L, in->width, k, tw, twist, ctx->tune, ctx->stream),
+                     "lde");
+    } catch (...) {
+        mat_unref(out);
+        throw;
+    }
+    if (k == 0) out->k = 0;
+    return out;
+}
+
+template <class H>
+p3h_mat* do_dft(p3h_ctx* ctx, int kind, const p3h_mat* in, uint64_t shift64) {
+    using T = typename H::T;
+    if (kind < 0 || kind > 3) throw Err{P3H_EINVAL, "dft kind must be 0..3"};
+    if (!H::canonical(shift64)) throw Err{P3H_EINVAL, "shift is not a canonical field element"};
+    if ((kind == 2 || kind == 3) && shift64 == 0) throw Err{P3H_EINVAL, "coset shift must be non-zero"};
+    uint32_t L = log2_exact(in->height);
+    if (L > H::MAX_BITS) throw Err{P3H_EINVAL, "height exceeds the field's two-adicity"};
+    if (in->width == 0) throw Err{P3H_EINVAL, "matrix width is zero"};
+    DevPtr tmpn;
+    const T* src = natural_data<H>(ctx, in, tmpn);
+    const T* tw = twiddles<H>(ctx, L);
+    const T* sc = row_scale<H>(ctx, L, kind, T(shift64));
+    p3h_mat* out = new_mat(ctx, H::FIELD, in->height, in->width);
+    try {
+        DevPtr tmp;
+        if (kind == 1 || kind == 3) tmp = dalloc(ctx, in->height * in->width * sizeof(T));
+        Phase ph(ctx, "dft");
+        launch_check(dft_device<typename H::F>(kind, src, out->data<T>(), tmp ? dptr<T>(tmp) : nullptr, L, in->width,
+                                               tw, sc, ctx->tune, ctx->stream),
+                     "dft");
+        if (tmp) hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    } catch (...) {
+        mat_unref(out);
+        throw;
+    }
+    return out;
+}
+
+// ----------------------------------------------------------------- trees ---
+template <class H>
+DevPtr upload_segs(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs) {
+    DevPtr d = dalloc(ctx, segs.size() * sizeof(RowSeg<typename H::T>));
+    h2d(ctx, d->ptr, segs.data(), segs.size() * sizeof(RowSeg<typename H::T>));
+    return d;
+}
+
+template <class H>
+p3h_tree* do_mmcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uint32_t cap_height) {
+    using T = typename H::T;
+    using D = typename H::D;
+    const auto& pp = p2<H>(ctx);
+    if (n == 0) throw Err{P3H_EINVAL, "no matrices to commit"};
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!mats[i] || mats[i]->field != H::FIELD) throw Err{P3H_EINVAL, "matrix field mismatch"};
+        log2_exact(mats[i]->height);
+    }
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return mats[a]->height > mats[b]->height; });
+    const uint64_t maxh = mats[order[0]]->height;
+    auto* t = new p3h_tree{};
+    t->ctx = ctx;
+    t->field = H::FIELD;
+    t->flat = false;
+    t->cap_height = cap_height;
+    try {
+        for (uint32_t i = 0; i < n; ++i) {
+            t->mats.push_back(const_cast<p3h_mat*>(mats[i]));
+            mat_ref(const_cast<p3h_mat*>(mats[i]));
+            t->heights.push_back(mats[i]->height);
+            t->widths.push_back(mats[i]->width);
+            t->total_width += mats[i]->width;
+        }
+        size_t next = 0;
+        auto take = [&](uint64_t h, uint32_t& w) {
+            std::vector<RowSeg<T>> segs;
+            w = 0;
+            while (next < n && mats[order[next]]->height == h) {
+                segs.push_back(seg_of<H>(mats[order[next]]));
+                w += mats[order[next]]->width;
+                ++next;
+            }
+            return segs;
+        };
+        uint32_t w0 = 0;
+        auto top = take(maxh, w0);
+        DevPtr dsegs = upload_segs<H>(ctx, top);
+        t->seg_tables.push_back(dsegs);
+        DevPtr leaves = dalloc(ctx, maxh * D::DIG * sizeof(T));
+        {
+            Phase ph(ctx, "leaf_hash");
+            launch_check(hash_rows_device<D>(dptr<RowSeg<T>>(dsegs), uint32_t(top.size()), w0, top[0], maxh,
+                                             dptr<T>(leaves), pp, ctx->stream),
+                         "leaf hash");
+        }
+        t->layers.push_back(leaves);
+        t->counts.push_back(maxh);
+        uint64_t level = maxh, cap = 1ull << cap_height;
+        while (level > cap) {
+            level >>= 1;
+            DevPtr nl = dalloc(ctx, level * D::DIG * sizeof(T));
+            uint32_t wi = 0;
+            auto inj = take(level, wi);
+            DevPtr di;
+            if (!inj.empty()) {
+                di = upload_segs<H>(ctx, inj);
+                t->seg_tables.push_back(di);
+            }
+            Phase ph(ctx, "compress");
+            launch_check(compress_layer_device<D>(dptr<T>(t->layers.back()), level * 2, dptr<T>(nl), level,
+                                                  di ? dptr<RowSeg<T>>(di) : nullptr, uint32_t(inj.size()), wi, pp,
+                                                  ctx->stream),
+                         "compress");
+            t->layers.push_back(nl);
+            t->counts.push_back(level);
+        }
+    } catch (...) {
+        p3h_tree_free(t);
+        throw;
+    }
+    return t;
+}
+
+// flat tree over one (possibly virtual) matrix given by segments
+template <class H>
+p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs, uint64_t height, uint32_t width,
+                    const std::vector<p3h_mat*>& refs) {
+    using T = typename H::T;
+    using D = typename H::D;
+    const auto& pp = p2<H>(ctx);
+    if (height == 0 || width == 0) throw Err{P3H_EINVAL, "commit_matrix: empty matrix"};
+    auto* t = new p3h_tree{};
+    t->ctx = ctx;
+    t->field = H::FIELD;
+    t->flat = true;
+    t->total_width = width;
+    try {
+        for (auto* m : refs) {
+            t->mats.push_back(m);
+            mat_ref(m);
+        }
+        t->heights.push_back(height);
+        t->widths.push_back(width);
+        t->flat_segs_host.resize(segs.size() * sizeof(RowSeg<T>));
+        std::memcpy(t->flat_segs_host.data(), segs.data(), t->flat_segs_host.size());
+        DevPtr dsegs = upload_segs<H>(ctx, segs);
+        t->seg_tables.push_back(dsegs);
+        DevPtr leaves = dalloc(ctx, height * D::DIG * sizeof(T));
+        {
+            Phase ph(ctx, "leaf_hash");
+            launch_check(hash_rows_device<D>(dptr<RowSeg<T>>(dsegs), uint32_t(segs.size()), width, segs[0], height,
+                                             dptr<T>(leaves), pp, ctx->stream),
+                         "leaf hash");
+        }
+        t->layers.push_back(leaves);
+        t->counts.push_back(height);
+        uint64_t cnt = height;
+        while (cnt > 1) {
+            uint64_t nxt = (cnt + 1) / 2;
+            DevPtr nl = dalloc(ctx, nxt * D::DIG * sizeof(T));
+            Phase ph(ctx, "compress");
+            launch_check(compress_layer_device<D>(dptr<T>(t->layers.back()), cnt, dptr<T>(nl), nxt, nullptr, 0, 0, pp,
+                                                  ctx->stream),
+                         "compress");
+            t->layers.push_back(nl);
+            t->counts.push_back(nxt);
+            cnt = nxt;
+        }
+    } catch (...) {
+        p3h_tree_free(t);
+        throw;
+    }
+    return t;
+}
+
+template <class H>
+void tree_open(p3h_ctx* ctx, const p3h_tree* t, const uint64_t* idx, uint32_t nq, typename H::T* values,
+               typename H::T* proof) {
+    using T = typename H::T;
+    constexpr uint32_t DIG = H::D::DIG;
+    const uint64_t maxh = t->counts[0];
+    for (uint32_t q = 0; q < nq; ++q)
+        if (idx[q] >= maxh) throw Err{P3H_EINVAL, "opening index out of range"};
+    if (nq == 0) return;
+    DevPtr didx = dalloc(ctx, nq * sizeof(uint64_t));
+    std::vector<uint64_t> rows(nq);
+    // values
+    if (values) {
+        std::vector<std::vector<T>> parts;
+        std::vector<RowSeg<T>> segs;
+        if (t->flat) {
+            segs.resize(t->flat_segs_host.size() / sizeof(RowSeg<T>));
+            std::memcpy(segs.data(), t->flat_segs_host.data(), t->flat_segs_host.size());
+            for (uint32_t q = 0; q < nq; ++q) rows[q] = idx[q];
+            h2d(ctx, didx->ptr, rows.data(), nq * sizeof(uint64_t));
+            DevPtr ds = upload_segs<H>(ctx, segs);
+            DevPtr out = dalloc(ctx, uint64_t(nq) * t->total_width * sizeof(T));
+            launch_check(gather_rows_device<T>(dptr<RowSeg<T>>(ds), uint32_t(segs.size()), t->total_width,
+                                               dptr<uint64_t>(didx), nq, dptr<T>(out), ctx->stream),
+                         "gather rows");
+            d2h(ctx, values, out->ptr, uint64_t(nq) * t->total_width * sizeof(T));
+        } else {
+            std::vector<T> host(uint64_t(nq) * t->total_width);
+            uint32_t col = 0;
+            for (size_t m = 0; m < t->mats.size(); ++m) {
+                uint32_t w = t->widths[m];
+                uint64_t stride = maxh / t->heights[m];
+                for (uint32_t q = 0; q < nq; ++q) rows[q] = idx[q] / stride;
+                h2d(ctx, didx->ptr, rows.data(), nq * sizeof(uint64_t));
+                std::vector<RowSeg<T>> one{seg_of<H>(t->mats[m])};
+                DevPtr ds = upload_segs<H>(ctx, one);
+                DevPtr out = dalloc(ctx, std::max<uint64_t>(1, uint64_t(nq) * w) * sizeof(T));
+                launch_check(gather_rows_device<T>(dptr<RowSeg<T>>(ds), 1, w, dptr<uint64_t>(didx), nq, dptr<T>(out),
+                                                   ctx->stream),
+                             "gather rows");
+                std::vector<T> part(uint64_t(nq) * w);
+                d2h(ctx, part.data(), out->ptr, part.size() * sizeof(T));
+                for (uint32_t q = 0; q < nq; ++q)
+                    std::memcpy(&host[uint64_t(q) * t->total_width + col], &part[uint64_t(q) * w], w * sizeof(T));
+                col += w;
+            }
+            std::memcpy(values, host.data(), host.size() * sizeof(T));
+        }
+    }
+    // proofs: sibling of each layer below the top
+    if (proof) {
+        size_t steps = t->layers.size() - 1;
+        std::vector<T> host(uint64_t(nq) * steps * DIG, T(0));
+        for (size_t s = 0; s < steps; ++s) {
+            std::vector<uint64_t> sib(nq);
+            std::vector<bool> valid(nq);
+            for (uint32_t q = 0; q < nq; ++q) {
+                uint64_t sidx = (idx[q] >> s) ^ 1;
+                valid[q] = sidx < t->counts[s];
+                sib[q] = valid[q] ? sidx : 0;
+            }
+            h2d(ctx, didx->ptr, sib.data(), nq * sizeof(uint64_t));
+            std::vector<RowSeg<T>> one{RowSeg<T>{dptr<T>(t->layers[s]), ~0ull, DIG, 0, 0, 0}};
+            DevPtr ds = upload_segs<H>(ctx, one);
+            DevPtr out = dalloc(ctx, uint64_t(nq) * DIG * sizeof(T));
+            launch_check(gather_rows_device<T>(dptr<RowSeg<T>>(ds), 1, DIG, dptr<uint64_t>(didx), nq, dptr<T>(out),
+                                               ctx->stream),
+                         "gather proof");
+            std::vector<T> part(uint64_t(nq) * DIG);
+            d2h(ctx, part.data(), out->ptr, part.size() * sizeof(T));
+            for (uint32_t q = 0; q < nq; ++q)
+                if (valid[q])
+                    std::memcpy(&host[(uint64_t(q) * steps + s) * DIG], &part[uint64_t(q) * DIG], DIG * sizeof(T));
+        }
+        std::memcpy(proof, host.data(), host.size() * sizeof(T));
+    }
+}
+
+// ------------------------------------------------------------------- FRI ---
+const uint32_t* fri_tinv(p3h_ctx* ctx, uint32_t LH) {
+    auto it = ctx->fri_tinv.find(LH);
+    if (it != ctx->fri_tinv.end()) return dptr<uint32_t>(it->second);
+    uint32_t bits = LH - 1;
+    DevPtr buf = dalloc(ctx, (uint64_t(1) << bits) * sizeof(uint32_t));
+    uint32_t base = bb_mont(bb_inv(bb_root(LH)));
+    DevPtr bb = dalloc(ctx, sizeof(uint32_t));
+    h2d(ctx, bb->ptr, &base, sizeof(uint32_t));
+    // out[p] = (omega_{2^LH}^-1)^bitrev(p, LH-1) * (1/2)
+    launch_check(lde_twist_device<BabyBear>(dptr<uint32_t>(buf), bits, 1, dptr<uint32_t>(bb), bb_mont(bb_inv(2)),
+                                            bb_mont(1), ctx->stream),
+                 "fri tinv");
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    ctx->fri_tinv[LH] = buf;
+    return dptr<uint32_t>(buf);
+}
+
+p3h_mat* do_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uint32_t* beta) {
+    if (cur->field != P3H_BABYBEAR || cur->width != 4 || cur->k != 0)
+        throw Err{P3H_EINVAL, "fold input must be a natural-order (n x 4) BabyBear matrix of BabyBear^4 values"};
+    uint32_t lh = log2_exact(cur->height);
+    if (log_arity < 1 || log_arity > 3 || log_arity > lh) throw Err{P3H_EINVAL, "log_arity must be 1, 2 or 3"};
+    if (lh > 27) throw Err{P3H_EINVAL, "height exceeds two-adicity"};
+    for (int q = 0; q < 4; ++q)
+        if (beta[q] >= BB_P) throw Err{P3H_EINVAL, "beta is not canonical"};
+    uint64_t n_out = cur->height >> log_arity;
+    p3h_mat* out = new_mat(ctx, P3H_BABYBEAR, n_out, 4);
+    try {
+        const uint32_t* tinv = log_arity == 1 && lh >= 1 ? fri_tinv(ctx, lh) : nullptr;
+        Phase ph(ctx, "fold");
+        launch_check(fold_device(cur->data<uint32_t>(), out->data<uint32_t>(), n_out, lh, log_arity, beta, tinv,
+                                 bb_mont(bb_root(lh)), bb_mont(bb_root(log_arity)),
+                                 bb_mont(bb_pow(bb_inv(2), log_arity)), ctx->stream),
+                     "fold");
+    } catch (...) {
+        mat_unref(out);
+        throw;
+    }
+    return out;
+}
+
+p3h_mat* view_of(p3h_mat* base, uint64_t h, uint32_t w) {
+    auto* v = new p3h_mat{};
+    v->ctx = base->ctx;
+    v->field = base->field;
+    v->height = h;
+    v->width = w;
+    v->buf = base->buf;
+    v->offset_elems = base->offset_elems;
+    return v;
+}
+
+template <class Fn>
+int api(p3h_ctx* ctx, Fn&& fn) {
+    try {
+        fn();
+        return P3H_OK;
+    } catch (const Err& e) {
+        if (ctx) ctx->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->err = "host allocation failed";
+        return P3H_ENOMEM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return P3H_EDEVICE;
+    }
+}
+
+}  // namespace
+
+// ================================================================== C ABI ===
+extern "C" {
+
+int p3h_ctx_create(int device, uint64_t budget, p3h_ctx** out) {
+    if (!out) return P3H_EINVAL;
+    *out = nullptr;
+    auto* ctx = new p3h_ctx{};
+    ctx->device = device;
+    int rc = api(ctx, [&] {
+        int n = 0;
+        hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (device < 0 || device >= n) throw Err{P3H_EINVAL, "no such device"};
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        size_t fr = 0, tot = 0;
+        hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+        ctx->budget = budget ? budget : tot;
+    });
+    if (rc != P3H_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return P3H_OK;
+}
+
+int p3h_ctx_destroy(p3h_ctx* ctx) {
+    if (!ctx) return P3H_EINVAL;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    resolve_pending(ctx);
+    ctx->twiddles.clear();
+    ctx->twists.clear();
+    ctx->row_scales.clear();
+    ctx->fri_tinv.clear();
+    trim(ctx);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return P3H_OK;
+}
+
+const char* p3h_last_error(const p3h_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int p3h_ctx_sync(p3h_ctx* ctx) {
+    return api(ctx, [&] { hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
+}
+
+int p3h_ctx_memory(const p3h_ctx* ctx, uint64_t* in_use, uint64_t* budget, uint64_t* cached) {
+    if (!ctx) return P3H_EINVAL;
+    if (in_use) *in_use = ctx->in_use;
+    if (budget) *budget = ctx->budget;
+    if (cached) *cached = ctx->cached;
+    return P3H_OK;
+}
+
+int p3h_ctx_trim(p3h_ctx* ctx) {
+    return api(ctx, [&] { trim(ctx); });
+}
+
+int p3h_ctx_set_profiling(p3h_ctx* ctx, int on) {
+    return api(ctx, [&] {
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        resolve_pending(ctx);
+        ctx->stats.clear();
+        ctx->profiling = on != 0;
+    });
+}
+
+int p3h_phase_stats(const p3h_ctx* cctx, const char* phase, uint64_t* launches, double* total_ms) {
+    auto* ctx = const_cast<p3h_ctx*>(cctx);
+    return api(ctx, [&] {
+        resolve_pending(ctx);
+        auto it = ctx->stats.find(phase ? phase : "");
+        if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.launches;
+        if (total_ms) *total_ms = it == ctx->stats.end() ? 0 : it->second.ms;
+    });
+}
+
+int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages) {
+    return api(ctx, [&] {
+        if (log_cols > 6 || (max_stages && max_stages < 1) || max_stages > 13)
+            throw Err{P3H_EINVAL, "tuning out of range"};
+        ctx->tune.log_cols = log_cols;
+        ctx->tune.max_stages = max_stages;
+    });
+}
+
+int p3h_poseidon2_set(p3h_ctx* ctx, int field, uint32_t width, uint32_t rf, uint32_t rp, const void* consts) {
+    return api(ctx, [&] {
+        if (!consts) throw Err{P3H_EINVAL, "null constants"};
+        if (rf % 2 || rf / 2 > uint32_t(P2_MAX_HALF_FULL) || rp > uint32_t(P2_MAX_PARTIAL))
+            throw Err{P3H_EINVAL, "round counts out of range (rounds_f even <= 8, rounds_p <= 32)"};
+        if (field == P3H_BABYBEAR) {
+            if (width != 16) throw Err{P3H_EINVAL, "BabyBear Poseidon2 width must be 16"};
+            const uint32_t* c = static_cast<const uint32_t*>(consts);
+            P2Params<uint32_t, 16> p{};
+            size_t k = 0;
+            for (uint32_t r = 0; r < rf / 2; ++r)
+                for (int i = 0; i < 16; ++i) p.init[r][i] = bb_mont(c[k++] % BB_P);
+            for (uint32_t r = 0; r < rf / 2; ++r)
+                for (int i = 0; i < 16; ++i) p.term[r][i] = bb_mont(c[k++] % BB_P);
+            for (uint32_t r = 0; r < rp; ++r) p.internal[r] = bb_mont(c[k++] % BB_P);
+            for (int i = 0; i < 16; ++i) p.diag[i] = bb_mont(bb_pow(2, i));
+            p.half_full = rf / 2;
+            p.partial = rp;
+            ctx->bbp = p;
+            ctx->p2_set[P3H_BABYBEAR] = true;
+        } else if (field == P3H_GOLDILOCKS) {
+            if (width != 8) throw Err{P3H_EINVAL, "Goldilocks Poseidon2 width must be 8"};
+            const uint64_t* c = static_cast<const uint64_t*>(consts);
+            P2Params<uint64_t, 8> p{};
+            size_t k = 0;
+            for (uint32_t r = 0; r < rf / 2; ++r)
+                for (int i = 0; i < 8; ++i) p.init[r][i] = c[k++] % GL_P;
+            for (uint32_t r = 0; r < rf / 2; ++r)
+                for (int i = 0; i < 8; ++i) p.term[r][i] = c[k++] % GL_P;
+            for (uint32_t r = 0; r < rp; ++r) p.internal[r] = c[k++] % GL_P;
+            for (int i = 0; i < 8; ++i) p.diag[i] = gl_pow(2, i);
+            p.half_full = rf / 2;
+            p.partial = rp;
+            ctx->glp = p;
+            ctx->p2_set[P3H_GOLDILOCKS] = true;
+        } else {
+            throw Err{P3H_EINVAL, "unknown field"};
+        }
+    });
+}
+
+int p3h_poseidon2_permute(p3h_ctx* ctx, int field, void* states, uint64_t n) {
+    return api(ctx, [&] {
+        if (!states && n) throw Err{P3H_EINVAL, "null states"};
+        if (field == P3H_BABYBEAR) {
+            const auto& pp = p2<HBB>(ctx);
+            DevPtr d = dalloc(ctx, std::max<uint64_t>(1, n * 16 * 4));
+            h2d(ctx, d->ptr, states, n * 16 * 4);
+            launch_check(permute_device<BBDom>(dptr<uint32_t>(d), n, pp, ctx->stream), "permute");
+            d2h(ctx, states, d->ptr, n * 16 * 4);
+        } else if (field == P3H_GOLDILOCKS) {
+            const auto& pp = p2<HGL>(ctx);
+            DevPtr d = dalloc(ctx, std::max<uint64_t>(1, n * 8 * 8));
+            h2d(ctx, d->ptr, states, n * 8 * 8);
+            launch_check(permute_device<GLDom>(dptr<uint64_t>(d), n, pp, ctx->stream), "permute");
+            d2h(ctx, states, d->ptr, n * 8 * 8);
+        } else {
+            throw Err{P3H_EINVAL, "unknown field"};
+        }
+    });
+}
+
+int p3h_mat_upload(p3h_ctx* ctx, int field, const void* host, uint64_t h, uint32_t w, p3h_mat** out) {
+    return api(ctx, [&] {
+        if (!out || (!host && h * w)) throw Err{P3H_EINVAL, "null argument"};
+        if (field != P3H_BABYBEAR && field != P3H_GOLDILOCKS) throw Err{P3H_EINVAL, "unknown field"};
+        p3h_mat* m = new_mat(ctx, field, h, w);
+        try {
+            h2d(ctx, m->data<char>(), host, h * w * elem_size(field));
+        } catch (...) {
+            mat_unref(m);
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int p3h_mat_alloc(p3h_ctx* ctx, int field, uint64_t h, uint32_t w, p3h_mat** out) {
+    return api(ctx, [&] {
+        if (!out) throw Err{P3H_EINVAL, "null argument"};
+        if (field != P3H_BABYBEAR && field != P3H_GOLDILOCKS) throw Err{P3H_EINVAL, "unknown field"};
+        *out = new_mat(ctx, field, h, w);
+    });
+}
+
+int p3h_mat_download_rows(p3h_ctx* ctx, const p3h_mat* m, uint64_t row0, uint64_t nrows, void* host) {
+    return api(ctx, [&] {
+        if (!m || (!host && nrows)) throw Err{P3H_EINVAL, "null argument"};
+        if (row0 + nrows > m->height) throw Err{P3H_EINVAL, "row range out of bounds"};
+        size_t es = elem_size(m->field);
+        size_t row_bytes = size_t(m->width) * es;
+        if (m->k == 0) {
+            d2h(ctx, host, m->data<char>() + row0 * row_bytes, nrows * row_bytes);
+            return;
+        }
+        DevPtr tmp = dalloc(ctx, m->height * row_bytes);
+        if (m->field == P3H_BABYBEAR)
+            launch_check(coset_to_natural_device<uint32_t>(m->data<uint32_t>(), dptr<uint32_t>(tmp), m->L, m->k,
+                                                           m->width, ctx->stream),
+                         "coset_to_natural");
+        else
+            launch_check(coset_to_natural_device<uint64_t>(m->data<uint64_t>(), dptr<uint64_t>(tmp), m->L, m->k,
+                                                           m->width, ctx->stream),
+                         "coset_to_natural");
+        d2h(ctx, host, static_cast<char*>(tmp->ptr) + row0 * row_bytes, nrows * row_bytes);
+    });
+}
+
+int p3h_mat_download(p3h_ctx* ctx, const p3h_mat* m, void* host) {
+    if (!m) return P3H_EINVAL;
+    return p3h_mat_download_rows(ctx, m, 0, m->height, host);
+}
+
+int p3h_mat_info(const p3h_mat* m, int* field, uint64_t* h, uint32_t* w) {
+    if (!m) return P3H_EINVAL;
+    if (field) *field = m->field;
+    if (h) *h = m->height;
+    if (w) *w = m->width;
+    return P3H_OK;
+}
+
+int p3h_mat_free(p3h_mat* m) {
+    if (!m) return P3H_EINVAL;
+    mat_unref(m);
+    return P3H_OK;
+}
+
+int p3h_dft(p3h_ctx* ctx, int kind, const p3h_mat* in, uint64_t shift, p3h_mat** out) {
+    return api(ctx, [&] {
+        if (!in || !out) throw Err{P3H_EINVAL, "null argument"};
+        *out = in->field == P3H_BABYBEAR ? do_dft<HBB>(ctx, kind, in, shift) : do_dft<HGL>(ctx, kind, in, shift);
+    });
+}
+
+int p3h_coset_lde(p3h_ctx* ctx, const p3h_mat* in, uint32_t added_bits, uint64_t shift, p3h_mat** out) {
+    return api(ctx, [&] {
+        if (!in || !out) throw Err{P3H_EINVAL, "null argument"};
+        *out = in->field == P3H_BABYBEAR ? do_lde<HBB>(ctx, in, added_bits, shift)
+                                         : do_lde<HGL>(ctx, in, added_bits, shift);
+    });
+}
+
+int p3h_mmcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uint32_t cap_height, p3h_tree** out) {
+    return api(ctx, [&] {
+        if (!mats || !out || n == 0 || !mats[0]) throw Err{P3H_EINVAL, "null argument"};
+        *out = mats[0]->field == P3H_BABYBEAR ? do_mmcs_commit<HBB>(ctx, mats, n, cap_height)
+                                              : do_mmcs_commit<HGL>(ctx, mats, n, cap_height);
+    });
+}
+
+int p3h_flat_commit(p3h_ctx* ctx, const p3h_mat* m, p3h_tree** out) {
+    return api(ctx, [&] {
+        if (!m || !out) throw Err{P3H_EINVAL, "null argument"};
+        std::vector<p3h_mat*> refs{const_cast<p3h_mat*>(m)};
+        if (m->field == P3H_BABYBEAR)
+            *out = flat_tree<HBB>(ctx, {seg_of<HBB>(m)}, m->height, m->width, refs);
+        else
+            *out = flat_tree<HGL>(ctx, {seg_of<HGL>(m)}, m->height, m->width, refs);
+    });
+}
+
+int p3h_tree_cap(p3h_ctx* ctx, const p3h_tree* t, void* cap_out, uint32_t* cap_len) {
+    return api(ctx, [&] {
+        if (!t) throw Err{P3H_EINVAL, "null tree"};
+        uint64_t cnt = t->counts.back();
+        size_t dig_bytes = t->field == P3H_BABYBEAR ? 32 : 32;
+        if (cap_len) *cap_len = uint32_t(cnt);
+        if (cap_out) d2h(ctx, cap_out, t->layers.back()->ptr, cnt * dig_bytes);
+    });
+}
+
+int p3h_tree_info(const p3h_tree* t, uint32_t* n_layers, uint64_t* n_leaves, uint32_t* total_width) {
+    if (!t) return P3H_EINVAL;
+    if (n_layers) *n_layers = uint32_t(t->layers.size());
+    if (n_leaves) *n_leaves = t->counts[0];
+    if (total_width) *total_width = t->total_width;
+    return P3H_OK;
+}
+
+int p3h_tree_layer(p3h_ctx* ctx, const p3h_tree* t, uint32_t layer, void* out) {
+    return api(ctx, [&] {
+        if (!t || !out || layer >= t->layers.size()) throw Err{P3H_EINVAL, "bad layer"};
+        d2h(ctx, out, t->layers[layer]->ptr, t->counts[layer] * 32);
+    });
+}
+
+int p3h_tree_open(p3h_ctx* ctx, const p3h_tree* t, const uint64_t* idx, uint32_t n, void* values, void* proof) {
+    return api(ctx, [&] {
+        if (!t || (!idx && n)) throw Err{P3H_EINVAL, "null argument"};
+        if (t->field == P3H_BABYBEAR)
+            tree_open<HBB>(ctx, t, idx, n, static_cast<uint32_t*>(values), static_cast<uint32_t*>(proof));
+        else
+            tree_open<HGL>(ctx, t, idx, n, static_cast<uint64_t*>(values), static_cast<uint64_t*>(proof));
+    });
+}
+
+int p3h_tree_free(p3h_tree* t) {
+    if (!t) return P3H_EINVAL;
+    for (auto* m : t->mats) mat_unref(m);
+    delete t;
+    return P3H_OK;
+}
+
+int p3h_fri_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uint32_t* beta, p3h_mat** out) {
+    return api(ctx, [&] {
+        if (!cur || !beta || !out) throw Err{P3H_EINVAL, "null argument"};
+        *out = do_fold(ctx, cur, log_arity, beta);
+    });
+}
+
+int p3h_fri_commit_phase(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, uint32_t log_final_poly_len,
+                         uint32_t max_log_arity, const uint32_t* betas, uint32_t* roots_out,
+                         uint32_t* final_poly_out, uint32_t* n_rounds, p3h_fri** keep) {
+    return api(ctx, [&] {
+        if (!codeword || !betas || !roots_out || !final_poly_out) throw Err{P3H_EINVAL, "null argument"};
+        if (codeword->field != P3H_BABYBEAR || codeword->width != 4 || codeword->k != 0)
+            throw Err{P3H_EINVAL, "codeword must be a natural-order (n x 4) BabyBear matrix"};
+        if (max_log_arity < 1 || max_log_arity > 3) throw Err{P3H_EINVAL, "max_log_arity must be 1..3"};
+        uint32_t lh0 = log2_exact(codeword->height);
+        uint32_t lfh = log_blowup + log_final_poly_len;
+        if (lh0 <= lfh) throw Err{P3H_EINVAL, "codeword must be longer than the final domain"};
+        auto* f = new p3h_fri{};
+        f->ctx = ctx;
+        try {
+            p3h_mat* cur = const_cast<p3h_mat*>(codeword);
+            mat_ref(cur);
+            f->layers.push_back(cur);
+            uint32_t round = 0;
+            while (cur->height > (1ull << lfh)) {
+                uint32_t lh = log2_exact(cur->height);
+                uint32_t la = std::min(lh - lfh, max_log_arity);
+                uint32_t arity = 1u << la;
+                p3h_mat* view = view_of(cur, cur->height >> la, 4 * arity);
+                p3h_tree* t = nullptr;
+                try {
+                    t = flat_tree<HBB>(ctx, {seg_of<HBB>(view)}, view->height, view->width, {view});
+                } catch (...) {
+                    mat_unref(view);
+                    throw;
+                }
+                mat_unref(view);  // the tree holds its own reference
+                f->trees.push_back(t);
+                p3h_mat* nxt = do_fold(ctx, cur, la, betas + 4 * round);
+                f->layers.push_back(nxt);
+                cur = nxt;
+                ++round;
+            }
+            for (uint32_t r = 0; r < round; ++r)
+                d2h(ctx, roots_out + 8 * r, f->trees[r]->layers.back()->ptr, 32);
+            // final polynomial: truncate to 2^log_final_poly_len, bit-reverse, inverse DFT (host, tiny)
+            uint32_t n = 1u << log_final_poly_len;
+            std::vector<uint32_t> last(uint64_t(cur->height) * 4);
+            d2h(ctx, last.data(), cur->data<uint32_t>(), last.size() * 4);
+            std::vector<uint32_t> br(4 * n);
+            for (uint32_t i = 0; i < n; ++i) {
+                uint32_t j = 0;
+                for (uint32_t b = 0; b < log_final_poly_len; ++b) j |= ((i >> b) & 1) << (log_final_poly_len - 1 - b);
+                for (int q = 0; q < 4; ++q) br[4 * j + q] = last[4 * i + q];
+            }
+            uint32_t winv = bb_inv(bb_root(log_final_poly_len)), ninv = bb_inv(n);
+            for (uint32_t k = 0; k < n; ++k)
+                for (int q = 0; q < 4; ++q) {
+                    uint64_t acc = 0;
+                    for (uint32_t j = 0; j < n; ++j)
+                        acc = (acc + uint64_t(bb_mul(br[4 * j + q], bb_pow(winv, uint64_t(k) * j)))) % BB_P;
+                    final_poly_out[4 * k + q] = bb_mul(uint32_t(acc), ninv);
+                }
+            if (n_rounds) *n_rounds = round;
+        } catch (...) {
+            p3h_fri_free(f);
+            throw;
+        }
+        if (keep)
+            *keep = f;
+        else
+            p3h_fri_free(f);
+    });
+}
+
+int p3h_fri_layer_tree(const p3h_fri* f, uint32_t round, const p3h_tree** out) {
+    if (!f || !out || round >= f->trees.size()) return P3H_EINVAL;
+    *out = f->trees[round];
+    return P3H_OK;
+}
+
+int p3h_fri_free(p3h_fri* f) {
+    if (!f) return P3H_EINVAL;
+    for (auto* t : f->trees) p3h_tree_free(t);
+    for (auto* m : f->layers) mat_unref(m);
+    delete f;
+    return P3H_OK;
+}
+
+int p3h_pcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, const uint64_t* shifts, uint32_t n, uint32_t log_blowup,
+                   p3h_tree** out) {
+    return api(ctx, [&] {
+        if (!mats || !shifts || !out || n == 0) throw Err{P3H_EINVAL, "null argument"};
+        std::vector<p3h_mat*> ldes;
+        try {
+            uint64_t maxh = 0;
+            uint32_t tw = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (mats[i]->field != P3H_BABYBEAR) throw Err{P3H_EINVAL, "PCS commit is BabyBear only"};
+                ldes.push_back(do_lde<HBB>(ctx, mats[i], log_blowup, shifts[i]));
+                maxh = std::max(maxh, ldes.back()->height);
+                tw += mats[i]->width;
+            }
+            if (tw == 0) throw Err{P3H_EINVAL, "TwoAdicFriPcs::commit: empty matrices (total_width=0)"};
+            uint32_t pw = 1;
+            while (pw < tw) pw <<= 1;
+            std::vector<RowSeg<uint32_t>> segs;
+            for (auto* l : ldes) {
+                RowSeg<uint32_t> s = seg_of<HBB>(l);
+                s.hmask = l->height - 1;
+                segs.push_back(s);
+            }
+            if (pw > tw) segs.push_back(RowSeg<uint32_t>{nullptr, ~0ull, pw - tw, 0, 0, 0});
+            *out = flat_tree<HBB>(ctx, segs, maxh, pw, ldes);
+        } catch (...) {
+            for (auto* l : ldes) mat_unref(l);
+            throw;
+        }
+        for (auto* l : ldes) mat_unref(l);
+    });
+}
+
+int p3h_batch_commit(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n,
+                     uint64_t shift, uint32_t cap_height, p3h_tree** out) {
+    return api(ctx, [&] {
+        if (!traces || !added_bits || !out || n == 0) throw Err{P3H_EINVAL, "null argument"};
+        std::vector<p3h_mat*> ldes;
+        try {
+            for (uint32_t i = 0; i < n; ++i) {
+                if (traces[i]->field != P3H_BABYBEAR) throw Err{P3H_EINVAL, "batch commit is BabyBear only"};
+                ldes.push_back(do_lde<HBB>(ctx, traces[i], added_bits[i], shift));
+            }
+            *out = do_mmcs_commit<HBB>(ctx, ldes.data(), n, cap_height);
+        } catch (...) {
+            for (auto* l : ldes) mat_unref(l);
+            throw;
+        }
+        for (auto* l : ldes) mat_unref(l);
+    });
+}
+
+}  // extern "C"

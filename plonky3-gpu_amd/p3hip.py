@@ -1,0 +1,456 @@
+"""Python host binding of libp3hip (include/p3hip.h) over ctypes.
+
+The classes mirror the reference's C++ API for the commit hot path
+(dloghin/plonky3-gpu, src/include/):
+
+* ``Radix2DitHip``      -- Radix2Dit / NttCuda: dft_batch, idft_batch,
+                           coset_dft_batch, coset_idft_batch, coset_lde_batch
+                           (radix2_dit.hpp:82-233, ntt_cuda.hpp:310-378)
+* ``MerkleTreeMmcsHip`` -- p3_merkle_tree::MerkleTreeMmcs: commit, open_batch
+                           (merkle_tree_mmcs.hpp:100-152)
+* ``FlatMerkleTreeMmcsHip`` -- p3_merkle::MerkleTreeMmcs: commit_matrix,
+                           open_row (fri_merkle_tree_mmcs.hpp:55-141)
+* ``fri_fold`` / ``fri_commit_phase`` -- fold_matrix_cuda and the prove_fri
+                           commit loop (fri_fold_cuda.hpp:306-348,
+                           fri_prover.hpp:83-178)
+* ``pcs_commit``        -- TwoAdicFriPcs::commit (two_adic_fri_pcs.hpp:207-266)
+
+Matrices are numpy arrays of canonical field elements (uint32 BabyBear,
+uint64 Goldilocks), shape (height, width).  Errors are raised as ValueError
+(the reference's std::invalid_argument) or RuntimeError (std::runtime_error).
+The library must be built (``python __graft_entry__.py`` or
+``make -C plonky3-gpu_amd/csrc``); there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+BABYBEAR = 0
+GOLDILOCKS = 1
+BB_P = 0x78000001
+GL_P = 0xFFFFFFFF00000001
+DTYPE = {BABYBEAR: np.uint32, GOLDILOCKS: np.uint64}
+DIGEST = {BABYBEAR: 8, GOLDILOCKS: 4}
+P2_WIDTH = {BABYBEAR: 16, GOLDILOCKS: 8}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libp3hip.so")
+
+_OK, _EINVAL, _ENOMEM, _EDEVICE, _EBUDGET, _ESTATE = range(6)
+
+
+class BudgetError(RuntimeError):
+    """A request exceeded the context's HBM budget."""
+
+
+_lib = None
+
+_vp = C.c_void_p
+_u32, _u64, _i32 = C.c_uint32, C.c_uint64, C.c_int
+_SIGS = {
+    "p3h_ctx_create": (_i32, [_i32, _u64, C.POINTER(_vp)]),
+    "p3h_ctx_destroy": (_i32, [_vp]),
+    "p3h_last_error": (C.c_char_p, [_vp]),
+    "p3h_ctx_sync": (_i32, [_vp]),
+    "p3h_ctx_memory": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
+    "p3h_ctx_trim": (_i32, [_vp]),
+    "p3h_ctx_set_profiling": (_i32, [_vp, _i32]),
+    "p3h_phase_stats": (_i32, [_vp, C.c_char_p, C.POINTER(_u64), C.POINTER(C.c_double)]),
+    "p3h_ctx_set_ntt_tuning": (_i32, [_vp, _u32, _u32]),
+    "p3h_poseidon2_set": (_i32, [_vp, _i32, _u32, _u32, _u32, _vp]),
+    "p3h_poseidon2_permute": (_i32, [_vp, _i32, _vp, _u64]),
+    "p3h_mat_upload": (_i32, [_vp, _i32, _vp, _u64, _u32, C.POINTER(_vp)]),
+    "p3h_mat_alloc": (_i32, [_vp, _i32, _u64, _u32, C.POINTER(_vp)]),
+    "p3h_mat_download": (_i32, [_vp, _vp, _vp]),
+    "p3h_mat_download_rows": (_i32, [_vp, _vp, _u64, _u64, _vp]),
+    "p3h_mat_info": (_i32, [_vp, C.POINTER(_i32), C.POINTER(_u64), C.POINTER(_u32)]),
+    "p3h_mat_free": (_i32, [_vp]),
+    "p3h_dft": (_i32, [_vp, _i32, _vp, _u64, C.POINTER(_vp)]),
+    "p3h_coset_lde": (_i32, [_vp, _vp, _u32, _u64, C.POINTER(_vp)]),
+    "p3h_mmcs_commit": (_i32, [_vp, C.POINTER(_vp), _u32, _u32, C.POINTER(_vp)]),
+    "p3h_flat_commit": (_i32, [_vp, _vp, C.POINTER(_vp)]),
+    "p3h_tree_cap": (_i32, [_vp, _vp, _vp, C.POINTER(_u32)]),
+    "p3h_tree_info": (_i32, [_vp, C.POINTER(_u32), C.POINTER(_u64), C.POINTER(_u32)]),
+    "p3h_tree_layer": (_i32, [_vp, _vp, _u32, _vp]),
+    "p3h_tree_open": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "p3h_tree_free": (_i32, [_vp]),
+    "p3h_fri_fold": (_i32, [_vp, _vp, _u32, _vp, C.POINTER(_vp)]),
+    "p3h_fri_commit_phase": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, C.POINTER(_u32), C.POINTER(_vp)]),
+    "p3h_fri_layer_tree": (_i32, [_vp, _u32, C.POINTER(_vp)]),
+    "p3h_fri_free": (_i32, [_vp]),
+    "p3h_pcs_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u32, C.POINTER(_vp)]),
+    "p3h_batch_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u64, _u32, C.POINTER(_vp)]),
+}
+
+
+def exported_symbols():
+    """Names the C ABI declares (include/p3hip.h)."""
+    return list(_SIGS)
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libp3hip.so and attach signatures; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libp3hip.so not built at {path}; run `python __graft_entry__.py`")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+def _field_of(dtype) -> int:
+    if dtype == np.uint32:
+        return BABYBEAR
+    if dtype == np.uint64:
+        return GOLDILOCKS
+    raise ValueError(f"unsupported dtype {dtype}")
+
+
+class Context:
+    """One device context (one HIP stream, a caching allocator, an HBM budget)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.p3h_ctx_create(device, hbm_budget, C.byref(h))
+        if rc != _OK:
+            raise RuntimeError(f"p3h_ctx_create failed with status {rc}")
+        self.h = h
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc: int):
+        if rc == _OK:
+            return
+        msg = self.lib.p3h_last_error(self.h).decode()
+        if rc == _EINVAL:
+            raise ValueError(msg)
+        if rc == _EBUDGET:
+            raise BudgetError(msg)
+        raise RuntimeError(f"[status {rc}] {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.p3h_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        self._check(self.lib.p3h_ctx_sync(self.h))
+
+    def memory(self):
+        a, b, c = _u64(), _u64(), _u64()
+        self._check(self.lib.p3h_ctx_memory(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"in_use": a.value, "budget": b.value, "cached": c.value}
+
+    def trim(self):
+        self._check(self.lib.p3h_ctx_trim(self.h))
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.p3h_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def phase_stats(self, phase: str):
+        n, ms = _u64(), C.c_double()
+        self._check(self.lib.p3h_phase_stats(self.h, phase.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def set_ntt_tuning(self, log_cols: int = 0, max_stages: int = 0):
+        self._check(self.lib.p3h_ctx_set_ntt_tuning(self.h, log_cols, max_stages))
+
+    # -- Poseidon2 --------------------------------------------------------
+    def set_poseidon2(self, field: int, consts: np.ndarray, rounds_f: int = 8, rounds_p: int = 13):
+        width = P2_WIDTH[field]
+        c = np.ascontiguousarray(consts, dtype=DTYPE[field])
+        if c.size != rounds_f * width + rounds_p:
+            raise ValueError("constant count does not match rounds_f * width + rounds_p")
+        self._check(self.lib.p3h_poseidon2_set(self.h, field, width, rounds_f, rounds_p, _ptr(c)))
+
+    def permute(self, field: int, states: np.ndarray) -> np.ndarray:
+        s = np.ascontiguousarray(states, dtype=DTYPE[field]).copy()
+        n = s.size // P2_WIDTH[field]
+        self._check(self.lib.p3h_poseidon2_permute(self.h, field, _ptr(s), n))
+        return s
+
+    # -- matrices ---------------------------------------------------------
+    def upload(self, a: np.ndarray) -> "Mat":
+        a = np.ascontiguousarray(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        field = _field_of(a.dtype)
+        h = _vp()
+        self._check(self.lib.p3h_mat_upload(self.h, field, _ptr(a), a.shape[0], a.shape[1], C.byref(h)))
+        return Mat(self, h)
+
+    def alloc(self, field: int, height: int, width: int) -> "Mat":
+        h = _vp()
+        self._check(self.lib.p3h_mat_alloc(self.h, field, height, width, C.byref(h)))
+        return Mat(self, h)
+
+    # -- operations -------------------------------------------------------
+    def dft(self, kind: int, m: "Mat", shift: int = 1) -> "Mat":
+        h = _vp()
+        self._check(self.lib.p3h_dft(self.h, kind, m.h, shift, C.byref(h)))
+        return Mat(self, h)
+
+    def coset_lde(self, m: "Mat", added_bits: int, shift: int) -> "Mat":
+        h = _vp()
+        self._check(self.lib.p3h_coset_lde(self.h, m.h, added_bits, shift, C.byref(h)))
+        return Mat(self, h)
+
+    def mmcs_commit(self, mats, cap_height: int = 0) -> "Tree":
+        arr = (_vp * len(mats))(*[m.h for m in mats])
+        h = _vp()
+        self._check(self.lib.p3h_mmcs_commit(self.h, arr, len(mats), cap_height, C.byref(h)))
+        return Tree(self, h, mats[0].field)
+
+    def flat_commit(self, m: "Mat") -> "Tree":
+        h = _vp()
+        self._check(self.lib.p3h_flat_commit(self.h, m.h, C.byref(h)))
+        return Tree(self, h, m.field)
+
+    def pcs_commit(self, mats, shifts, log_blowup: int) -> "Tree":
+        arr = (_vp * len(mats))(*[m.h for m in mats])
+        sh = np.ascontiguousarray(shifts, dtype=np.uint64)
+        h = _vp()
+        self._check(self.lib.p3h_pcs_commit(self.h, arr, _ptr(sh), len(mats), log_blowup, C.byref(h)))
+        return Tree(self, h, BABYBEAR)
+
+    def batch_commit(self, traces, added_bits, shift: int, cap_height: int = 0) -> "Tree":
+        arr = (_vp * len(traces))(*[m.h for m in traces])
+        ab = np.ascontiguousarray(added_bits, dtype=np.uint32)
+        h = _vp()
+        self._check(self.lib.p3h_batch_commit(self.h, arr, _ptr(ab), len(traces), shift, cap_height, C.byref(h)))
+        return Tree(self, h, BABYBEAR)
+
+    def fri_fold(self, cur: "Mat", log_arity: int, beta) -> "Mat":
+        b = np.ascontiguousarray(beta, dtype=np.uint32)
+        h = _vp()
+        self._check(self.lib.p3h_fri_fold(self.h, cur.h, log_arity, _ptr(b), C.byref(h)))
+        return Mat(self, h)
+
+    def fri_commit_phase(self, codeword: "Mat", log_blowup: int, log_final_poly_len: int, max_log_arity: int,
+                         betas: np.ndarray, keep: bool = False):
+        """Returns (roots [rounds x 8], final_poly [2^lfpl x 4], Fri or None)."""
+        b = np.ascontiguousarray(betas, dtype=np.uint32).reshape(-1)
+        max_rounds = max(1, b.size // 4)
+        roots = np.zeros((max_rounds, 8), dtype=np.uint32)
+        final = np.zeros((1 << log_final_poly_len, 4), dtype=np.uint32)
+        nr = _u32()
+        kp = _vp()
+        self._check(self.lib.p3h_fri_commit_phase(self.h, codeword.h, log_blowup, log_final_poly_len, max_log_arity,
+                                                  _ptr(b), _ptr(roots), _ptr(final), C.byref(nr),
+                                                  C.byref(kp) if keep else None))
+        fri = Fri(self, kp) if keep else None
+        return roots[: nr.value], final, fri
+
+
+class Mat:
+    """Device-resident matrix handle."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+        f, hh, w = _i32(), _u64(), _u32()
+        ctx._check(ctx.lib.p3h_mat_info(h, C.byref(f), C.byref(hh), C.byref(w)))
+        self.field, self.height, self.width = f.value, hh.value, w.value
+
+    @property
+    def shape(self):
+        return (self.height, self.width)
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.height, self.width), dtype=DTYPE[self.field])
+        self.ctx._check(self.ctx.lib.p3h_mat_download(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def download_rows(self, row0: int, nrows: int) -> np.ndarray:
+        out = np.empty((nrows, self.width), dtype=DTYPE[self.field])
+        self.ctx._check(self.ctx.lib.p3h_mat_download_rows(self.ctx.h, self.h, row0, nrows, _ptr(out)))
+        return out
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.p3h_mat_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Tree:
+    """Device-resident Merkle tree (prover data)."""
+
+    def __init__(self, ctx: Context, h, field: int, owned: bool = True):
+        self.ctx, self.h, self.field, self.owned = ctx, h, field, owned
+        nl, nleaf, tw = _u32(), _u64(), _u32()
+        ctx._check(ctx.lib.p3h_tree_info(h, C.byref(nl), C.byref(nleaf), C.byref(tw)))
+        self.n_layers, self.n_leaves, self.total_width = nl.value, nleaf.value, tw.value
+
+    def cap(self) -> np.ndarray:
+        n = _u32()
+        self.ctx._check(self.ctx.lib.p3h_tree_cap(self.ctx.h, self.h, None, C.byref(n)))
+        out = np.empty((n.value, DIGEST[self.field]), dtype=DTYPE[self.field])
+        self.ctx._check(self.ctx.lib.p3h_tree_cap(self.ctx.h, self.h, _ptr(out), C.byref(n)))
+        return out
+
+    def root(self) -> np.ndarray:
+        return self.cap()[0]
+
+    def layer(self, i: int) -> np.ndarray:
+        cnt = self.n_leaves
+        # counts per layer: computed by halving (flat trees round up)
+        for _ in range(i):
+            cnt = (cnt + 1) // 2
+        out = np.empty((cnt, DIGEST[self.field]), dtype=DTYPE[self.field])
+        self.ctx._check(self.ctx.lib.p3h_tree_layer(self.ctx.h, self.h, i, _ptr(out)))
+        return out
+
+    def open(self, indices):
+        """Returns (values [n x total_width], proof [n x (n_layers-1) x DIGEST])."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64).reshape(-1)
+        vals = np.empty((idx.size, self.total_width), dtype=DTYPE[self.field])
+        proof = np.empty((idx.size, self.n_layers - 1, DIGEST[self.field]), dtype=DTYPE[self.field])
+        self.ctx._check(self.ctx.lib.p3h_tree_open(self.ctx.h, self.h, _ptr(idx), idx.size, _ptr(vals), _ptr(proof)))
+        return vals, proof
+
+    def free(self):
+        if self.h and self.owned:
+            self.ctx.lib.p3h_tree_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Fri:
+    """Kept FRI commit-phase layers."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def layer_tree(self, r: int) -> Tree:
+        t = _vp()
+        self.ctx._check(self.ctx.lib.p3h_fri_layer_tree(self.h, r, C.byref(t)))
+        return Tree(self.ctx, t, BABYBEAR, owned=False)
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.p3h_fri_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------- reference API --
+class Radix2DitHip:
+    """Drop-in for Radix2Dit<F> / NttCuda<F> on numpy matrices (host in, host out)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def _run(self, kind, mat, shift=1):
+        m = self.ctx.upload(mat)
+        try:
+            if m.height <= 1:
+                return np.array(mat, copy=True)
+            out = self.ctx.dft(kind, m, shift)
+            try:
+                return out.download()
+            finally:
+                out.free()
+        finally:
+            m.free()
+
+    def dft_batch(self, mat):
+        return self._run(0, mat)
+
+    def idft_batch(self, mat):
+        return self._run(1, mat)
+
+    def coset_dft_batch(self, mat, shift):
+        return self._run(2, mat, shift)
+
+    def coset_idft_batch(self, mat, shift):
+        return self._run(3, mat, shift)
+
+    def coset_lde_batch(self, mat, added_bits, shift):
+        m = self.ctx.upload(mat)
+        try:
+            out = self.ctx.coset_lde(m, added_bits, shift)
+            try:
+                return out.download()
+            finally:
+                out.free()
+        finally:
+            m.free()
+
+
+class MerkleTreeMmcsHip:
+    """Drop-in for p3_merkle_tree::MerkleTreeMmcs (multi-height, cap)."""
+
+    def __init__(self, ctx: Context, cap_height: int = 0):
+        self.ctx, self.cap_height = ctx, cap_height
+
+    def commit(self, matrices):
+        mats = [self.ctx.upload(m) for m in matrices]
+        tree = self.ctx.mmcs_commit(mats, self.cap_height)
+        for m in mats:
+            m.free()  # the tree keeps its own references
+        return tree.cap(), tree
+
+    def open_batch(self, index: int, tree: Tree):
+        """(opened_values per matrix in commit order, proof digests)."""
+        vals, proof = tree.open([index])
+        return vals[0], proof[0]
+
+
+class FlatMerkleTreeMmcsHip:
+    """Drop-in for p3_merkle::MerkleTreeMmcs (single matrix, zero-padded odd layers)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def commit_matrix(self, data, width: int):
+        arr = np.ascontiguousarray(data)
+        if width == 0 or arr.size % width != 0:
+            raise ValueError("commit_matrix: bad dimensions")
+        if arr.size == 0:
+            raise ValueError("commit_matrix: empty matrix")
+        m = self.ctx.upload(arr.reshape(-1, width))
+        tree = self.ctx.flat_commit(m)
+        m.free()
+        return tree.root(), tree
+
+    def open_row(self, tree: Tree, row_index: int):
+        vals, proof = tree.open([row_index])
+        return vals[0], proof[0]

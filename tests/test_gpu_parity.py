@@ -1,0 +1,233 @@
+"""GPU parity: libp3hip (through the C ABI) against the golden fixtures of the
+reference build and against the CPU oracle on the same seeded inputs.
+Bit-exact everywhere: this is integer field arithmetic."""
+
+import numpy as np
+import pytest
+
+import native
+import p3hip
+from native import BB_P, GL_P, U32, U64
+
+pytestmark = pytest.mark.gpu
+RF = 8
+BB, GL = p3hip.BABYBEAR, p3hip.GOLDILOCKS
+
+
+def rnd(seed, shape, f="bb"):
+    rng = np.random.default_rng(seed)
+    p = BB_P if f == "bb" else GL_P
+    return rng.integers(0, p, size=shape, dtype=native.DT[f])
+
+
+def mats_of(golden, prefix):
+    out, j = [], 0
+    while f"{prefix}_m{j}" in golden:
+        out.append(golden[f"{prefix}_m{j}"])
+        j += 1
+    return out
+
+
+# -------------------------------------------------------------- Poseidon2 --
+def test_permutation(ctx, golden):
+    assert (ctx.permute(BB, golden["perm_bb_in"]) == golden["perm_bb_out"]).all()
+    assert (ctx.permute(GL, golden["perm_gl_in"]) == golden["perm_gl_out"]).all()
+
+
+def test_permutation_many_states(ctx, golden, orc):
+    st = rnd(1, 16 * 3000)
+    assert (ctx.permute(BB, st) == orc.permute("bb", golden["consts_bb"], RF, 13, st)).all()
+    stg = rnd(2, 8 * 1000, "gl")
+    assert (ctx.permute(GL, stg) == orc.permute("gl", golden["consts_gl"], RF, 22, stg)).all()
+
+
+# ------------------------------------------------------------------- DFT --
+def test_dft_golden(ctx, golden):
+    dft = p3hip.Radix2DitHip(ctx)
+    for kind, fn in enumerate((dft.dft_batch, dft.idft_batch)):
+        assert (fn(golden["dft_bb_in"]) == golden[f"dft_bb_k{kind}"]).all(), kind
+        assert (fn(golden["dft_gl_in"]) == golden[f"dft_gl_k{kind}"]).all(), kind
+    assert (dft.coset_dft_batch(golden["dft_bb_in"], 31) == golden["dft_bb_k2"]).all()
+    assert (dft.coset_idft_batch(golden["dft_bb_in"], 31) == golden["dft_bb_k3"]).all()
+    assert (dft.coset_dft_batch(golden["dft_gl_in"], 7) == golden["dft_gl_k2"]).all()
+    assert (dft.coset_idft_batch(golden["dft_gl_in"], 7) == golden["dft_gl_k3"]).all()
+
+
+@pytest.mark.parametrize("f,lh,w", [("bb", 1, 3), ("bb", 7, 5), ("bb", 10, 16), ("bb", 11, 3), ("bb", 14, 20),
+                                    ("bb", 17, 2), ("gl", 3, 2), ("gl", 9, 4), ("gl", 12, 5)])
+def test_dft_kinds_vs_oracle(ctx, orc, f, lh, w):
+    m = rnd(lh * 7 + w, (1 << lh, w), f)
+    shift = 31 if f == "bb" else 7
+    for kind in range(4):
+        got = ctx.dft(kind, ctx.upload(m), shift).download()
+        assert (got == orc.dft(f, kind, m, shift)).all(), kind
+
+
+# ------------------------------------------------------------------- LDE --
+def test_lde_golden(ctx, golden):
+    dft = p3hip.Radix2DitHip(ctx)
+    assert (dft.coset_lde_batch(golden["lde_bb_in"], 1, 31) == golden["lde_bb_a1"]).all()
+    assert (dft.coset_lde_batch(golden["lde_bb_in"], 2, 31) == golden["lde_bb_a2"]).all()
+    assert (dft.coset_lde_batch(golden["lde_gl_in"], 2, 7) == golden["lde_gl_a2"]).all()
+
+
+@pytest.mark.parametrize("f,lh,w,k", [("bb", 0, 5, 1), ("bb", 1, 3, 1), ("bb", 4, 8, 1), ("bb", 6, 5, 2),
+                                      ("bb", 10, 16, 1), ("bb", 11, 3, 1), ("bb", 12, 33, 2), ("bb", 13, 7, 3),
+                                      ("bb", 16, 32, 1), ("bb", 15, 64, 0), ("gl", 0, 3, 2), ("gl", 5, 3, 2),
+                                      ("gl", 9, 4, 2), ("gl", 10, 5, 1), ("gl", 13, 3, 2)])
+def test_lde_vs_oracle(ctx, orc, f, lh, w, k):
+    m = rnd(100 + lh * 3 + w + k, (1 << lh, w), f)
+    shift = 31 if f == "bb" else 7
+    got = ctx.coset_lde(ctx.upload(m), k, shift).download()
+    assert (got == orc.lde(f, m, k, shift)).all()
+
+
+# ------------------------------------------------------------------ MMCS --
+def test_flat_commit_golden(ctx, golden):
+    mmcs = p3hip.FlatMerkleTreeMmcsHip(ctx)
+    for i in range(6):
+        d = golden[f"flat_bb_{i}_in"]
+        root, _ = mmcs.commit_matrix(d.reshape(-1), d.shape[1])
+        assert (root == golden[f"flat_bb_{i}_root"]).all(), i
+    for i in range(3):
+        d = golden[f"flat_gl_{i}_in"]
+        root, _ = mmcs.commit_matrix(d.reshape(-1), d.shape[1])
+        assert (root == golden[f"flat_gl_{i}_root"]).all(), i
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (2, 8), (3, 9), (999, 16), (1024, 24), (513, 7), (4096, 256)])
+def test_flat_commit_and_open_vs_oracle(ctx, golden, orc, h, w):
+    d = rnd(h + w, (h, w))
+    t = ctx.flat_commit(ctx.upload(d))
+    root, layers = orc.flat_commit("bb", golden["consts_bb"], RF, 13, d, with_tree=True)
+    assert (t.root() == root).all()
+    assert t.n_layers == len(layers)
+    idx = [0, h - 1, h // 2]
+    vals, proof = t.open(idx)
+    for q, i in enumerate(idx):
+        row, pr = orc.flat_open(golden["consts_bb"], RF, 13, d, i)
+        assert (vals[q] == row).all()
+        assert (proof[q] == pr).all()
+
+
+def test_mmcs_golden(ctx, golden):
+    for s in range(4):
+        mats = mats_of(golden, f"mm_bb_{s}")
+        for cap in (0, 1):
+            t = ctx.mmcs_commit([ctx.upload(m) for m in mats], cap)
+            assert (t.cap() == golden[f"mm_bb_{s}_cap{cap}"]).all(), (s, cap)
+            idx = int(golden[f"mm_bb_{s}_cap{cap}_idx"][0])
+            vals, proof = t.open([idx])
+            assert (vals[0] == golden[f"mm_bb_{s}_cap{cap}_vals"]).all()
+            assert (proof[0] == golden[f"mm_bb_{s}_cap{cap}_proof"]).all()
+    mats = mats_of(golden, "mm_gl")
+    t = ctx.mmcs_commit([ctx.upload(m) for m in mats], 1)
+    assert (t.cap() == golden["mm_gl_cap1"]).all()
+    vals, proof = t.open([6])
+    assert (vals[0] == golden["mm_gl_vals"]).all() and (proof[0] == golden["mm_gl_proof"]).all()
+
+
+def test_mmcs_random_heights_vs_oracle(ctx, golden, orc):
+    rng = np.random.default_rng(5)
+    shapes = [(1 << int(rng.integers(0, 11)), int(rng.integers(1, 70))) for _ in range(9)]
+    shapes.append((1024, 8))
+    mats = [rnd(300 + i, s) for i, s in enumerate(shapes)]
+    for cap in (0, 3):
+        t = ctx.mmcs_commit([ctx.upload(m) for m in mats], cap)
+        for idx in (0, 513, 1023):
+            c, vals, proof = orc.mmcs("bb", golden["consts_bb"], RF, 13, mats, cap, idx)
+            assert (t.cap() == c).all()
+            v, p = t.open([idx])
+            assert (v[0] == vals).all() and (p[0] == proof).all()
+            assert orc.mmcs_verify(golden["consts_bb"], RF, 13, shapes, cap, c, idx, v[0], p[0]) == 1
+
+
+def test_config1_commit_root(ctx, golden, orc):
+    """BabyBear 2^16 x 32, log_blowup 1, shift 31, flat and multi-height roots."""
+    trace = native.seeded(2024, (1 << 16) * 32).reshape(1 << 16, 32)
+    lde = ctx.coset_lde(ctx.upload(trace), 1, 31)
+    want_lde = orc.lde("bb", trace, 1, 31)
+    assert (lde.download() == want_lde).all()
+    want_root = orc.flat_commit("bb", golden["consts_bb"], RF, 13, want_lde)
+    assert (ctx.flat_commit(lde).root() == want_root).all()
+    assert (ctx.mmcs_commit([lde], 0).root() == want_root).all()
+
+
+def test_goldilocks_lde_commit(ctx, golden, orc):
+    trace = rnd(77, (1 << 10, 8), "gl")
+    lde = ctx.coset_lde(ctx.upload(trace), 2, 7)
+    want = orc.lde("gl", trace, 2, 7)
+    assert (lde.download() == want).all()
+    assert (ctx.mmcs_commit([lde]).root() == orc.flat_commit("gl", golden["consts_gl"], RF, 22, want)).all()
+
+
+# ------------------------------------------------------------------- FRI --
+def test_fold_golden(ctx, golden):
+    for la in (1, 2, 3):
+        cw = golden[f"fold_a{la}_in"]
+        out = ctx.fri_fold(ctx.upload(cw), la, golden[f"fold_a{la}_beta"]).download()
+        assert (out == golden[f"fold_a{la}_out"]).all(), la
+
+
+def test_fri_commit_golden(ctx, golden):
+    for case in range(3):
+        log_len, lb, lfpl, mla = (int(v) for v in golden[f"fri_{case}_params"])
+        roots, final, _ = ctx.fri_commit_phase(ctx.upload(golden[f"fri_{case}_cw"]), lb, lfpl, mla,
+                                               golden[f"fri_{case}_betas"])
+        assert (roots == golden[f"fri_{case}_roots"]).all(), case
+        assert (final == golden[f"fri_{case}_final"]).all(), case
+
+
+def test_fri_commit_vs_oracle(ctx, golden, orc):
+    cw = rnd(9, ((1 << 14), 4))
+    betas = rnd(10, (20, 4))
+    roots, final, fri = ctx.fri_commit_phase(ctx.upload(cw), 1, 3, 1, betas, keep=True)
+    want_roots, want_final = orc.fri_commit(golden["consts_bb"], RF, 13, cw, 1, 3, 1, betas)
+    assert len(roots) == 10
+    assert (roots == want_roots).all() and (final == want_final).all()
+    t0 = fri.layer_tree(0)
+    assert (t0.root() == want_roots[0]).all()
+    fri.free()
+
+
+# ------------------------------------------------------------------- PCS --
+def test_pcs_commit_golden(ctx, golden):
+    for case in range(3):
+        mats = [ctx.upload(m) for m in mats_of(golden, f"pcs_{case}")]
+        t = ctx.pcs_commit(mats, golden[f"pcs_{case}_shifts"], int(golden[f"pcs_{case}_lb"][0]))
+        assert (t.root() == golden[f"pcs_{case}_root"]).all(), case
+
+
+def test_batch_commit_vs_oracle(ctx, golden, orc):
+    rng = np.random.default_rng(11)
+    traces, ks, ldes = [], [], []
+    for i in range(6):
+        h, w, k = 1 << int(rng.integers(3, 10)), int(rng.integers(8, 40)), int(rng.integers(1, 3))
+        tr = rnd(500 + i, (h, w))
+        traces.append(tr)
+        ks.append(k)
+        ldes.append(orc.lde("bb", tr, k, 31))
+    t = ctx.batch_commit([ctx.upload(x) for x in traces], ks, 31, 1)
+    c, vals, proof = orc.mmcs("bb", golden["consts_bb"], RF, 13, ldes, 1, 77)
+    assert (t.cap() == c).all()
+    v, p = t.open([77])
+    assert (v[0] == vals).all() and (p[0] == proof).all()
+
+
+# ---------------------------------------------------------------- errors --
+def test_errors_are_raised(ctx):
+    with pytest.raises(ValueError):
+        ctx.coset_lde(ctx.upload(rnd(1, (3, 4))), 1, 31)  # height not a power of two
+    with pytest.raises(ValueError):
+        ctx.coset_lde(ctx.upload(rnd(1, (4, 4))), 1, BB_P)  # non-canonical shift
+    with pytest.raises(ValueError):
+        ctx.fri_fold(ctx.upload(rnd(1, (8, 4))), 4, [1, 2, 3, 4])
+
+
+def test_budget_rejects_oversized_request(golden):
+    with p3hip.Context(0, hbm_budget=1 << 20) as small:
+        small.set_poseidon2(BB, golden["consts_bb"], 8, 13)
+        with pytest.raises(p3hip.BudgetError):
+            small.upload(np.zeros((1 << 20, 4), dtype=U32))
+        m = small.upload(np.zeros((16, 8), dtype=U32))  # still usable afterwards
+        assert small.flat_commit(m).n_leaves == 16

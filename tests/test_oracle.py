@@ -1,0 +1,267 @@
+"""Pins for the CPU oracle (oracle/oracle.cpp) -- CPU only.
+
+What pins each oracle function (a plausible mistake in it fails at least one):
+
+| oracle function           | pinned by                                                                 |
+|---------------------------|---------------------------------------------------------------------------|
+| BabyBear / Goldilocks ops | reference KAT 440564289 (src/tests/test_two_adic.cpp:92-96), root orders  |
+| BabyBear^4 mul            | reference KATs (src/tests/test_extension_field.cpp:175-180, 239-243)      |
+| Poseidon2 permutation     | golden vectors from the reference build (BB RP13, BB zero RP22, GL RP22)  |
+| sponge / compression      | golden digests for inputs of length 0..33 / 0..17; empty input = zero     |
+| dft/idft/coset variants   | golden vectors; reference closed forms (src/tests/test_dft.cpp:127-161);  |
+|                           | brute-force DFT; idft(dft(x)) == x                                         |
+| coset LDE (BB and GL)     | golden vectors; O(n*N) polynomial evaluation (independent of the FFT)      |
+| flat MMCS commit / open   | golden roots (odd heights exercise the zero-digest padding)               |
+| multi-height MMCS         | golden caps, openings and proofs (cap 0/1, height ties, injection)         |
+| verify_batch              | accepts golden openings, rejects single-element tampering                 |
+| FRI fold                  | golden folds (arity 2/4/8); arity-2 closed form in pure Python integers    |
+| FRI commit phase          | golden layer roots and final polynomials                                   |
+| PCS commit                | golden roots from the reference's TwoAdicFriPcs::commit                    |
+
+When the reference build (oracle/_ref) is present, random cross-checks against
+it run as well.
+"""
+
+import numpy as np
+import pytest
+
+import native
+from native import BB_P, GL_P, U32, U64
+
+RF = 8
+
+
+def _mats(golden, prefix):
+    out, j = [], 0
+    while f"{prefix}_m{j}" in golden:
+        out.append(golden[f"{prefix}_m{j}"])
+        j += 1
+    return out
+
+
+# ------------------------------------------------------------------ fields --
+def test_babybear_two_adic_known_answer(orc):
+    assert orc.lib.orc_bb_two_adic_generator(27) == 440564289
+    for bits in (1, 5, 20, 27):
+        g = orc.lib.orc_bb_two_adic_generator(bits)
+        assert orc.lib.orc_bb_pow(g, 1 << bits) == 1
+        assert orc.lib.orc_bb_pow(g, 1 << (bits - 1)) == BB_P - 1
+
+
+def test_goldilocks_two_adic_structure(orc):
+    g32 = orc.lib.orc_gl_two_adic_generator(32)
+    assert g32 == pow(7, (GL_P - 1) >> 32, GL_P)
+    assert orc.lib.orc_gl_pow(g32, 1 << 32) == 1
+    assert orc.lib.orc_gl_pow(g32, 1 << 31) == GL_P - 1
+    # 7 generates the multiplicative group: not a square, not a cube, ...
+    for q in (2, 3, 5, 17, 257, 65537):  # p - 1 = 2^32 * 3 * 5 * 17 * 257 * 65537
+        assert (GL_P - 1) % q == 0
+        assert pow(7, (GL_P - 1) // q, GL_P) != 1
+    for a, b in ((GL_P - 1, GL_P - 1), (2**63, 2**63 + 5), (12345, 2**64 - 2**32)):
+        assert orc.lib.orc_gl_mul(a % GL_P, b % GL_P) == (a % GL_P) * (b % GL_P) % GL_P
+
+
+def test_bb4_mul_known_answers(orc, golden):
+    assert list(orc.bb4_mul([1, 2, 3, 4], [5, 6, 7, 8])) == [676, 588, 386, 60]
+    assert list(orc.bb4_mul([1, 2, 3, 4], [1, 2, 3, 4])) == [276, 268, 186, 20]
+    assert list(golden["bb4_mul"]) == [676, 588, 386, 60]
+
+
+# --------------------------------------------------------------- Poseidon2 --
+def test_permutation_golden(orc, golden):
+    assert (orc.permute("bb", golden["consts_bb"], RF, 13, golden["perm_bb_in"]) == golden["perm_bb_out"]).all()
+    assert (orc.permute("bb", golden["consts_bb_zero22"], RF, 22, golden["perm_bb_in"])
+            == golden["perm_bb_zero22_out"]).all()
+    assert (orc.permute("gl", golden["consts_gl"], RF, 22, golden["perm_gl_in"]) == golden["perm_gl_out"]).all()
+
+
+def test_sponge_golden(orc, golden):
+    s = golden["sponge_bb_stream"]
+    for n in range(34):
+        assert (orc.sponge("bb", golden["consts_bb"], RF, 13, s[:n]) == golden["sponge_bb_out"][n]).all(), n
+    assert (golden["sponge_bb_out"][0] == 0).all()  # empty input: no permutation
+    sg = golden["sponge_gl_stream"]
+    for n in range(18):
+        assert (orc.sponge("gl", golden["consts_gl"], RF, 22, sg[:n]) == golden["sponge_gl_out"][n]).all(), n
+
+
+def test_compress_golden(orc, golden):
+    for p, o in zip(golden["compress_bb_in"], golden["compress_bb_out"]):
+        assert (orc.compress("bb", golden["consts_bb"], RF, 13, p) == o).all()
+    for p, o in zip(golden["compress_gl_in"], golden["compress_gl_out"]):
+        assert (orc.compress("gl", golden["consts_gl"], RF, 22, p) == o).all()
+
+
+# --------------------------------------------------------------------- DFT --
+def test_dft_golden(orc, golden):
+    for kind in range(4):
+        assert (orc.dft("bb", kind, golden["dft_bb_in"], 31) == golden[f"dft_bb_k{kind}"]).all(), kind
+        assert (orc.dft("gl", kind, golden["dft_gl_in"], 7) == golden[f"dft_gl_k{kind}"]).all(), kind
+
+
+def test_dft_closed_forms(orc):
+    # src/tests/test_dft.cpp:127-161: constant polynomial and all-ones polynomial
+    c = np.zeros((8, 1), U32)
+    c[0, 0] = 7
+    assert (orc.dft("bb", 0, c) == 7).all()
+    ones = np.ones((8, 1), U32)
+    out = orc.dft("bb", 0, ones)[:, 0]
+    assert out[0] == 8 and (out[1:] == 0).all()
+    # delta at position 1 evaluates to omega^k
+    for f, p, root in (("bb", BB_P, orc.lib.orc_bb_two_adic_generator(4)),
+                       ("gl", GL_P, orc.lib.orc_gl_two_adic_generator(4))):
+        d = np.zeros((16, 1), native.DT[f])
+        d[1, 0] = 1
+        out = orc.dft(f, 0, d)[:, 0]
+        assert [int(x) for x in out] == [pow(int(root), k, p) for k in range(16)]
+
+
+def test_dft_round_trip_and_brute_force(orc):
+    rng = np.random.default_rng(3)
+    for f, p in (("bb", BB_P), ("gl", GL_P)):
+        for lh in (0, 1, 4, 7):
+            m = rng.integers(0, p, size=(1 << lh, 3), dtype=native.DT[f])
+            assert (orc.dft(f, 1, orc.dft(f, 0, m)) == m).all()
+            assert (orc.dft(f, 3, orc.dft(f, 2, m, 5), 5) == m).all()
+        m = rng.integers(0, p, size=(8, 2), dtype=native.DT[f])
+        root = int(orc.lib.orc_bb_two_adic_generator(3) if f == "bb" else orc.lib.orc_gl_two_adic_generator(3))
+        want = [[sum(int(m[j, c]) * pow(root, j * k, p) for j in range(8)) % p for c in range(2)] for k in range(8)]
+        assert orc.dft(f, 0, m).tolist() == want
+
+
+@pytest.mark.parametrize("f,lh,added,shift", [("bb", 0, 1, 31), ("bb", 3, 1, 31), ("bb", 5, 2, 31),
+                                              ("bb", 6, 3, 7), ("gl", 0, 2, 7), ("gl", 4, 1, 7),
+                                              ("gl", 5, 2, 7)])
+def test_lde_matches_polynomial_evaluation(orc, f, lh, added, shift):
+    rng = np.random.default_rng(lh * 10 + added)
+    p = BB_P if f == "bb" else GL_P
+    m = rng.integers(0, p, size=(1 << lh, 3), dtype=native.DT[f])
+    assert (orc.lde(f, m, added, shift) == orc.lde_brute(f, m, added, shift)).all()
+
+
+def test_lde_golden(orc, golden):
+    assert (orc.lde("bb", golden["lde_bb_in"], 1, 31) == golden["lde_bb_a1"]).all()
+    assert (orc.lde("bb", golden["lde_bb_in"], 2, 31) == golden["lde_bb_a2"]).all()
+    assert (orc.lde("gl", golden["lde_gl_in"], 2, 7) == golden["lde_gl_a2"]).all()
+
+
+# -------------------------------------------------------------------- MMCS --
+def test_flat_commit_golden(orc, golden):
+    for i in range(6):
+        assert (orc.flat_commit("bb", golden["consts_bb"], RF, 13, golden[f"flat_bb_{i}_in"])
+                == golden[f"flat_bb_{i}_root"]).all(), i
+    for i in range(3):
+        assert (orc.flat_commit("gl", golden["consts_gl"], RF, 22, golden[f"flat_gl_{i}_in"])
+                == golden[f"flat_gl_{i}_root"]).all(), i
+
+
+def test_flat_open_consistent_with_tree(orc, golden):
+    m = golden["flat_bb_4_in"]  # 7 rows: odd layers with zero padding
+    root, layers = orc.flat_commit("bb", golden["consts_bb"], RF, 13, m, with_tree=True)
+    for idx in range(7):
+        row, proof = orc.flat_open(golden["consts_bb"], RF, 13, m, idx)
+        assert (row == m[idx]).all()
+        cur = idx
+        for s, sib in enumerate(proof):
+            j = cur ^ 1
+            want = layers[s][j] if j < len(layers[s]) else np.zeros(8, U32)
+            assert (sib == want).all()
+            cur >>= 1
+
+
+def test_mmcs_golden_and_verify(orc, golden):
+    cb = golden["consts_bb"]
+    for s in range(4):
+        mats = _mats(golden, f"mm_bb_{s}")
+        dims = [m.shape for m in mats]
+        for cap in (0, 1):
+            idx = int(golden[f"mm_bb_{s}_cap{cap}_idx"][0])
+            c, vals, proof = orc.mmcs("bb", cb, RF, 13, mats, cap, idx)
+            assert (c == golden[f"mm_bb_{s}_cap{cap}"]).all(), (s, cap)
+            assert (vals == golden[f"mm_bb_{s}_cap{cap}_vals"]).all()
+            assert (proof == golden[f"mm_bb_{s}_cap{cap}_proof"]).all()
+            assert orc.mmcs_verify(cb, RF, 13, dims, cap, c, idx, vals, proof) == 1
+            bad = vals.copy()
+            bad[0] = (int(bad[0]) + 1) % BB_P
+            assert orc.mmcs_verify(cb, RF, 13, dims, cap, c, idx, bad, proof) == 0
+            if proof.size:
+                badp = proof.copy()
+                badp[0, 0] ^= 1
+                assert orc.mmcs_verify(cb, RF, 13, dims, cap, c, idx, vals, badp) == 0
+    mats = _mats(golden, "mm_gl")
+    c, vals, proof = orc.mmcs("gl", golden["consts_gl"], RF, 22, mats, 1, 6)
+    assert (c == golden["mm_gl_cap1"]).all()
+    assert (vals == golden["mm_gl_vals"]).all() and (proof == golden["mm_gl_proof"]).all()
+
+
+# --------------------------------------------------------------------- FRI --
+def _bb4_mul(a, b):
+    r = [0, 0, 0, 0]
+    for i in range(4):
+        for j in range(4):
+            t = a[i] * b[j]
+            if i + j < 4:
+                r[i + j] += t
+            else:
+                r[i + j - 4] += 11 * t
+    return [x % BB_P for x in r]
+
+
+def _bitrev(x, bits):
+    return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def test_fold_golden_and_closed_form(orc, golden):
+    for la, lh in ((1, 6), (2, 6), (3, 7)):
+        out = orc.fold(lh, la, golden[f"fold_a{la}_beta"], golden[f"fold_a{la}_in"])
+        assert (out == golden[f"fold_a{la}_out"]).all(), la
+    # arity 2: (e0+e1)/2 + beta (e0-e1) / (2x), x = omega_{2^lh}^bitrev(i, lh-1)
+    lh = 6
+    cw = [[int(v) for v in r] for r in golden["fold_a1_in"]]
+    beta = [int(v) for v in golden["fold_a1_beta"]]
+    w = int(orc.lib.orc_bb_two_adic_generator(lh))
+    inv2 = pow(2, BB_P - 2, BB_P)
+    for i in range(1 << (lh - 1)):
+        x = pow(w, _bitrev(i, lh - 1), BB_P)
+        e0, e1 = cw[2 * i], cw[2 * i + 1]
+        s = [(a + b) * inv2 % BB_P for a, b in zip(e0, e1)]
+        d = [(a - b) * inv2 * pow(x, BB_P - 2, BB_P) % BB_P for a, b in zip(e0, e1)]
+        bd = _bb4_mul(beta, d)
+        want = [(a + b) % BB_P for a, b in zip(s, bd)]
+        assert [int(v) for v in golden["fold_a1_out"][i]] == want
+
+
+def test_fri_commit_golden(orc, golden):
+    for case in range(3):
+        log_len, lb, lfpl, mla = (int(v) for v in golden[f"fri_{case}_params"])
+        roots, final = orc.fri_commit(golden["consts_bb"], RF, 13, golden[f"fri_{case}_cw"], lb, lfpl, mla,
+                                      golden[f"fri_{case}_betas"])
+        assert (roots == golden[f"fri_{case}_roots"]).all(), case
+        assert (final == golden[f"fri_{case}_final"]).all(), case
+
+
+def test_pcs_commit_golden(orc, golden):
+    for case in range(3):
+        mats = _mats(golden, f"pcs_{case}")
+        root = orc.pcs_commit(golden["consts_bb"], RF, 13, mats, golden[f"pcs_{case}_shifts"],
+                              int(golden[f"pcs_{case}_lb"][0]))
+        assert (root == golden[f"pcs_{case}_root"]).all(), case
+
+
+# ---------------------------------------- random cross-checks vs reference --
+def test_random_cross_check_against_reference(orc, ref):
+    rng = np.random.default_rng(77)
+    cb = native.p2_consts(9, "bb", 8, 13)
+    for _ in range(3):
+        h, w = int(rng.integers(1, 20)), int(rng.integers(1, 40))
+        m = rng.integers(0, BB_P, size=(h, w), dtype=U32)
+        assert (orc.flat_commit("bb", cb, RF, 13, m) == ref.flat_commit("bb", cb, RF, 13, m)).all()
+    shapes = [(16, 5), (8, 3), (16, 2), (4, 9), (2, 1)]
+    mats = [rng.integers(0, BB_P, size=s, dtype=U32) for s in shapes]
+    for cap in (0, 2):
+        a = orc.mmcs("bb", cb, RF, 13, mats, cap, 11)
+        b = ref.mmcs("bb", cb, RF, 13, mats, cap, 11)
+        for x, y in zip(a, b):
+            assert (x == y).all()
+    m = rng.integers(0, BB_P, size=(128, 6), dtype=U32)
+    assert (orc.lde("bb", m, 2, 31) == ref.lde("bb", m, 2, 31)).all()
