@@ -2,17 +2,24 @@
 //
 // The matrix is row-major (height x width, the reference's RowMajorMatrix),
 // so every column is an independent transform whose elements sit `width`
-// apart.  A pass covers a contiguous range of index bits [a, a+s): each
-// workgroup stages 2^s x G elements in LDS (G = sub-transforms = low indices
-// x columns), runs the s radix-2 stages there, and writes back.  Global loads
-// and stores walk the tile column-fastest, so each (row, column-tile) segment is
-// one contiguous burst.
+// apart.  A pass covers a contiguous range of index bits [a, a+S): each
+// workgroup stages a tile of 2^S positions x C columns in LDS, runs the S
+// radix-2 stages there and writes back.  Global loads and stores walk the tile
+// column-fastest, so each (row, column-tile) segment is one contiguous burst.
+//
+// Inside a pass the stages run in at most two register rounds of radix 2^R
+// (R <= 5): a thread takes a group of 2^R positions of one column, does R
+// butterfly stages in VGPRs and writes the group back.  S is a template
+// parameter, so every LDS offset and twiddle index inside a round is a
+// compile-time immediate; the LDS tile is stored column-major with one word of
+// padding per column, which keeps both the global staging and the round
+// accesses free of bank conflicts.
 //
 // Coset LDE (reference: Radix2Dit::coset_lde_batch, radix2_dit.hpp:221-233)
 // is computed as
 //   1. inverse transform by decimation in frequency (natural order in,
 //      bit-reversed coefficients out): outer passes over the high bits, then
-//   2. one fused inner pass per block of 2^s rows: the last DIF stages, then for
+//   2. one fused inner pass per block of 2^S rows: the last DIF stages, then for
 //      each coset r of the 2^k cosets of the small domain inside the large one
 //      the twist a_j * (shift * omega_N^r)^j / n and the first decimation-in-
 //      time stages of a forward transform (bit-reversed in, natural out),
@@ -31,6 +38,18 @@ namespace {
 
 constexpr int NTT_THREADS = 512;
 constexpr int TILE_BYTES = 65536;
+constexpr uint32_t MAX_STAGES = 10;
+
+template <class T>
+constexpr uint32_t tile_log() {
+    return 31 - __builtin_clz(uint32_t(TILE_BYTES / sizeof(T)));
+}
+// passes span at most tile_log - 4 bits (>= 16 columns per tile), so a pass
+// needs at most 2^(tile_log - 4) - 1 twiddles
+template <class T>
+constexpr uint32_t tw_lds() {
+    return 1u << (tile_log<T>() - 4);
+}
 
 template <class T>
 struct NttArgs {
@@ -41,18 +60,17 @@ struct NttArgs {
     uint64_t src_region_rows;  // row offset between regions in src
     uint64_t dst_region_rows;  // row offset between regions in dst
     uint32_t L;                // log2 transform size n
-    uint32_t a, s;             // this pass covers index bits [a, a+s)
+    uint32_t a;                // this pass covers index bits [a, a+S)
     uint32_t w;                // matrix width (row stride in elements)
     uint32_t logC;             // columns per tile = 2^logC
-    uint32_t logG;             // sub-transforms per tile = 2^logG (>= logC)
     uint32_t n_col_tiles;
     uint32_t n_regions;        // cosets (LDE) or 1
 };
 
 enum Mode : int {
-    OUTER_DIF_INV = 0,   // DIF stages, bits descending, in place or src->dst
-    OUTER_DIT_FWD = 1,   // DIT stages, bits ascending, per region
-    INNER_LDE = 2,       // DIF stages + per-coset twist + DIT stages
+    OUTER_DIF_INV = 0,     // DIF stages, bits descending
+    OUTER_DIT_FWD = 1,     // DIT stages, bits ascending, per region
+    INNER_LDE = 2,         // DIF stages + per-coset twist + DIT stages
     INNER_DIT_GATHER = 3,  // load rows bitrev(p) (times scale[row]), DIT stages
     INNER_DIF_SCATTER = 4  // DIF stages, store position p at row bitrev(p) times scale[row]
 };
@@ -61,157 +79,227 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits) {
     return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
 }
 
-template <class F, bool DIT>
-__device__ __forceinline__ void stage(typename F::T* lds, const typename F::T* tw, uint32_t L, uint32_t a,
-                                      uint32_t q, uint32_t s, uint32_t logG, uint32_t logC, uint32_t lo0) {
+// Twiddles of one pass.  Stage q (bit a+q) at tile position t, low index lo0
+// (the whole tile shares one low index), uses
+//   omega_{2^(a+q+1)}^(((t mod 2^q) << a) + lo0),
+// stored at tws[(2^q - 1) + (t mod 2^q)], gathered once per workgroup from the
+// global table (omega_n^e, e < n/2).
+template <class T>
+__device__ __forceinline__ void build_twiddles(T* tws, const T* tw, uint32_t L, uint32_t a, uint32_t S,
+                                               uint32_t lo0) {
+    const uint32_t count = (1u << S) - 1;
+    for (uint32_t i = threadIdx.x; i < count; i += NTT_THREADS) {
+        uint32_t jj = i + 1;  // 2^q + j
+        uint32_t q = 31 - __clz(jj);
+        uint32_t j = jj - (1u << q);
+        uint64_t e = uint64_t((j << a) + lo0) << (L - a - q - 1);
+        tws[i] = tw[e];
+    }
+}
+
+// LDS tile: element (t, c) at c * (2^S + 1) + t.
+template <int S>
+struct Tile {
+    static constexpr uint32_t B = 1u << S;
+    static constexpr uint32_t STRIDE = B + 1;
+};
+
+// Butterflies of one radix-2^R round on a register group x[0..2^R) holding
+// tile positions tb + m * 2^Q0; twb = tws + (tb mod 2^Q0).
+template <class F, int R, int Q0, bool DIT>
+__device__ __forceinline__ void butterflies(typename F::T* x, const typename F::T* twb) {
     using T = typename F::T;
-    const uint32_t G = 1u << logG;
-    const uint32_t nb = (1u << (s - 1)) << logG;  // butterflies in the tile
-    const uint32_t qmask = (1u << q) - 1;
-    const uint32_t sh = L - (a + q) - 1;           // omega_{2^(a+q+1)} = omega_n^(2^sh)
-    for (uint32_t u = threadIdx.x; u < nb; u += NTT_THREADS) {
-        uint32_t g = u & (G - 1);
-        uint32_t v = u >> logG;
-        uint32_t t0 = ((v >> q) << (q + 1)) | (v & qmask);
-        uint32_t t1 = t0 | (1u << q);
-        uint32_t lo = lo0 + (g >> logC);
-        uint32_t e = (((t0 & qmask) << a) | lo) << sh;
-        T wv = tw[e];
-        T x = lds[(t0 << logG) | g];
-        T y = lds[(t1 << logG) | g];
-        if (DIT) {
-            T yw = F::mul_c(y, wv);
-            lds[(t0 << logG) | g] = F::add(x, yw);
-            lds[(t1 << logG) | g] = F::sub(x, yw);
-        } else {
-            lds[(t0 << logG) | g] = F::add(x, y);
-            lds[(t1 << logG) | g] = F::mul_c(F::sub(x, y), wv);
+    constexpr uint32_t M = 1u << R;
+    if constexpr (DIT) {
+#pragma unroll
+        for (uint32_t j = 0; j < R; ++j) {
+#pragma unroll
+            for (uint32_t m = 0; m < M; ++m) {
+                if (m & (1u << j)) continue;
+                const T w = twb[(1u << (Q0 + j)) - 1 + ((m & ((1u << j) - 1)) << Q0)];
+                const T yw = F::mul_c(x[m | (1u << j)], w);
+                const T xv = x[m];
+                x[m] = F::add(xv, yw);
+                x[m | (1u << j)] = F::sub(xv, yw);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int jj = R - 1; jj >= 0; --jj) {
+            const uint32_t j = uint32_t(jj);
+#pragma unroll
+            for (uint32_t m = 0; m < M; ++m) {
+                if (m & (1u << j)) continue;
+                const T w = twb[(1u << (Q0 + j)) - 1 + ((m & ((1u << j) - 1)) << Q0)];
+                const T xv = x[m], yv = x[m | (1u << j)];
+                x[m] = F::add(xv, yv);
+                x[m | (1u << j)] = F::mul_c(F::sub(xv, yv), w);
+            }
         }
     }
 }
 
-template <class F, int MODE>
+// Group geometry of a round: thread-group gi -> column c and base position tb.
+template <int R, int Q0>
+struct Group {
+    uint32_t c, b, tb;
+    __device__ __forceinline__ Group(uint32_t gi, uint32_t logC) {
+        c = gi & ((1u << logC) - 1);
+        b = gi >> logC;
+        tb = ((b >> Q0) << (Q0 + R)) | (b & ((1u << Q0) - 1));
+    }
+};
+
+// A full round: every group is loaded with ld(c, t), transformed, stored with
+// st(c, t, v).  Loads and stores are either the LDS tile or global memory.
+template <class F, int S, int R, int Q0, bool DIT, class Ld, class St>
+__device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC, Ld ld, St st) {
+    using T = typename F::T;
+    constexpr uint32_t M = 1u << R;
+    const uint32_t groups = ((1u << S) >> R) << logC;
+    for (uint32_t gi = threadIdx.x; gi < groups; gi += NTT_THREADS) {
+        Group<R, Q0> g(gi, logC);
+        T x[M];
+#pragma unroll
+        for (uint32_t m = 0; m < M; ++m) x[m] = ld(g.c, g.tb + (m << Q0));
+        butterflies<F, R, Q0, DIT>(x, tws + (g.b & ((1u << Q0) - 1)));
+#pragma unroll
+        for (uint32_t m = 0; m < M; ++m) st(g.c, g.tb + (m << Q0), x[m]);
+    }
+}
+
+template <class F, int MODE, int S>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A) {
     using T = typename F::T;
+    constexpr uint32_t STRIDE = Tile<S>::STRIDE;
     constexpr uint32_t TILE = TILE_BYTES / sizeof(T);
-    __shared__ T lds[TILE];
+    constexpr uint32_t TW_LDS = tw_lds<T>();
+    // the host never plans a pass wider than tile_log - 4 bits for this field
+    if constexpr (uint32_t(S) > tile_log<T>() - 4) {
+        return;
+    } else {
+    // column-major exchange tile with one padding word per column; C <= TILE / 16
+    __shared__ T lds[TILE + TILE / 16];
+    __shared__ T tws[(MODE == INNER_LDE ? 2 : 1) * TW_LDS];
+    // rounds: low = bits [0, R0), high = bits [R0, S)
+    constexpr int R0 = S <= 5 ? S : (S + 1) / 2;
+    constexpr int R1 = S - R0;
 
-    const uint32_t s = A.s, a = A.a, logG = A.logG, logC = A.logC;
-    const uint32_t G = 1u << logG, C = 1u << logC;
-    const uint32_t logNLow = logG - logC;
-    const uint32_t B = 1u << s;
-    const uint32_t elems = B << logG;
+    const uint32_t a = A.a, logC = A.logC;
 
-    // block -> (column tile, low tile, high index, region)
+    // block -> (column tile, low index, high index, region)
     uint32_t bid = blockIdx.x;
     const uint32_t ct = bid % A.n_col_tiles;
     bid /= A.n_col_tiles;
-    const uint32_t low_tiles_log = a - logNLow;  // 2^a / NLow
-    const uint32_t lt = bid & ((1u << low_tiles_log) - 1);
-    bid >>= low_tiles_log;
-    const uint32_t hi_bits = A.L - a - s;
+    const uint32_t lo0 = bid & ((1u << a) - 1);
+    bid >>= a;
+    const uint32_t hi_bits = A.L - a - S;
     const uint32_t hi = bid & ((1u << hi_bits) - 1);
-    const uint32_t region = bid >> hi_bits;
-    const uint32_t lo0 = lt << logNLow;
+    const uint32_t region = (MODE == INNER_LDE) ? 0u : (bid >> hi_bits);
     const uint32_t c0 = ct << logC;
     const uint64_t n = 1ull << A.L;
+    const uint32_t w = A.w;
 
-    // position (within the length-n transform) of tile element (t, g)
-    auto pos = [&](uint32_t t, uint32_t g) -> uint64_t {
-        return (uint64_t(hi) << (a + s)) | (uint64_t(t) << a) | (lo0 + (g >> logC));
-    };
+    // position (within the length-n transform) of tile row t
+    auto pos = [&](uint32_t t) -> uint64_t { return (uint64_t(hi) << (a + S)) | (uint64_t(t) << a) | lo0; };
 
-    // ---- load
-    const T* src = A.src + uint64_t(region) * A.src_region_rows * A.w;
-    for (uint32_t idx = threadIdx.x; idx < elems; idx += NTT_THREADS) {
-        uint32_t t = idx >> logG, g = idx & (G - 1);
-        uint32_t col = c0 + (g & (C - 1));
-        T v = 0;
-        if (col < A.w) {
-            uint64_t p = pos(t, g);
-            if (MODE == INNER_DIT_GATHER) {
-                uint64_t row = bitrev(uint32_t(p), A.L);
-                v = src[row * A.w + col];
-                if (A.scale) v = F::mul_c(v, A.scale[row]);
-            } else {
-                v = src[p * A.w + col];
-            }
-        }
-        lds[idx] = v;
-    }
-
-    if (MODE == OUTER_DIF_INV || MODE == INNER_LDE || MODE == INNER_DIF_SCATTER) {
-        for (int q = int(s) - 1; q >= 0; --q) {
-            __syncthreads();
-            stage<F, false>(lds, A.tw, A.L, a, uint32_t(q), s, logG, logC, lo0);
-        }
-    }
-    if (MODE == OUTER_DIT_FWD || MODE == INNER_DIT_GATHER) {
-        for (uint32_t q = 0; q < s; ++q) {
-            __syncthreads();
-            stage<F, true>(lds, A.tw, A.L, a, q, s, logG, logC, lo0);
-        }
-    }
-
-    if (MODE == INNER_LDE) {
-        // coefficients (bit-reversed order) -> registers, then one forward
-        // transform per coset.  A.scale holds, per coset r and position p,
-        // (shift * omega_N^r)^bitrev(p) / n in multiply-constant form.
-        // A.tw is the inverse table; the forward table follows it at + n/2.
-        constexpr uint32_t PER = TILE / NTT_THREADS;
-        T coef[PER];
+    if constexpr (S > 0) {
+        build_twiddles(tws, A.tw, A.L, a, S, lo0);
+        if (MODE == INNER_LDE) build_twiddles(tws + TW_LDS, A.tw + (n >> 1), A.L, a, S, lo0);
         __syncthreads();
-#pragma unroll
-        for (uint32_t k = 0; k < PER; ++k) {
-            uint32_t idx = threadIdx.x + k * NTT_THREADS;
-            coef[k] = idx < elems ? lds[idx] : T(0);
+    }
+
+    const T* src = A.src + uint64_t(region) * A.src_region_rows * w;
+    T* dst = A.dst + uint64_t(region) * A.dst_region_rows * w;
+    auto gload = [&](uint32_t c, uint32_t t) -> T {
+        const uint32_t col = c0 + c;
+        if (col >= w) return T(0);
+        const uint64_t p = pos(t);
+        if (MODE == INNER_DIT_GATHER) {
+            const uint64_t row = bitrev(uint32_t(p), A.L);
+            T v = src[row * w + col];
+            return A.scale ? F::mul_c(v, A.scale[row]) : v;
         }
-        const T* fwd = A.tw + (n >> 1);
-        for (int r = int(A.n_regions) - 1; r >= 0; --r) {
+        return src[p * w + col];
+    };
+    auto gstore = [&](uint32_t c, uint32_t t, T v) {
+        const uint32_t col = c0 + c;
+        if (col >= w) return;
+        const uint64_t p = pos(t);
+        if (MODE == INNER_DIF_SCATTER) {
+            const uint64_t row = bitrev(uint32_t(p), A.L);
+            dst[row * w + col] = A.scale ? F::mul_c(v, A.scale[row]) : v;
+        } else {
+            dst[p * w + col] = v;
+        }
+    };
+    auto lload = [&](uint32_t c, uint32_t t) -> T { return lds[c * STRIDE + t]; };
+    auto lstore = [&](uint32_t c, uint32_t t, T v) { lds[c * STRIDE + t] = v; };
+
+    if constexpr (MODE == INNER_LDE) {
+        // DIF over all S bits, then per coset: twist, DIT over all S bits.
+        // The DIF low round and the DIT low round group the same positions, so
+        // the coefficients stay in that round's registers across the cosets.
+        // The host keeps (2^S >> R0) * C <= NTT_THREADS: one low group per thread.
+        if constexpr (R1 > 0) {
+            round_io<F, S, R1, R0, false>(tws, logC, gload, lstore);
             __syncthreads();
-            const T* tws = A.scale + uint64_t(r) * n;
+        }
+        constexpr uint32_t M = 1u << R0;
+        const uint32_t groups = ((1u << S) >> R0) << logC;
+        const bool active = threadIdx.x < groups;
+        Group<R0, 0> g(threadIdx.x, logC);
+        T x[M];
+        if (active) {
 #pragma unroll
-            for (uint32_t k = 0; k < PER; ++k) {
-                uint32_t idx = threadIdx.x + k * NTT_THREADS;
-                if (idx < elems) {
-                    uint32_t t = idx >> logG;
-                    lds[idx] = F::mul_c(coef[k], tws[pos(t, idx & (G - 1))]);
+            for (uint32_t m = 0; m < M; ++m) x[m] = R1 > 0 ? lload(g.c, g.tb + m) : gload(g.c, g.tb + m);
+            butterflies<F, R0, 0, false>(x, tws);
+        }
+        const T* twf = tws + TW_LDS;
+        for (int r = int(A.n_regions) - 1; r >= 0; --r) {
+            const T* twist = A.scale + uint64_t(r) * n;
+            T* dst_r = A.dst + uint64_t(r) * A.dst_region_rows * w;
+            auto store_r = [&](uint32_t c, uint32_t t, T v) {
+                const uint32_t col = c0 + c;
+                if (col < w) dst_r[pos(t) * w + col] = v;
+            };
+            if constexpr (R1 > 0) __syncthreads();  // previous coset's high round has read the tile
+            if (active) {
+                T y[M];
+#pragma unroll
+                for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_c(x[m], twist[pos(g.tb + m)]);
+                butterflies<F, R0, 0, true>(y, twf);
+#pragma unroll
+                for (uint32_t m = 0; m < M; ++m) {
+                    if constexpr (R1 > 0) lstore(g.c, g.tb + m, y[m]);
+                    else store_r(g.c, g.tb + m, y[m]);
                 }
             }
-            for (uint32_t q = 0; q < s; ++q) {
+            if constexpr (R1 > 0) {
                 __syncthreads();
-                stage<F, true>(lds, fwd, A.L, 0, q, s, logG, logC, lo0);
-            }
-            __syncthreads();
-            T* dst = A.dst + uint64_t(r) * A.dst_region_rows * A.w;
-            for (uint32_t idx = threadIdx.x; idx < elems; idx += NTT_THREADS) {
-                uint32_t t = idx >> logG, g = idx & (G - 1);
-                uint32_t col = c0 + (g & (C - 1));
-                if (col < A.w) dst[pos(t, g) * A.w + col] = lds[idx];
+                round_io<F, S, R1, R0, true>(twf, logC, lload, store_r);
             }
         }
         return;
-    }
-
-    // ---- store
-    __syncthreads();
-    T* dst = A.dst + uint64_t(region) * A.dst_region_rows * A.w;
-    for (uint32_t idx = threadIdx.x; idx < elems; idx += NTT_THREADS) {
-        uint32_t t = idx >> logG, g = idx & (G - 1);
-        uint32_t col = c0 + (g & (C - 1));
-        if (col >= A.w) continue;
-        uint64_t p = pos(t, g);
-        T v = lds[idx];
-        if (MODE == INNER_DIF_SCATTER) {
-            uint64_t row = bitrev(uint32_t(p), A.L);
-            if (A.scale) v = F::mul_c(v, A.scale[row]);
-            dst[row * A.w + col] = v;
+    } else if constexpr (MODE == OUTER_DIT_FWD || MODE == INNER_DIT_GATHER) {
+        if constexpr (R1 > 0) {
+            round_io<F, S, R0, 0, true>(tws, logC, gload, lstore);
+            __syncthreads();
+            round_io<F, S, R1, R0, true>(tws, logC, lload, gstore);
         } else {
-            dst[p * A.w + col] = v;
+            round_io<F, S, R0, 0, true>(tws, logC, gload, gstore);
+        }
+    } else {
+        if constexpr (R1 > 0) {
+            round_io<F, S, R1, R0, false>(tws, logC, gload, lstore);
+            __syncthreads();
+            round_io<F, S, R0, 0, false>(tws, logC, lload, gstore);
+        } else {
+            round_io<F, S, R0, 0, false>(tws, logC, gload, gstore);
         }
     }
+    }
 }
-
 // table[i] = base^i * mult for i < count (multiply-constant form of each value)
 template <class F>
 __global__ void powers_kernel(typename F::T* out, uint64_t count, typename F::T base_c,
@@ -267,47 +355,71 @@ __global__ void coset_to_natural_kernel(const T* in, T* out, uint32_t L, uint32_
 
 // ------------------------------------------------------------------ host ----
 
+// Pass widths for an L-bit transform: the inner pass takes the low bits, the
+// outer passes split the rest as evenly as possible (top pass first).
 struct Plan {
     uint32_t s_inner;
-    uint32_t outer[4];  // outer pass widths, top (for DIF order) first
+    uint32_t outer[4];
     uint32_t n_outer;
-    uint32_t logC;
 };
 
 template <class F>
-Plan make_plan(uint32_t L, uint32_t w, const NttTuning& tune) {
+Plan make_plan(uint32_t L, const NttTuning& tune) {
     using T = typename F::T;
-    const uint32_t tile_log = 31 - __builtin_clz(uint32_t(TILE_BYTES / sizeof(T)));
-    uint32_t logC = 0;
-    uint32_t cmax = tune.log_cols ? tune.log_cols : 4;
-    while (logC < cmax && (1u << logC) < w) ++logC;
-    Plan p{};
-    p.logC = logC;
-    uint32_t smax = tile_log - logC;  // inner pass: G = C
+    uint32_t smax = tile_log<T>() - 4;  // keep at least 16 columns per tile
+    if (smax > MAX_STAGES) smax = MAX_STAGES;
     if (tune.max_stages && tune.max_stages < smax) smax = tune.max_stages;
+    Plan p{};
     p.s_inner = L < smax ? L : smax;
     uint32_t rest = L - p.s_inner;
-    uint32_t outer_max = tile_log - 4;  // outer passes keep at least 16 sub-transforms
-    if (tune.max_stages && tune.max_stages < outer_max) outer_max = tune.max_stages;
-    uint32_t np = (rest + outer_max - 1) / outer_max;
+    uint32_t np = (rest + smax - 1) / smax;
     p.n_outer = np;
     for (uint32_t i = 0; i < np; ++i) p.outer[i] = rest / np + (i < rest % np ? 1 : 0);
     return p;
 }
 
+// columns per tile: as many as the LDS tile holds at this pass width (capped
+// by the tuning knob and by the matrix width rounded up to a power of two)
+template <class T>
+uint32_t cols_log(uint32_t s, uint32_t w, const NttTuning& tune) {
+    uint32_t lc = tile_log<T>() - (s < 4 ? 4 : s);
+    if (tune.log_cols && tune.log_cols < lc) lc = tune.log_cols;
+    uint32_t need = 0;
+    while ((1u << need) < w) ++need;
+    return lc < need ? lc : need;
+}
+
+template <class F, int MODE, int S>
+void launch_s(NttArgs<typename F::T> A, hipStream_t st) {
+    uint64_t blocks = uint64_t(A.n_col_tiles) << (A.L - S);
+    if (MODE != INNER_LDE) blocks *= A.n_regions;
+    hipLaunchKernelGGL((ntt_pass<F, MODE, S>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, A);
+}
+
 template <class F, int MODE>
-void launch_pass(NttArgs<typename F::T> A, hipStream_t st) {
+void launch_pass(NttArgs<typename F::T> A, uint32_t s, const NttTuning& tune, hipStream_t st) {
     using T = typename F::T;
-    const uint32_t tile_log = 31 - __builtin_clz(uint32_t(TILE_BYTES / sizeof(T)));
-    uint32_t logG = tile_log - A.s;
-    uint32_t max_logG = A.logC + A.a;  // NLow <= 2^a
-    if (logG > max_logG) logG = max_logG;
-    if (logG < A.logC) logG = A.logC;
-    A.logG = logG;
+    A.logC = cols_log<T>(s, A.w, tune);
+    if (MODE == INNER_LDE) {
+        // one low-round group per thread: (2^s >> R0) * C <= NTT_THREADS
+        const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;
+        const uint32_t max_lc = 9 + r0 - s;  // log2(NTT_THREADS) = 9
+        if (A.logC > max_lc) A.logC = max_lc;
+    }
     A.n_col_tiles = (A.w + (1u << A.logC) - 1) >> A.logC;
-    uint64_t blocks = uint64_t(A.n_col_tiles) << (A.L - A.s - (logG - A.logC));
-    blocks *= A.n_regions;
-    hipLaunchKernelGGL((ntt_pass<F, MODE>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, A);
+    switch (s) {
+        case 0: launch_s<F, MODE, 0>(A, st); break;
+        case 1: launch_s<F, MODE, 1>(A, st); break;
+        case 2: launch_s<F, MODE, 2>(A, st); break;
+        case 3: launch_s<F, MODE, 3>(A, st); break;
+        case 4: launch_s<F, MODE, 4>(A, st); break;
+        case 5: launch_s<F, MODE, 5>(A, st); break;
+        case 6: launch_s<F, MODE, 6>(A, st); break;
+        case 7: launch_s<F, MODE, 7>(A, st); break;
+        case 8: launch_s<F, MODE, 8>(A, st); break;
+        case 9: launch_s<F, MODE, 9>(A, st); break;
+        default: launch_s<F, MODE, 10>(A, st); break;
+    }
 }
 
 }  // namespace
@@ -316,60 +428,44 @@ template <class F>
 int lde_device(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t w, uint32_t k,
                const typename F::T* tw_inv_fwd, const typename F::T* twist, const NttTuning& tune,
                hipStream_t st) {
-    Plan p = make_plan<F>(L, w, tune);
+    Plan p = make_plan<F>(L, tune);
     const uint64_t n = 1ull << L;
     NttArgs<typename F::T> A{};
     A.L = L;
     A.w = w;
-    A.logC = p.logC;
     A.n_regions = 1;
     A.tw = tw_inv_fwd;  // inverse half first
     // 1. outer DIF passes over the high bits, top down: input -> region 0 of out
     uint32_t top = L;
     const typename F::T* cur = in;
     for (uint32_t i = 0; i < p.n_outer; ++i) {
-        A.s = p.outer[i];
-        A.a = top - A.s;
+        A.a = top - p.outer[i];
         A.src = cur;
         A.dst = out;
-        A.src_region_rows = A.dst_region_rows = 0;
-        launch_pass<F, OUTER_DIF_INV>(A, st);
+        launch_pass<F, OUTER_DIF_INV>(A, p.outer[i], tune, st);
         cur = out;
         top = A.a;
     }
-    // 2. inner fused pass: rest of the inverse, then per-coset twist + DIT
+    // 2. inner fused pass: rest of the inverse, then per-coset twist + DIT;
+    //    the grid covers the blocks, the kernel loops over the cosets
     A.a = 0;
-    A.s = p.s_inner;
     A.src = cur;
     A.dst = out;
     A.scale = twist;
     A.n_regions = 1u << k;
     A.dst_region_rows = n;
-    {
-        NttArgs<typename F::T> B = A;
-        B.n_regions = 1;  // grid covers blocks only; the kernel loops over cosets
-        using T = typename F::T;
-        const uint32_t tile_log = 31 - __builtin_clz(uint32_t(TILE_BYTES / sizeof(T)));
-        B.logG = B.logC;
-        (void)tile_log;
-        B.n_col_tiles = (B.w + (1u << B.logC) - 1) >> B.logC;
-        uint64_t blocks = uint64_t(B.n_col_tiles) << (L - B.s);
-        B.n_regions = 1u << k;
-        hipLaunchKernelGGL((ntt_pass<F, INNER_LDE>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, B);
-    }
+    launch_pass<F, INNER_LDE>(A, p.s_inner, tune, st);
     // 3. outer DIT passes per coset, bottom up, in place
     A.scale = nullptr;
     A.tw = tw_inv_fwd + (n >> 1);
     A.src = out;
     A.dst = out;
     A.src_region_rows = A.dst_region_rows = n;
-    A.n_regions = 1u << k;
     uint32_t bot = p.s_inner;
     for (int i = int(p.n_outer) - 1; i >= 0; --i) {
         A.a = bot;
-        A.s = p.outer[i];
-        launch_pass<F, OUTER_DIT_FWD>(A, st);
-        bot += A.s;
+        launch_pass<F, OUTER_DIT_FWD>(A, p.outer[i], tune, st);
+        bot += p.outer[i];
     }
     return int(hipGetLastError());
 }
@@ -378,31 +474,28 @@ template <class F>
 int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F::T* tmp, uint32_t L,
                uint32_t w, const typename F::T* tw_inv_fwd, const typename F::T* row_scale,
                const NttTuning& tune, hipStream_t st) {
-    Plan p = make_plan<F>(L, w, tune);
+    Plan p = make_plan<F>(L, tune);
     const uint64_t n = 1ull << L;
     NttArgs<typename F::T> A{};
     A.L = L;
     A.w = w;
-    A.logC = p.logC;
     A.n_regions = 1;
     if (kind == 0 || kind == 2) {
         // forward (optionally coset): gather rows in bit-reversed order with
         // the row scale, DIT inner, DIT outer in place.
         A.tw = tw_inv_fwd + (n >> 1);
         A.a = 0;
-        A.s = p.s_inner;
         A.src = in;
         A.dst = out;
         A.scale = row_scale;
-        launch_pass<F, INNER_DIT_GATHER>(A, st);
+        launch_pass<F, INNER_DIT_GATHER>(A, p.s_inner, tune, st);
         A.scale = nullptr;
         A.src = out;
         uint32_t bot = p.s_inner;
         for (int i = int(p.n_outer) - 1; i >= 0; --i) {
             A.a = bot;
-            A.s = p.outer[i];
-            launch_pass<F, OUTER_DIT_FWD>(A, st);
-            bot += A.s;
+            launch_pass<F, OUTER_DIT_FWD>(A, p.outer[i], tune, st);
+            bot += p.outer[i];
         }
     } else {
         // inverse (optionally coset): DIF outer (in -> tmp), DIF inner with
@@ -411,20 +504,18 @@ int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F
         uint32_t top = L;
         const typename F::T* cur = in;
         for (uint32_t i = 0; i < p.n_outer; ++i) {
-            A.s = p.outer[i];
-            A.a = top - A.s;
+            A.a = top - p.outer[i];
             A.src = cur;
             A.dst = tmp;
-            launch_pass<F, OUTER_DIF_INV>(A, st);
+            launch_pass<F, OUTER_DIF_INV>(A, p.outer[i], tune, st);
             cur = tmp;
             top = A.a;
         }
         A.a = 0;
-        A.s = p.s_inner;
         A.src = cur;
         A.dst = out;
         A.scale = row_scale;
-        launch_pass<F, INNER_DIF_SCATTER>(A, st);
+        launch_pass<F, INNER_DIF_SCATTER>(A, p.s_inner, tune, st);
     }
     return int(hipGetLastError());
 }
