@@ -166,8 +166,8 @@ size_t size_class(size_t bytes) {
 }
 
 void trim(p3h_ctx* ctx) {
-    hipStreamSynchronize(ctx->stream);
-    for (auto& kv : ctx->free_blocks) hipFree(kv.second);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
     ctx->free_blocks.clear();
     ctx->cached = 0;
 }
@@ -224,15 +224,15 @@ struct Phase {
     hipEvent_t a = nullptr;
     Phase(p3h_ctx* c, const char* n) : ctx(c), name(n) {
         if (ctx->profiling) {
-            hipEventCreate(&a);
-            hipEventRecord(a, ctx->stream);
+            (void)hipEventCreate(&a);
+            (void)hipEventRecord(a, ctx->stream);
         }
     }
     ~Phase() {
         if (ctx->profiling) {
             hipEvent_t b;
-            hipEventCreate(&b);
-            hipEventRecord(b, ctx->stream);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(b, ctx->stream);
             ctx->pending.push_back({name, a, b});
         }
     }
@@ -240,14 +240,14 @@ struct Phase {
 
 void resolve_pending(p3h_ctx* ctx) {
     for (auto& p : ctx->pending) {
-        hipEventSynchronize(p.b);
+        (void)hipEventSynchronize(p.b);
         float ms = 0;
-        hipEventElapsedTime(&ms, p.a, p.b);
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
         auto& s = ctx->stats[p.phase];
         s.launches++;
         s.ms += ms;
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
     }
     ctx->pending.clear();
 }
@@ -791,15 +791,15 @@ int p3h_ctx_create(int device, uint64_t budget, p3h_ctx** out) {
 
 int p3h_ctx_destroy(p3h_ctx* ctx) {
     if (!ctx) return P3H_EINVAL;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     resolve_pending(ctx);
     ctx->twiddles.clear();
     ctx->twists.clear();
     ctx->row_scales.clear();
     ctx->fri_tinv.clear();
     trim(ctx);
-    hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return P3H_OK;
 }
