@@ -37,6 +37,7 @@ typedef struct p3h_ctx p3h_ctx;
 typedef struct p3h_mat p3h_mat;   /* device-resident matrix */
 typedef struct p3h_tree p3h_tree; /* device-resident Merkle tree (prover data) */
 typedef struct p3h_fri p3h_fri;   /* device-resident FRI commit-phase data */
+typedef struct p3h_batch p3h_batch; /* planned batched commit */
 typedef struct p3h_serve p3h_serve; /* serving queue */
 
 enum { P3H_BABYBEAR = 0, P3H_GOLDILOCKS = 1 };
@@ -90,6 +91,8 @@ int p3h_mat_info(const p3h_mat* m, int* field, uint64_t* height, uint32_t* width
 /* download rows [row0, row0 + nrows) in natural order */
 int p3h_mat_download_rows(p3h_ctx* ctx, const p3h_mat* m, uint64_t row0, uint64_t nrows, void* host);
 int p3h_mat_free(p3h_mat* m);
+/* overwrite a natural-order matrix with new host contents (same shape) */
+int p3h_mat_write(p3h_ctx* ctx, p3h_mat* m, const void* host);
 
 /* ------------------------------------------------------------------- DFT */
 /* Replaces NttCuda<F> / Radix2Dit<F> (ntt_cuda.hpp:310-378, radix2_dit.hpp:82-233):
@@ -164,6 +167,30 @@ int p3h_pcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, const uint64_t* shi
  * p3h_mmcs_commit over the individual p3h_coset_lde outputs. */
 int p3h_batch_commit(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits,
                      uint32_t n_traces, uint64_t shift, uint32_t cap_height, p3h_tree** out);
+
+/* Planned batched commit: every buffer is allocated and every table uploaded
+ * once; the LDE passes of all traces run as work-list launches grouped by pass
+ * shape, and with use_graph != 0 the whole commit (LDE passes, leaf hashing,
+ * tree levels with injection) is captured as one HIP graph and replayed by
+ * p3h_batch_run on the traces' current contents.  No counterpart in the
+ * reference.  cap_out (nullable): 2^cap_height digests. */
+int p3h_batch_plan(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n_traces,
+                   uint64_t shift, uint32_t cap_height, int use_graph, p3h_batch** out);
+int p3h_batch_run(p3h_batch* b, void* cap_out);
+/* tree of the last run (borrowed; overwritten by the next run) */
+int p3h_batch_tree(const p3h_batch* b, const p3h_tree** out);
+int p3h_batch_free(p3h_batch* b);
+
+/* ------------------------------------------------------------ autotuner */
+/* Time the LDE of one shape under candidate pass shapes (columns per LDS tile,
+ * stages per pass) and record the fastest for that (field, log_n, width,
+ * added_bits); later LDEs and batch plans of that shape use it.  No
+ * counterpart in the reference. */
+int p3h_autotune_lde(p3h_ctx* ctx, int field, uint32_t log_n, uint32_t width, uint32_t added_bits,
+                     uint32_t iters, uint32_t* best_log_cols, uint32_t* best_max_stages, double* best_ms);
+/* the recorded table as text, one "field log_n width added log_cols max_stages" per line */
+int p3h_tuning_save(const p3h_ctx* ctx, const char* path);
+int p3h_tuning_load(p3h_ctx* ctx, const char* path);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "field.h"
 
 namespace p3h {
@@ -13,6 +15,59 @@ struct NttTuning {
     uint32_t log_cols;    // log2 columns per tile (default 4)
     uint32_t max_stages;  // cap on radix-2 stages per LDS pass (default: tile-limited)
 };
+
+// Arguments of one NTT pass over a row-major matrix (see ntt.hip).
+template <class T>
+struct NttArgs {
+    const T* src;
+    T* dst;
+    const T* tw;     // twiddles omega^(+-e), e < n/2, in multiply-constant form
+    const T* scale;  // mode-dependent table, or nullptr
+    uint64_t src_region_rows;  // row offset between regions in src
+    uint64_t dst_region_rows;  // row offset between regions in dst
+    uint32_t L;                // log2 transform size n
+    uint32_t a;                // the pass covers index bits [a, a+S)
+    uint32_t w;                // matrix width (row stride in elements)
+    uint32_t logC;             // columns per tile = 2^logC
+    uint32_t n_col_tiles;
+    uint32_t n_regions;        // cosets (LDE) or 1
+};
+
+// One descriptor of a work-list launch; block_start = first block of it.
+template <class T>
+struct NttWork {
+    NttArgs<T> args;
+    uint32_t block_start;
+    uint32_t pad;
+};
+
+// One coset LDE of a batch (same meaning as lde_device's arguments).
+template <class T>
+struct LdeJob {
+    const T* in;
+    T* out;
+    uint32_t L, w, k;
+    const T* tw;
+    const T* twist;
+    NttTuning tune;
+};
+
+// All passes of a batch that share mode and width run as one launch.
+template <class T>
+struct LdePassGroup {
+    int mode;
+    uint32_t s;
+    uint32_t blocks;
+    std::vector<NttWork<T>> work;
+};
+
+// Plan the passes of many LDEs: groups in execution order.
+template <class F>
+std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob<typename F::T>>& jobs);
+
+// Launch one group; d_work holds g.work in device memory.
+template <class F>
+int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typename F::T>* d_work, hipStream_t st);
 
 // Coset LDE, coset-major output: out[(r * n + i) * w + c] = LDE row (i << k | r).
 // tw_inv_fwd: [omega_n^-e for e < n/2][omega_n^e for e < n/2] (multiply-constant form);

@@ -51,22 +51,6 @@ constexpr uint32_t tw_lds() {
     return 1u << (tile_log<T>() - 4);
 }
 
-template <class T>
-struct NttArgs {
-    const T* src;
-    T* dst;
-    const T* tw;     // twiddles omega^(+-e), e < n/2, in multiply-constant form
-    const T* scale;  // mode-dependent table (see kernel), or nullptr
-    uint64_t src_region_rows;  // row offset between regions in src
-    uint64_t dst_region_rows;  // row offset between regions in dst
-    uint32_t L;                // log2 transform size n
-    uint32_t a;                // this pass covers index bits [a, a+S)
-    uint32_t w;                // matrix width (row stride in elements)
-    uint32_t logC;             // columns per tile = 2^logC
-    uint32_t n_col_tiles;
-    uint32_t n_regions;        // cosets (LDE) or 1
-};
-
 enum Mode : int {
     OUTER_DIF_INV = 0,     // DIF stages, bits descending
     OUTER_DIT_FWD = 1,     // DIT stages, bits ascending, per region
@@ -169,7 +153,7 @@ __device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC
 }
 
 template <class F, int MODE, int S>
-__global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A) {
+__device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, uint32_t block) {
     using T = typename F::T;
     constexpr uint32_t STRIDE = Tile<S>::STRIDE;
     constexpr uint32_t TILE = TILE_BYTES / sizeof(T);
@@ -188,7 +172,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A
     const uint32_t a = A.a, logC = A.logC;
 
     // block -> (column tile, low index, high index, region)
-    uint32_t bid = blockIdx.x;
+    uint32_t bid = block;
     const uint32_t ct = bid % A.n_col_tiles;
     bid /= A.n_col_tiles;
     const uint32_t lo0 = bid & ((1u << a) - 1);
@@ -300,6 +284,28 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A
     }
     }
 }
+template <class F, int MODE, int S>
+__global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A) {
+    ntt_pass_body<F, MODE, S>(A, blockIdx.x);
+}
+
+// Work-list launch: many transforms (one per descriptor) that share MODE and S
+// run in one grid; block_start[i] is the first block of descriptor i
+// (ascending, block_start[count] = total).
+template <class F, int MODE, int S>
+__global__ __launch_bounds__(NTT_THREADS) void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count) {
+    // wave-uniform binary search for the descriptor owning this block
+    uint32_t lo = 0, hi = count;
+    const uint32_t b = blockIdx.x;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (work[mid].block_start <= b) lo = mid;
+        else hi = mid;
+    }
+    const NttArgs<typename F::T> A = work[lo].args;
+    ntt_pass_body<F, MODE, S>(A, b - work[lo].block_start);
+}
+
 // table[i] = base^i * mult for i < count (multiply-constant form of each value)
 template <class F>
 __global__ void powers_kernel(typename F::T* out, uint64_t count, typename F::T base_c,
@@ -389,37 +395,66 @@ uint32_t cols_log(uint32_t s, uint32_t w, const NttTuning& tune) {
     return lc < need ? lc : need;
 }
 
-template <class F, int MODE, int S>
-void launch_s(NttArgs<typename F::T> A, hipStream_t st) {
-    uint64_t blocks = uint64_t(A.n_col_tiles) << (A.L - S);
-    if (MODE != INNER_LDE) blocks *= A.n_regions;
-    hipLaunchKernelGGL((ntt_pass<F, MODE, S>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, A);
-}
-
-template <class F, int MODE>
-void launch_pass(NttArgs<typename F::T> A, uint32_t s, const NttTuning& tune, hipStream_t st) {
-    using T = typename F::T;
+// Fill the tile geometry of a pass; returns its block count.
+template <class T>
+uint64_t finalize_args(NttArgs<T>& A, int mode, uint32_t s, const NttTuning& tune) {
     A.logC = cols_log<T>(s, A.w, tune);
-    if (MODE == INNER_LDE) {
+    if (mode == INNER_LDE) {
         // one low-round group per thread: (2^s >> R0) * C <= NTT_THREADS
         const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;
         const uint32_t max_lc = 9 + r0 - s;  // log2(NTT_THREADS) = 9
         if (A.logC > max_lc) A.logC = max_lc;
     }
     A.n_col_tiles = (A.w + (1u << A.logC) - 1) >> A.logC;
+    uint64_t blocks = uint64_t(A.n_col_tiles) << (A.L - s);
+    if (mode != INNER_LDE) blocks *= A.n_regions;
+    return blocks;
+}
+
+template <class F, int MODE, int S>
+void launch_s(const NttArgs<typename F::T>& A, uint64_t blocks, const NttWork<typename F::T>* work, uint32_t count,
+              hipStream_t st) {
+    if (work)
+        hipLaunchKernelGGL((ntt_pass_list<F, MODE, S>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, work,
+                           count);
+    else
+        hipLaunchKernelGGL((ntt_pass<F, MODE, S>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, A);
+}
+
+template <class F, int MODE>
+void launch_mode(uint32_t s, const NttArgs<typename F::T>& A, uint64_t blocks, const NttWork<typename F::T>* work,
+                 uint32_t count, hipStream_t st) {
     switch (s) {
-        case 0: launch_s<F, MODE, 0>(A, st); break;
-        case 1: launch_s<F, MODE, 1>(A, st); break;
-        case 2: launch_s<F, MODE, 2>(A, st); break;
-        case 3: launch_s<F, MODE, 3>(A, st); break;
-        case 4: launch_s<F, MODE, 4>(A, st); break;
-        case 5: launch_s<F, MODE, 5>(A, st); break;
-        case 6: launch_s<F, MODE, 6>(A, st); break;
-        case 7: launch_s<F, MODE, 7>(A, st); break;
-        case 8: launch_s<F, MODE, 8>(A, st); break;
-        case 9: launch_s<F, MODE, 9>(A, st); break;
-        default: launch_s<F, MODE, 10>(A, st); break;
+        case 0: launch_s<F, MODE, 0>(A, blocks, work, count, st); break;
+        case 1: launch_s<F, MODE, 1>(A, blocks, work, count, st); break;
+        case 2: launch_s<F, MODE, 2>(A, blocks, work, count, st); break;
+        case 3: launch_s<F, MODE, 3>(A, blocks, work, count, st); break;
+        case 4: launch_s<F, MODE, 4>(A, blocks, work, count, st); break;
+        case 5: launch_s<F, MODE, 5>(A, blocks, work, count, st); break;
+        case 6: launch_s<F, MODE, 6>(A, blocks, work, count, st); break;
+        case 7: launch_s<F, MODE, 7>(A, blocks, work, count, st); break;
+        case 8: launch_s<F, MODE, 8>(A, blocks, work, count, st); break;
+        case 9: launch_s<F, MODE, 9>(A, blocks, work, count, st); break;
+        default: launch_s<F, MODE, 10>(A, blocks, work, count, st); break;
     }
+}
+
+template <class F>
+void launch_any(int mode, uint32_t s, const NttArgs<typename F::T>& A, uint64_t blocks,
+                const NttWork<typename F::T>* work, uint32_t count, hipStream_t st) {
+    switch (mode) {
+        case OUTER_DIF_INV: launch_mode<F, OUTER_DIF_INV>(s, A, blocks, work, count, st); break;
+        case OUTER_DIT_FWD: launch_mode<F, OUTER_DIT_FWD>(s, A, blocks, work, count, st); break;
+        case INNER_LDE: launch_mode<F, INNER_LDE>(s, A, blocks, work, count, st); break;
+        case INNER_DIT_GATHER: launch_mode<F, INNER_DIT_GATHER>(s, A, blocks, work, count, st); break;
+        default: launch_mode<F, INNER_DIF_SCATTER>(s, A, blocks, work, count, st); break;
+    }
+}
+
+template <class F, int MODE>
+void launch_pass(NttArgs<typename F::T> A, uint32_t s, const NttTuning& tune, hipStream_t st) {
+    uint64_t blocks = finalize_args(A, MODE, s, tune);
+    launch_mode<F, MODE>(s, A, blocks, nullptr, 0, st);
 }
 
 }  // namespace
@@ -546,6 +581,93 @@ int coset_to_natural_device(const T* in, T* out, uint32_t L, uint32_t k, uint32_
     hipLaunchKernelGGL((coset_to_natural_kernel<T>), dim3(blocks), dim3(256), 0, st, in, out, L, k, w);
     return int(hipGetLastError());
 }
+
+
+template <class F>
+std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob<typename F::T>>& jobs) {
+    using T = typename F::T;
+    struct Pass {
+        int mode;
+        uint32_t s;
+        int slot;  // order: DIF at negative slots (farthest first), inner 0, DIT positive
+        NttArgs<T> A;
+        NttTuning tune;
+    };
+    std::vector<Pass> passes;
+    for (const auto& j : jobs) {
+        Plan p = make_plan<F>(j.L, j.tune);
+        const uint64_t n = 1ull << j.L;
+        NttArgs<T> A{};
+        A.L = j.L;
+        A.w = j.w;
+        A.n_regions = 1;
+        A.tw = j.tw;
+        uint32_t top = j.L;
+        const T* cur = j.in;
+        for (uint32_t i = 0; i < p.n_outer; ++i) {
+            A.a = top - p.outer[i];
+            A.src = cur;
+            A.dst = j.out;
+            passes.push_back({OUTER_DIF_INV, p.outer[i], -int(p.n_outer - i), A, j.tune});
+            cur = j.out;
+            top = A.a;
+        }
+        A.a = 0;
+        A.src = cur;
+        A.dst = j.out;
+        A.scale = j.twist;
+        A.n_regions = 1u << j.k;
+        A.dst_region_rows = n;
+        passes.push_back({INNER_LDE, p.s_inner, 0, A, j.tune});
+        A.scale = nullptr;
+        A.tw = j.tw + (n >> 1);
+        A.src = j.out;
+        A.dst = j.out;
+        A.src_region_rows = A.dst_region_rows = n;
+        uint32_t bot = p.s_inner;
+        for (int i = int(p.n_outer) - 1; i >= 0; --i) {
+            A.a = bot;
+            passes.push_back({OUTER_DIT_FWD, p.outer[i], int(p.n_outer) - i, A, j.tune});
+            bot += p.outer[i];
+        }
+    }
+    std::vector<LdePassGroup<T>> groups;
+    int lo = 0, hi = 0;
+    for (const auto& ps : passes) {
+        lo = ps.slot < lo ? ps.slot : lo;
+        hi = ps.slot > hi ? ps.slot : hi;
+    }
+    for (int slot = lo; slot <= hi; ++slot)
+        for (uint32_t s = 0; s <= MAX_STAGES; ++s) {
+            LdePassGroup<T> g;
+            g.mode = slot < 0 ? OUTER_DIF_INV : (slot == 0 ? INNER_LDE : OUTER_DIT_FWD);
+            g.s = s;
+            g.blocks = 0;
+            for (const auto& ps : passes) {
+                if (ps.slot != slot || ps.s != s) continue;
+                NttWork<T> wk{};
+                wk.args = ps.A;
+                uint64_t b = finalize_args(wk.args, ps.mode, ps.s, ps.tune);
+                wk.block_start = g.blocks;
+                g.blocks += uint32_t(b);
+                g.work.push_back(wk);
+            }
+            if (!g.work.empty()) groups.push_back(std::move(g));
+        }
+    return groups;
+}
+
+template <class F>
+int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typename F::T>* d_work, hipStream_t st) {
+    if (g.blocks == 0) return 0;
+    launch_any<F>(g.mode, g.s, g.work[0].args, g.blocks, d_work, uint32_t(g.work.size()), st);
+    return int(hipGetLastError());
+}
+
+template std::vector<LdePassGroup<uint32_t>> plan_lde_batch<BabyBear>(const std::vector<LdeJob<uint32_t>>&);
+template std::vector<LdePassGroup<uint64_t>> plan_lde_batch<Goldilocks>(const std::vector<LdeJob<uint64_t>>&);
+template int launch_lde_group<BabyBear>(const LdePassGroup<uint32_t>&, const NttWork<uint32_t>*, hipStream_t);
+template int launch_lde_group<Goldilocks>(const LdePassGroup<uint64_t>&, const NttWork<uint64_t>*, hipStream_t);
 
 template int lde_device<BabyBear>(const uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t, const uint32_t*,
                                   const uint32_t*, const NttTuning&, hipStream_t);

@@ -8,8 +8,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <atomic>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -135,7 +137,10 @@ struct p3h_ctx {
     std::map<std::tuple<int, uint32_t, int, uint64_t>, DevPtr> row_scales;     // (field, L, kind, shift)
     std::map<uint32_t, DevPtr> fri_tinv;                                        // LH
     NttTuning tune{};
+    // autotuned pass shapes per (field, log_n, width, added bits)
+    std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, NttTuning> tuned;
     bool profiling = false;
+    bool capturing = false;
     std::map<std::string, PhaseStat> stats;
     struct Pending {
         std::string phase;
@@ -223,13 +228,13 @@ struct Phase {
     std::string name;
     hipEvent_t a = nullptr;
     Phase(p3h_ctx* c, const char* n) : ctx(c), name(n) {
-        if (ctx->profiling) {
+        if (ctx->profiling && !ctx->capturing) {
             (void)hipEventCreate(&a);
             (void)hipEventRecord(a, ctx->stream);
         }
     }
     ~Phase() {
-        if (ctx->profiling) {
+        if (a) {
             hipEvent_t b;
             (void)hipEventCreate(&b);
             (void)hipEventRecord(b, ctx->stream);
@@ -420,6 +425,11 @@ const typename H::T* natural_data(p3h_ctx* ctx, const p3h_mat* m, DevPtr& tmp) {
 }
 
 // ------------------------------------------------------------------ LDE ---
+NttTuning tuning_for(const p3h_ctx* ctx, int field, uint32_t L, uint32_t w, uint32_t k) {
+    auto it = ctx->tuned.find(std::make_tuple(field, L, w, k));
+    return it == ctx->tuned.end() ? ctx->tune : it->second;
+}
+
 template <class H>
 p3h_mat* do_lde(p3h_ctx* ctx, const p3h_mat* in, uint32_t k, uint64_t shift64) {
     using T = typename H::T;
@@ -436,7 +446,8 @@ p3h_mat* do_lde(p3h_ctx* ctx, const p3h_mat* in, uint32_t k, uint64_t shift64) {
     out->L = L;
     try {
         Phase ph(ctx, "lde");
-        launch_check(lde_device<typename H::F>(src, out->data<T>(), L, in->width, k, tw, twist, ctx->tune, ctx->stream),
+        launch_check(lde_device<typename H::F>(src, out->data<T>(), L, in->width, k, tw, twist,
+                                               tuning_for(ctx, H::FIELD, L, in->width, k), ctx->stream),
                      "lde");
     } catch (...) {
         mat_unref(out);
@@ -483,11 +494,18 @@ DevPtr upload_segs(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs)
     return d;
 }
 
+// A recorded device launch on the context stream (throws on launch errors).
+using LaunchFn = std::function<void(p3h_ctx*)>;
+using LaunchList = std::vector<std::pair<const char*, LaunchFn>>;
+
+// Multi-height tree: allocate every layer and upload the segment tables now,
+// return the kernel launches (leaf hashing, then one compress-with-injection
+// launch per level) for the caller to run or capture.
 template <class H>
-p3h_tree* do_mmcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uint32_t cap_height) {
+p3h_tree* prepare_mmcs(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uint32_t cap_height, LaunchList& out) {
     using T = typename H::T;
     using D = typename H::D;
-    const auto& pp = p2<H>(ctx);
+    const P2Params<T, D::W> pp = p2<H>(ctx);
     if (n == 0) throw Err{P3H_EINVAL, "no matrices to commit"};
     for (uint32_t i = 0; i < n; ++i) {
         if (!mats[i] || mats[i]->field != H::FIELD) throw Err{P3H_EINVAL, "matrix field mismatch"};
@@ -528,10 +546,13 @@ p3h_tree* do_mmcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, u
         t->seg_tables.push_back(dsegs);
         DevPtr leaves = dalloc(ctx, maxh * D::DIG * sizeof(T));
         {
-            Phase ph(ctx, "leaf_hash");
-            launch_check(hash_rows_device<D>(dptr<RowSeg<T>>(dsegs), uint32_t(top.size()), w0, top[0], maxh,
-                                             dptr<T>(leaves), pp, ctx->stream),
-                         "leaf hash");
+            const RowSeg<T>* ds = dptr<RowSeg<T>>(dsegs);
+            uint32_t ns = uint32_t(top.size());
+            RowSeg<T> s0 = top[0];
+            T* lv = dptr<T>(leaves);
+            out.emplace_back("leaf_hash", [=](p3h_ctx* c) {
+                launch_check(hash_rows_device<D>(ds, ns, w0, s0, maxh, lv, pp, c->stream), "leaf hash");
+            });
         }
         t->layers.push_back(leaves);
         t->counts.push_back(maxh);
@@ -546,14 +567,38 @@ p3h_tree* do_mmcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, u
                 di = upload_segs<H>(ctx, inj);
                 t->seg_tables.push_back(di);
             }
-            Phase ph(ctx, "compress");
-            launch_check(compress_layer_device<D>(dptr<T>(t->layers.back()), level * 2, dptr<T>(nl), level,
-                                                  di ? dptr<RowSeg<T>>(di) : nullptr, uint32_t(inj.size()), wi, pp,
-                                                  ctx->stream),
-                         "compress");
+            const T* prev = dptr<T>(t->layers.back());
+            T* nxt = dptr<T>(nl);
+            const RowSeg<T>* dinj = di ? dptr<RowSeg<T>>(di) : nullptr;
+            uint32_t ninj = uint32_t(inj.size());
+            uint64_t lv = level;
+            out.emplace_back("compress", [=](p3h_ctx* c) {
+                launch_check(compress_layer_device<D>(prev, lv * 2, nxt, lv, dinj, ninj, wi, pp, c->stream),
+                             "compress");
+            });
             t->layers.push_back(nl);
             t->counts.push_back(level);
         }
+    } catch (...) {
+        p3h_tree_free(t);
+        throw;
+    }
+    return t;
+}
+
+void run_launches(p3h_ctx* ctx, const LaunchList& ls) {
+    for (const auto& l : ls) {
+        Phase ph(ctx, l.first);
+        l.second(ctx);
+    }
+}
+
+template <class H>
+p3h_tree* do_mmcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uint32_t cap_height) {
+    LaunchList ls;
+    p3h_tree* t = prepare_mmcs<H>(ctx, mats, n, cap_height, ls);
+    try {
+        run_launches(ctx, ls);
     } catch (...) {
         p3h_tree_free(t);
         throw;
@@ -742,6 +787,108 @@ p3h_mat* view_of(p3h_mat* base, uint64_t h, uint32_t w) {
     v->buf = base->buf;
     v->offset_elems = base->offset_elems;
     return v;
+}
+
+// Batched commit of many BabyBear traces: every buffer allocated and every
+// table uploaded at planning time; the LDE passes run as work-list launches
+// grouped by (mode, pass width) and the tree as leaf hashing + one launch per
+// level.  With use_graph the whole sequence is captured once as a HIP graph.
+struct BatchPlan {
+    p3h_ctx* ctx = nullptr;
+    std::vector<p3h_mat*> traces;  // referenced
+    std::vector<p3h_mat*> ldes;    // referenced (the tree holds its own)
+    std::vector<LdePassGroup<uint32_t>> groups;
+    std::vector<DevPtr> work_tables;
+    p3h_tree* tree = nullptr;
+    LaunchList tree_launches;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void launch_all() {
+        for (size_t g = 0; g < groups.size(); ++g) {
+            Phase ph(ctx, "lde");
+            launch_check(launch_lde_group<BabyBear>(groups[g], dptr<NttWork<uint32_t>>(work_tables[g]), ctx->stream),
+                         "batched lde");
+        }
+        run_launches(ctx, tree_launches);
+    }
+    ~BatchPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (tree) p3h_tree_free(tree);
+        for (auto* m : ldes) mat_unref(m);
+        for (auto* m : traces) mat_unref(m);
+    }
+};
+
+BatchPlan* make_batch_plan(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n,
+                           uint64_t shift, uint32_t cap_height, bool use_graph) {
+    if (!traces || !added_bits || n == 0) throw Err{P3H_EINVAL, "null argument"};
+    if (shift >= BB_P) throw Err{P3H_EINVAL, "shift is not a canonical field element"};
+    auto* bp = new BatchPlan{};
+    bp->ctx = ctx;
+    try {
+        std::vector<LdeJob<uint32_t>> jobs;
+        for (uint32_t i = 0; i < n; ++i) {
+            const p3h_mat* tr = traces[i];
+            if (!tr || tr->field != P3H_BABYBEAR || tr->k != 0)
+                throw Err{P3H_EINVAL, "batch traces must be natural-order BabyBear matrices"};
+            if (tr->width == 0) throw Err{P3H_EINVAL, "matrix width is zero"};
+            uint32_t L = log2_exact(tr->height), k = added_bits[i];
+            if (L + k > 27) throw Err{P3H_EINVAL, "LDE height exceeds the field's two-adicity"};
+            bp->traces.push_back(const_cast<p3h_mat*>(tr));
+            mat_ref(const_cast<p3h_mat*>(tr));
+            p3h_mat* out = new_mat(ctx, P3H_BABYBEAR, tr->height << k, tr->width);
+            out->k = k;
+            out->L = L;
+            bp->ldes.push_back(out);
+            jobs.push_back(LdeJob<uint32_t>{tr->data<uint32_t>(), out->data<uint32_t>(), L, tr->width, k,
+                                            twiddles<HBB>(ctx, L), lde_twist<HBB>(ctx, L, k, uint32_t(shift)),
+                                            tuning_for(ctx, P3H_BABYBEAR, L, tr->width, k)});
+        }
+        bp->groups = plan_lde_batch<BabyBear>(jobs);
+        for (const auto& g : bp->groups) {
+            DevPtr d = dalloc(ctx, g.work.size() * sizeof(NttWork<uint32_t>));
+            h2d(ctx, d->ptr, g.work.data(), g.work.size() * sizeof(NttWork<uint32_t>));
+            bp->work_tables.push_back(d);
+        }
+        std::vector<const p3h_mat*> lp(bp->ldes.begin(), bp->ldes.end());
+        bp->tree = prepare_mmcs<HBB>(ctx, lp.data(), n, cap_height, bp->tree_launches);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        if (use_graph) {
+            ctx->capturing = true;
+            hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                try {
+                    bp->launch_all();
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(ctx->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    ctx->capturing = false;
+                    throw;
+                }
+                e = hipStreamEndCapture(ctx->stream, &bp->graph);
+            }
+            ctx->capturing = false;
+            hip_check(e, "graph capture");
+            hip_check(hipGraphInstantiate(&bp->exec, bp->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        }
+    } catch (...) {
+        delete bp;
+        throw;
+    }
+    return bp;
+}
+
+void run_batch_plan(BatchPlan* bp) {
+    p3h_ctx* ctx = bp->ctx;
+    if (bp->exec) {
+        Phase ph(ctx, "batch_graph");
+        hip_check(hipGraphLaunch(bp->exec, ctx->stream), "hipGraphLaunch");
+    } else {
+        bp->launch_all();
+    }
 }
 
 template <class Fn>
@@ -1179,19 +1326,140 @@ int p3h_pcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, const uint64_t* shi
 int p3h_batch_commit(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n,
                      uint64_t shift, uint32_t cap_height, p3h_tree** out) {
     return api(ctx, [&] {
-        if (!traces || !added_bits || !out || n == 0) throw Err{P3H_EINVAL, "null argument"};
-        std::vector<p3h_mat*> ldes;
+        if (!out) throw Err{P3H_EINVAL, "null argument"};
+        BatchPlan* bp = make_batch_plan(ctx, traces, added_bits, n, shift, cap_height, false);
         try {
-            for (uint32_t i = 0; i < n; ++i) {
-                if (traces[i]->field != P3H_BABYBEAR) throw Err{P3H_EINVAL, "batch commit is BabyBear only"};
-                ldes.push_back(do_lde<HBB>(ctx, traces[i], added_bits[i], shift));
-            }
-            *out = do_mmcs_commit<HBB>(ctx, ldes.data(), n, cap_height);
+            run_batch_plan(bp);
         } catch (...) {
-            for (auto* l : ldes) mat_unref(l);
+            delete bp;
             throw;
         }
-        for (auto* l : ldes) mat_unref(l);
+        *out = bp->tree;  // hand the tree over; it keeps the LDEs alive
+        bp->tree = nullptr;
+        delete bp;
+    });
+}
+
+struct p3h_batch {
+    BatchPlan* plan;
+};
+
+int p3h_batch_plan(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n, uint64_t shift,
+                   uint32_t cap_height, int use_graph, p3h_batch** out) {
+    return api(ctx, [&] {
+        if (!out) throw Err{P3H_EINVAL, "null argument"};
+        BatchPlan* bp = make_batch_plan(ctx, traces, added_bits, n, shift, cap_height, use_graph != 0);
+        *out = new p3h_batch{bp};
+    });
+}
+
+int p3h_batch_run(p3h_batch* b, void* cap_out) {
+    if (!b) return P3H_EINVAL;
+    p3h_ctx* ctx = b->plan->ctx;
+    return api(ctx, [&] {
+        run_batch_plan(b->plan);
+        if (cap_out) d2h(ctx, cap_out, b->plan->tree->layers.back()->ptr, b->plan->tree->counts.back() * 32);
+    });
+}
+
+int p3h_batch_tree(const p3h_batch* b, const p3h_tree** out) {
+    if (!b || !out) return P3H_EINVAL;
+    *out = b->plan->tree;
+    return P3H_OK;
+}
+
+int p3h_batch_free(p3h_batch* b) {
+    if (!b) return P3H_EINVAL;
+    (void)hipStreamSynchronize(b->plan->ctx->stream);
+    delete b->plan;
+    delete b;
+    return P3H_OK;
+}
+
+int p3h_mat_write(p3h_ctx* ctx, p3h_mat* m, const void* host) {
+    return api(ctx, [&] {
+        if (!m || !host) throw Err{P3H_EINVAL, "null argument"};
+        if (m->k != 0) throw Err{P3H_EINVAL, "only natural-order matrices can be overwritten"};
+        h2d(ctx, m->data<char>(), host, m->height * m->width * elem_size(m->field));
+    });
+}
+
+// Time one LDE shape under candidate pass shapes and keep the fastest.
+int p3h_autotune_lde(p3h_ctx* ctx, int field, uint32_t log_n, uint32_t width, uint32_t added_bits, uint32_t iters,
+                     uint32_t* best_log_cols, uint32_t* best_max_stages, double* best_ms) {
+    return api(ctx, [&] {
+        if (field != P3H_BABYBEAR && field != P3H_GOLDILOCKS) throw Err{P3H_EINVAL, "unknown field"};
+        if (width == 0 || iters == 0) throw Err{P3H_EINVAL, "width and iters must be positive"};
+        uint64_t h = 1ull << log_n;
+        p3h_mat* in = new_mat(ctx, field, h, width);
+        std::unique_ptr<p3h_mat, void (*)(p3h_mat*)> guard(in, mat_unref);
+        hip_check(hipMemsetAsync(in->buf->ptr, 0, h * width * elem_size(field), ctx->stream), "memset");
+        const uint32_t tile_log = field == P3H_BABYBEAR ? 14 : 13;
+        const uint32_t smax = tile_log - 4;
+        std::vector<NttTuning> cands;
+        for (uint32_t lc = 2; lc <= 7; ++lc)
+            for (uint32_t ms : {0u, smax - 1, smax - 2, smax - 3}) cands.push_back(NttTuning{lc, ms});
+        auto key = std::make_tuple(field, log_n, width, added_bits);
+        NttTuning saved = ctx->tune;
+        bool had = ctx->tuned.count(key) != 0;
+        NttTuning prev = had ? ctx->tuned[key] : NttTuning{};
+        double best = 1e30;
+        NttTuning bt{};
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        for (const auto& c : cands) {
+            ctx->tuned[key] = c;
+            double tmin = 1e30;
+            for (uint32_t it = 0; it <= iters; ++it) {
+                (void)hipEventRecord(e0, ctx->stream);
+                p3h_mat* o = field == P3H_BABYBEAR ? do_lde<HBB>(ctx, in, added_bits, 31)
+                                                   : do_lde<HGL>(ctx, in, added_bits, 7);
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                mat_unref(o);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (it > 0 && ms < tmin) tmin = ms;  // first run warms the table caches
+            }
+            if (tmin < best) {
+                best = tmin;
+                bt = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->tune = saved;
+        ctx->tuned[key] = bt;
+        (void)prev;
+        if (best_log_cols) *best_log_cols = bt.log_cols;
+        if (best_max_stages) *best_max_stages = bt.max_stages;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+// Tuning table as text lines "field log_n width added log_cols max_stages".
+int p3h_tuning_save(const p3h_ctx* ctx, const char* path) {
+    auto* c = const_cast<p3h_ctx*>(ctx);
+    return api(c, [&] {
+        FILE* f = std::fopen(path, "w");
+        if (!f) throw Err{P3H_EINVAL, std::string("cannot write ") + path};
+        for (const auto& kv : c->tuned)
+            std::fprintf(f, "%d %u %u %u %u %u\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                         std::get<2>(kv.first), std::get<3>(kv.first), kv.second.log_cols, kv.second.max_stages);
+        std::fclose(f);
+    });
+}
+
+int p3h_tuning_load(p3h_ctx* ctx, const char* path) {
+    return api(ctx, [&] {
+        FILE* f = std::fopen(path, "r");
+        if (!f) throw Err{P3H_EINVAL, std::string("cannot read ") + path};
+        int field;
+        unsigned L, w, k, lc, ms;
+        while (std::fscanf(f, "%d %u %u %u %u %u", &field, &L, &w, &k, &lc, &ms) == 6)
+            ctx->tuned[std::make_tuple(field, L, w, k)] = NttTuning{lc, ms};
+        std::fclose(f);
     });
 }
 

@@ -84,6 +84,15 @@ _SIGS = {
     "p3h_fri_free": (_i32, [_vp]),
     "p3h_pcs_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u32, C.POINTER(_vp)]),
     "p3h_batch_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u64, _u32, C.POINTER(_vp)]),
+    "p3h_mat_write": (_i32, [_vp, _vp, _vp]),
+    "p3h_batch_plan": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u64, _u32, _i32, C.POINTER(_vp)]),
+    "p3h_batch_run": (_i32, [_vp, _vp]),
+    "p3h_batch_tree": (_i32, [_vp, C.POINTER(_vp)]),
+    "p3h_batch_free": (_i32, [_vp]),
+    "p3h_autotune_lde": (_i32, [_vp, _i32, _u32, _u32, _u32, _u32, C.POINTER(_u32), C.POINTER(_u32),
+                                C.POINTER(C.c_double)]),
+    "p3h_tuning_save": (_i32, [_vp, C.c_char_p]),
+    "p3h_tuning_load": (_i32, [_vp, C.c_char_p]),
 }
 
 
@@ -240,6 +249,26 @@ class Context:
         self._check(self.lib.p3h_batch_commit(self.h, arr, _ptr(ab), len(traces), shift, cap_height, C.byref(h)))
         return Tree(self, h, BABYBEAR)
 
+    def batch_plan(self, traces, added_bits, shift: int, cap_height: int = 0, use_graph: bool = True) -> "BatchPlan":
+        arr = (_vp * len(traces))(*[m.h for m in traces])
+        ab = np.ascontiguousarray(added_bits, dtype=np.uint32)
+        h = _vp()
+        self._check(self.lib.p3h_batch_plan(self.h, arr, _ptr(ab), len(traces), shift, cap_height,
+                                            1 if use_graph else 0, C.byref(h)))
+        return BatchPlan(self, h, traces)
+
+    def autotune_lde(self, field: int, log_n: int, width: int, added_bits: int, iters: int = 3):
+        lc, ms, t = _u32(), _u32(), C.c_double()
+        self._check(self.lib.p3h_autotune_lde(self.h, field, log_n, width, added_bits, iters, C.byref(lc),
+                                              C.byref(ms), C.byref(t)))
+        return {"log_cols": lc.value, "max_stages": ms.value, "ms": t.value}
+
+    def tuning_save(self, path: str):
+        self._check(self.lib.p3h_tuning_save(self.h, path.encode()))
+
+    def tuning_load(self, path: str):
+        self._check(self.lib.p3h_tuning_load(self.h, path.encode()))
+
     def fri_fold(self, cur: "Mat", log_arity: int, beta) -> "Mat":
         b = np.ascontiguousarray(beta, dtype=np.uint32)
         h = _vp()
@@ -279,6 +308,12 @@ class Mat:
         out = np.empty((self.height, self.width), dtype=DTYPE[self.field])
         self.ctx._check(self.ctx.lib.p3h_mat_download(self.ctx.h, self.h, _ptr(out)))
         return out
+
+    def write(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=DTYPE[self.field])
+        if a.size != self.height * self.width:
+            raise ValueError("shape mismatch")
+        self.ctx._check(self.ctx.lib.p3h_mat_write(self.ctx.h, self.h, _ptr(a)))
 
     def download_rows(self, row0: int, nrows: int) -> np.ndarray:
         out = np.empty((nrows, self.width), dtype=DTYPE[self.field])
@@ -338,6 +373,42 @@ class Tree:
         if self.h and self.owned:
             self.ctx.lib.p3h_tree_free(self.h)
         self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class BatchPlan:
+    """A planned batched commit (optionally a captured HIP graph)."""
+
+    def __init__(self, ctx: Context, h, traces):
+        self.ctx, self.h, self.traces = ctx, h, traces
+
+    def run(self, want_cap: bool = True):
+        if not want_cap:
+            self.ctx._check(self.ctx.lib.p3h_batch_run(self.h, None))
+            return None
+        n = _u32()
+        tr = _vp()
+        self.ctx._check(self.ctx.lib.p3h_batch_tree(self.h, C.byref(tr)))
+        self.ctx._check(self.ctx.lib.p3h_tree_cap(self.ctx.h, tr, None, C.byref(n)))
+        out = np.empty((n.value, 8), dtype=np.uint32)
+        self.ctx._check(self.ctx.lib.p3h_batch_run(self.h, _ptr(out)))
+        return out
+
+    def tree(self) -> "Tree":
+        tr = _vp()
+        self.ctx._check(self.ctx.lib.p3h_batch_tree(self.h, C.byref(tr)))
+        return Tree(self.ctx, tr, BABYBEAR, owned=False)
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.p3h_batch_free(self.h)
+            self.h = None
 
     def __del__(self):
         try:

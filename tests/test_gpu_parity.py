@@ -231,3 +231,49 @@ def test_budget_rejects_oversized_request(golden):
             small.upload(np.zeros((1 << 20, 4), dtype=U32))
         m = small.upload(np.zeros((16, 8), dtype=U32))  # still usable afterwards
         assert small.flat_commit(m).n_leaves == 16
+
+
+# ------------------------------------------------------- batch plan / graph --
+def _batch_inputs(seed, count, hmin=3, hmax=10):
+    rng = np.random.default_rng(seed)
+    traces, ks = [], []
+    for i in range(count):
+        h, w, k = 1 << int(rng.integers(hmin, hmax + 1)), int(rng.integers(8, 40)), int(rng.integers(1, 3))
+        traces.append(rnd(seed * 100 + i, (h, w)))
+        ks.append(k)
+    return traces, ks
+
+
+def _oracle_batch(orc, golden, traces, ks, cap):
+    ldes = [orc.lde("bb", t, k, 31) for t, k in zip(traces, ks)]
+    return orc.mmcs("bb", golden["consts_bb"], RF, 13, ldes, cap, 3)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_plan_replays_match_oracle(ctx, golden, orc, use_graph):
+    traces, ks = _batch_inputs(21, 7)
+    mats = [ctx.upload(t) for t in traces]
+    plan = ctx.batch_plan(mats, ks, 31, cap_height=1, use_graph=use_graph)
+    c, vals, proof = _oracle_batch(orc, golden, traces, ks, 1)
+    for rep in range(3):
+        cap = plan.run()
+        assert (cap == c).all(), rep
+        v, p = plan.tree().open([3])
+        assert (v[0] == vals).all() and (p[0] == proof).all()
+        # eager work on the same stream between replays
+        ctx.coset_lde(ctx.upload(rnd(rep, (64, 8))), 1, 31).download()
+    # new contents in the same device buffers: the replay recomputes
+    traces2 = [rnd(900 + i, t.shape) for i, t in enumerate(traces)]
+    for m, t in zip(mats, traces2):
+        m.write(t)
+    c2, _, _ = _oracle_batch(orc, golden, traces2, ks, 1)
+    for rep in range(2):
+        assert (plan.run() == c2).all(), rep
+    plan.free()
+
+
+def test_autotuned_lde_matches_oracle(ctx, orc):
+    best = ctx.autotune_lde(p3hip.BABYBEAR, 12, 24, 1, iters=1)
+    assert best["ms"] > 0
+    m = rnd(31, (1 << 12, 24))
+    assert (ctx.coset_lde(ctx.upload(m), 1, 31).download() == orc.lde("bb", m, 1, 31)).all()
