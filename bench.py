@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + list(SPECIAL))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hash-engine", type=int, default=1, help="BabyBear Poseidon2: 0 uint32, 1 binary64")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
@@ -112,12 +114,14 @@ def main():
     shift = 31 if field == "bb" else 7
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(fid, p2_constants(field, rp), 8, rp)
+    ctx.set_hash_engine(args.hash_engine)
     rng = np.random.default_rng(2024)
     p = BB_P if field == "bb" else GL_P
     dtype = np.uint32 if field == "bb" else np.uint64
     trace = rng.integers(0, p, size=(n, width), dtype=dtype)
     m = ctx.upload(trace)
     del trace
+    tuned = None if args.no_autotune else ctx.autotune_lde(fid, log_n, width, lb, iters=2)
 
     def step():
         lde = ctx.coset_lde(m, lb, shift)
@@ -177,6 +181,8 @@ def main():
                          "frac": lde_gbs / HBM_PEAK_GBS, "traffic": None,
                          "algorithmic_bytes": lde_bytes},
         "root": [int(x) for x in roots[0]],
+        "hash_engine": "binary64" if args.hash_engine == 1 and field == "bb" else "uint32",
+        "lde_tuning": tuned,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field, width, lb, rp)
@@ -199,6 +205,7 @@ def bench_batch(p3hip, args):
     root per trace) on the same GPU and inputs."""
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(p3hip.BABYBEAR, p2_constants("bb", 13), 8, 13)
+    ctx.set_hash_engine(args.hash_engine)
     shapes, traces = batch_traces()
     mats = [ctx.upload(t) for t in traces]
     del traces

@@ -65,6 +65,10 @@ int p3h_ctx_trim(p3h_ctx* ctx);
 int p3h_ctx_set_profiling(p3h_ctx* ctx, int on);
 /* phase: "lde", "leaf_hash", "compress", "fold"; totals since profiling was switched on */
 int p3h_phase_stats(const p3h_ctx* ctx, const char* phase, uint64_t* launches, double* total_ms);
+/* BabyBear Poseidon2 evaluation engine: 0 = 32-bit integer Montgomery,
+ * 1 = exact binary64 on the FP64 VALU (default; fewer instructions per
+ * permutation).  Both give identical digests. */
+int p3h_ctx_set_hash_engine(p3h_ctx* ctx, int engine);
 /* NTT pass shape: log2 columns per LDS tile and max radix-2 stages per pass (0 = default) */
 int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages);
 

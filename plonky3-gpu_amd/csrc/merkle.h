@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "poseidon2.h"
+#include "poseidon2_f64.h"
 
 namespace p3h {
 
@@ -47,5 +48,14 @@ int permute_device(typename D::T* states, uint64_t n, const P2Params<typename D:
 template <class T>
 int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows,
                        uint32_t nq, T* out, hipStream_t st);
+
+// BabyBear kernels with the binary64 permutation (poseidon2_f64.h); same
+// contracts as the uint32 versions above.
+int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t total_w, const RowSeg<uint32_t>& seg0,
+                         uint64_t rows, uint32_t* out, const P2ParamsF64& p, hipStream_t st);
+int compress_layer_f64_device(const uint32_t* prev, uint64_t prev_count, uint32_t* next, uint64_t next_count,
+                              const RowSeg<uint32_t>* inj, uint32_t ninj, uint32_t inj_w, const P2ParamsF64& p,
+                              hipStream_t st);
+int permute_f64_device(uint32_t* states, uint64_t n, const P2ParamsF64& p, hipStream_t st);
 
 }  // namespace p3h

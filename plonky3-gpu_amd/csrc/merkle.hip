@@ -12,6 +12,7 @@
 // compress(node, sponge(rows of the matrices whose height is this level)).
 
 #include "merkle.h"
+#include "poseidon2_f64.h"
 
 namespace p3h {
 
@@ -161,6 +162,109 @@ __global__ void gather_rows_kernel(const RowSeg<T>* segs, uint32_t nsegs, uint32
     out[idx] = s.ptr ? s.ptr[stored_row(s, rows[q]) * s.w + c] : T(0);
 }
 
+
+// ---------------------------------------------------------------- FP64 ----
+// The same sponge / compression with the BabyBear permutation evaluated in
+// binary64 (poseidon2_f64.h).  Inputs and outputs are canonical uint32.
+__device__ __forceinline__ void sponge_row_f64(double* s, const RowSeg<uint32_t>* segs, uint32_t total_w, uint64_t j,
+                                               const P2ParamsF64& p) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = 0.0;
+    uint32_t seg = 0, off = 0;
+    RowSeg<uint32_t> cur = segs[0];
+    const uint32_t* rowp = cur.ptr + stored_row(cur, j) * cur.w;
+    for (uint32_t start = 0; start < total_w; start += 8) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (start + q < total_w) {
+                while (off >= cur.w) {
+                    off = 0;
+                    ++seg;
+                    cur = segs[seg];
+                    rowp = cur.ptr + stored_row(cur, j) * cur.w;
+                }
+                s[q] = cur.ptr ? double(rowp[off]) : 0.0;
+                ++off;
+            }
+        }
+        Poseidon2F64::permute(s, p);
+    }
+}
+
+__global__ __launch_bounds__(HASH_THREADS) void hash_rows_f64_kernel(const RowSeg<uint32_t>* segs, uint32_t total_w,
+                                                                     uint64_t rows, uint32_t* out, P2ParamsF64 p) {
+    uint64_t j = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (j >= rows) return;
+    double s[16];
+    sponge_row_f64(s, segs, total_w, j, p);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);
+}
+
+__global__ __launch_bounds__(HASH_THREADS) void hash_rows_dense_f64_kernel(RowSeg<uint32_t> seg, uint64_t rows,
+                                                                           uint32_t* out, P2ParamsF64 p) {
+    uint64_t j = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (j >= rows) return;
+    double s[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = 0.0;
+    const uint32_t* rowp = seg.ptr + stored_row(seg, j) * seg.w;
+    for (uint32_t start = 0; start < seg.w; start += 8) {
+        const uint4* v = reinterpret_cast<const uint4*>(rowp + start);
+        uint4 a = v[0], b = v[1];
+        s[0] = double(a.x);
+        s[1] = double(a.y);
+        s[2] = double(a.z);
+        s[3] = double(a.w);
+        s[4] = double(b.x);
+        s[5] = double(b.y);
+        s[6] = double(b.z);
+        s[7] = double(b.w);
+        Poseidon2F64::permute(s, p);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);
+}
+
+__global__ __launch_bounds__(HASH_THREADS) void compress_f64_kernel(const uint32_t* prev, uint64_t prev_count,
+                                                                    uint32_t* next, uint64_t next_count,
+                                                                    const RowSeg<uint32_t>* inj, uint32_t inj_w,
+                                                                    P2ParamsF64 p) {
+    uint64_t i = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (i >= next_count) return;
+    double s[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[q] = 0.0;
+    const uint32_t* l = prev + 2 * i * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] = double(l[q]);
+    if (2 * i + 1 < prev_count) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[8 + q] = double(l[8 + q]);
+    }
+    Poseidon2F64::permute(s, p);
+    if (inj) {
+        double h[16];
+        sponge_row_f64(h, inj, inj_w, i, p);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[8 + q] = h[q];
+        Poseidon2F64::permute(s, p);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) next[i * 8 + q] = BBF64::canonical(s[q]);
+}
+
+__global__ __launch_bounds__(HASH_THREADS) void permute_f64_kernel(uint32_t* st, uint64_t n, P2ParamsF64 p) {
+    uint64_t i = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (i >= n) return;
+    double s[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[q] = double(st[i * 16 + q]);
+    Poseidon2F64::permute(s, p);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) st[i * 16 + q] = BBF64::canonical(s[q]);
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t threads) { return uint32_t((n + threads - 1) / threads); }
 
 }  // namespace
@@ -207,6 +311,35 @@ int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, 
     if (total == 0) return 0;
     hipLaunchKernelGGL((gather_rows_kernel<T>), dim3(grid_for(total, 256)), dim3(256), 0, st, segs, nsegs, total_w,
                        rows, nq, out);
+    return int(hipGetLastError());
+}
+
+
+int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t total_w, const RowSeg<uint32_t>& seg0,
+                         uint64_t rows, uint32_t* out, const P2ParamsF64& p, hipStream_t st) {
+    if (rows == 0) return 0;
+    bool dense = nsegs == 1 && seg0.ptr && seg0.w > 0 && seg0.w % 8 == 0 && (reinterpret_cast<uintptr_t>(seg0.ptr) & 15) == 0;
+    if (dense)
+        hipLaunchKernelGGL(hash_rows_dense_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st,
+                           seg0, rows, out, p);
+    else
+        hipLaunchKernelGGL(hash_rows_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st, segs,
+                           total_w, rows, out, p);
+    return int(hipGetLastError());
+}
+
+int compress_layer_f64_device(const uint32_t* prev, uint64_t prev_count, uint32_t* next, uint64_t next_count,
+                              const RowSeg<uint32_t>* inj, uint32_t ninj, uint32_t inj_w, const P2ParamsF64& p,
+                              hipStream_t st) {
+    if (next_count == 0) return 0;
+    hipLaunchKernelGGL(compress_f64_kernel, dim3(grid_for(next_count, HASH_THREADS)), dim3(HASH_THREADS), 0, st, prev,
+                       prev_count, next, next_count, ninj ? inj : nullptr, inj_w, p);
+    return int(hipGetLastError());
+}
+
+int permute_f64_device(uint32_t* states, uint64_t n, const P2ParamsF64& p, hipStream_t st) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(permute_f64_kernel, dim3(grid_for(n, HASH_THREADS)), dim3(HASH_THREADS), 0, st, states, n, p);
     return int(hipGetLastError());
 }
 

@@ -132,6 +132,8 @@ struct p3h_ctx {
     bool p2_set[2] = {false, false};
     P2Params<uint32_t, 16> bbp{};
     P2Params<uint64_t, 8> glp{};
+    P2ParamsF64 bbp64{};     // BabyBear constants as doubles (binary64 engine)
+    int hash_engine = 1;     // BabyBear Poseidon2: 0 = uint32 Montgomery, 1 = binary64 (default)
     std::map<std::tuple<int, uint32_t>, DevPtr> twiddles;                      // (field, L)
     std::map<std::tuple<int, uint32_t, uint32_t, uint64_t>, DevPtr> twists;    // (field, L, k, shift)
     std::map<std::tuple<int, uint32_t, int, uint64_t>, DevPtr> row_scales;     // (field, L, kind, shift)
@@ -550,7 +552,15 @@ p3h_tree* prepare_mmcs(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uin
             uint32_t ns = uint32_t(top.size());
             RowSeg<T> s0 = top[0];
             T* lv = dptr<T>(leaves);
+            const bool f64 = H::FIELD == P3H_BABYBEAR && ctx->hash_engine == 1;
+            const P2ParamsF64 p64 = ctx->bbp64;
             out.emplace_back("leaf_hash", [=](p3h_ctx* c) {
+                if constexpr (H::FIELD == P3H_BABYBEAR) {
+                    if (f64) {
+                        launch_check(hash_rows_f64_device(ds, ns, w0, s0, maxh, lv, p64, c->stream), "leaf hash");
+                        return;
+                    }
+                }
                 launch_check(hash_rows_device<D>(ds, ns, w0, s0, maxh, lv, pp, c->stream), "leaf hash");
             });
         }
@@ -572,7 +582,16 @@ p3h_tree* prepare_mmcs(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uin
             const RowSeg<T>* dinj = di ? dptr<RowSeg<T>>(di) : nullptr;
             uint32_t ninj = uint32_t(inj.size());
             uint64_t lv = level;
+            const bool f64c = H::FIELD == P3H_BABYBEAR && ctx->hash_engine == 1;
+            const P2ParamsF64 p64c = ctx->bbp64;
             out.emplace_back("compress", [=](p3h_ctx* c) {
+                if constexpr (H::FIELD == P3H_BABYBEAR) {
+                    if (f64c) {
+                        launch_check(compress_layer_f64_device(prev, lv * 2, nxt, lv, dinj, ninj, wi, p64c, c->stream),
+                                     "compress");
+                        return;
+                    }
+                }
                 launch_check(compress_layer_device<D>(prev, lv * 2, nxt, lv, dinj, ninj, wi, pp, c->stream),
                              "compress");
             });
@@ -633,9 +652,19 @@ p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs
         DevPtr leaves = dalloc(ctx, height * D::DIG * sizeof(T));
         {
             Phase ph(ctx, "leaf_hash");
-            launch_check(hash_rows_device<D>(dptr<RowSeg<T>>(dsegs), uint32_t(segs.size()), width, segs[0], height,
-                                             dptr<T>(leaves), pp, ctx->stream),
-                         "leaf hash");
+            bool done = false;
+            if constexpr (H::FIELD == P3H_BABYBEAR) {
+                if (ctx->hash_engine == 1) {
+                    launch_check(hash_rows_f64_device(dptr<RowSeg<T>>(dsegs), uint32_t(segs.size()), width, segs[0],
+                                                      height, dptr<T>(leaves), ctx->bbp64, ctx->stream),
+                                 "leaf hash");
+                    done = true;
+                }
+            }
+            if (!done)
+                launch_check(hash_rows_device<D>(dptr<RowSeg<T>>(dsegs), uint32_t(segs.size()), width, segs[0], height,
+                                                 dptr<T>(leaves), pp, ctx->stream),
+                             "leaf hash");
         }
         t->layers.push_back(leaves);
         t->counts.push_back(height);
@@ -644,9 +673,19 @@ p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs
             uint64_t nxt = (cnt + 1) / 2;
             DevPtr nl = dalloc(ctx, nxt * D::DIG * sizeof(T));
             Phase ph(ctx, "compress");
-            launch_check(compress_layer_device<D>(dptr<T>(t->layers.back()), cnt, dptr<T>(nl), nxt, nullptr, 0, 0, pp,
-                                                  ctx->stream),
-                         "compress");
+            bool done = false;
+            if constexpr (H::FIELD == P3H_BABYBEAR) {
+                if (ctx->hash_engine == 1) {
+                    launch_check(compress_layer_f64_device(dptr<T>(t->layers.back()), cnt, dptr<T>(nl), nxt, nullptr,
+                                                           0, 0, ctx->bbp64, ctx->stream),
+                                 "compress");
+                    done = true;
+                }
+            }
+            if (!done)
+                launch_check(compress_layer_device<D>(dptr<T>(t->layers.back()), cnt, dptr<T>(nl), nxt, nullptr, 0, 0,
+                                                      pp, ctx->stream),
+                             "compress");
             t->layers.push_back(nl);
             t->counts.push_back(nxt);
             cnt = nxt;
@@ -997,6 +1036,13 @@ int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages)
     });
 }
 
+int p3h_ctx_set_hash_engine(p3h_ctx* ctx, int engine) {
+    return api(ctx, [&] {
+        if (engine != 0 && engine != 1) throw Err{P3H_EINVAL, "hash engine must be 0 (uint32) or 1 (binary64)"};
+        ctx->hash_engine = engine;
+    });
+}
+
 int p3h_poseidon2_set(p3h_ctx* ctx, int field, uint32_t width, uint32_t rf, uint32_t rp, const void* consts) {
     return api(ctx, [&] {
         if (!consts) throw Err{P3H_EINVAL, "null constants"};
@@ -1016,6 +1062,16 @@ int p3h_poseidon2_set(p3h_ctx* ctx, int field, uint32_t width, uint32_t rf, uint
             p.half_full = rf / 2;
             p.partial = rp;
             ctx->bbp = p;
+            P2ParamsF64 q{};
+            k = 0;
+            for (uint32_t r = 0; r < rf / 2; ++r)
+                for (int i = 0; i < 16; ++i) q.init[r][i] = double(c[k++] % BB_P);
+            for (uint32_t r = 0; r < rf / 2; ++r)
+                for (int i = 0; i < 16; ++i) q.term[r][i] = double(c[k++] % BB_P);
+            for (uint32_t r = 0; r < rp; ++r) q.internal[r] = double(c[k++] % BB_P);
+            q.half_full = rf / 2;
+            q.partial = rp;
+            ctx->bbp64 = q;
             ctx->p2_set[P3H_BABYBEAR] = true;
         } else if (field == P3H_GOLDILOCKS) {
             if (width != 8) throw Err{P3H_EINVAL, "Goldilocks Poseidon2 width must be 8"};
@@ -1045,7 +1101,10 @@ int p3h_poseidon2_permute(p3h_ctx* ctx, int field, void* states, uint64_t n) {
             const auto& pp = p2<HBB>(ctx);
             DevPtr d = dalloc(ctx, std::max<uint64_t>(1, n * 16 * 4));
             h2d(ctx, d->ptr, states, n * 16 * 4);
-            launch_check(permute_device<BBDom>(dptr<uint32_t>(d), n, pp, ctx->stream), "permute");
+            if (ctx->hash_engine == 1)
+                launch_check(permute_f64_device(dptr<uint32_t>(d), n, ctx->bbp64, ctx->stream), "permute");
+            else
+                launch_check(permute_device<BBDom>(dptr<uint32_t>(d), n, pp, ctx->stream), "permute");
             d2h(ctx, states, d->ptr, n * 16 * 4);
         } else if (field == P3H_GOLDILOCKS) {
             const auto& pp = p2<HGL>(ctx);

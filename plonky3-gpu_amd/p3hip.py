@@ -61,6 +61,7 @@ _SIGS = {
     "p3h_ctx_set_profiling": (_i32, [_vp, _i32]),
     "p3h_phase_stats": (_i32, [_vp, C.c_char_p, C.POINTER(_u64), C.POINTER(C.c_double)]),
     "p3h_ctx_set_ntt_tuning": (_i32, [_vp, _u32, _u32]),
+    "p3h_ctx_set_hash_engine": (_i32, [_vp, _i32]),
     "p3h_poseidon2_set": (_i32, [_vp, _i32, _u32, _u32, _u32, _vp]),
     "p3h_poseidon2_permute": (_i32, [_vp, _i32, _vp, _u64]),
     "p3h_mat_upload": (_i32, [_vp, _i32, _vp, _u64, _u32, C.POINTER(_vp)]),
@@ -180,6 +181,10 @@ class Context:
         n, ms = _u64(), C.c_double()
         self._check(self.lib.p3h_phase_stats(self.h, phase.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def set_hash_engine(self, engine: int):
+        """BabyBear Poseidon2 engine: 0 = uint32 Montgomery, 1 = exact binary64."""
+        self._check(self.lib.p3h_ctx_set_hash_engine(self.h, engine))
 
     def set_ntt_tuning(self, log_cols: int = 0, max_stages: int = 0):
         self._check(self.lib.p3h_ctx_set_ntt_tuning(self.h, log_cols, max_stages))

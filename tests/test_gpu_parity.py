@@ -277,3 +277,52 @@ def test_autotuned_lde_matches_oracle(ctx, orc):
     assert best["ms"] > 0
     m = rnd(31, (1 << 12, 24))
     assert (ctx.coset_lde(ctx.upload(m), 1, 31).download() == orc.lde("bb", m, 1, 31)).all()
+
+
+# ---------------------------------------------- uint32 Montgomery engine --
+@pytest.fixture
+def ctx64(ctx):
+    """The session context runs the default binary64 engine; this fixture
+    switches to the uint32 Montgomery engine for the same checks."""
+    ctx.set_hash_engine(0)
+    yield ctx
+    ctx.set_hash_engine(1)
+
+
+def test_u32_engine_permutation_and_sponge(ctx64, golden, orc):
+    assert (ctx64.permute(BB, golden["perm_bb_in"]) == golden["perm_bb_out"]).all()
+    st = rnd(3, 16 * 5000)
+    assert (ctx64.permute(BB, st) == orc.permute("bb", golden["consts_bb"], RF, 13, st)).all()
+    # extreme inputs: all p-1, all zero
+    ext = np.concatenate([np.full(16 * 64, BB_P - 1, dtype=U32), np.zeros(16 * 64, dtype=U32)])
+    assert (ctx64.permute(BB, ext) == orc.permute("bb", golden["consts_bb"], RF, 13, ext)).all()
+    mmcs = p3hip.FlatMerkleTreeMmcsHip(ctx64)
+    for i in range(6):
+        d = golden[f"flat_bb_{i}_in"]
+        root, _ = mmcs.commit_matrix(d.reshape(-1), d.shape[1])
+        assert (root == golden[f"flat_bb_{i}_root"]).all(), i
+
+
+def test_u32_engine_trees(ctx64, golden, orc):
+    for s in range(4):
+        mats = mats_of(golden, f"mm_bb_{s}")
+        for cap in (0, 1):
+            t = ctx64.mmcs_commit([ctx64.upload(m) for m in mats], cap)
+            assert (t.cap() == golden[f"mm_bb_{s}_cap{cap}"]).all(), (s, cap)
+    trace = native.seeded(2024, (1 << 14) * 40).reshape(1 << 14, 40)
+    lde = ctx64.coset_lde(ctx64.upload(trace), 1, 31)
+    want = orc.flat_commit("bb", golden["consts_bb"], RF, 13, orc.lde("bb", trace, 1, 31))
+    assert (ctx64.mmcs_commit([lde], 0).root() == want).all()
+    full = np.full((256, 24), BB_P - 1, dtype=U32)
+    assert (ctx64.flat_commit(ctx64.upload(full)).root() == orc.flat_commit("bb", golden["consts_bb"], RF, 13,
+                                                                           full)).all()
+    for case in range(3):
+        log_len, lb, lfpl, mla = (int(v) for v in golden[f"fri_{case}_params"])
+        roots, final, _ = ctx64.fri_commit_phase(ctx64.upload(golden[f"fri_{case}_cw"]), lb, lfpl, mla,
+                                                 golden[f"fri_{case}_betas"])
+        assert (roots == golden[f"fri_{case}_roots"]).all(), case
+    traces, ks = _batch_inputs(44, 5)
+    plan = ctx64.batch_plan([ctx64.upload(t) for t in traces], ks, 31, cap_height=1, use_graph=True)
+    c, _, _ = _oracle_batch(orc, golden, traces, ks, 1)
+    assert (plan.run() == c).all() and (plan.run() == c).all()
+    plan.free()
