@@ -31,8 +31,14 @@ GL_P = 0xFFFFFFFF00000001
 
 # Published MI355X peaks used for roofline fractions.
 HBM_PEAK_GBS = 8000.0
-# 32-bit VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (full-rate ops).
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+FP64_PEAK_TFLOPS = 78.6  # vector and matrix FP64 are the same figure on MI355X
+# The BabyBear leaf hash runs the exact binary64 Poseidon2 engine: one width-16
+# permutation (8 full + 13 partial rounds) is 2362 add + 564 mul + 2335 fma
+# binary64 operations = 7596 flops in 5261 FP64 VALU instructions (counted from
+# poseidon2_f64.h); each digest lane's final canonical reduction adds 5 flops.
+P2_F64_FLOPS_PER_PERM = 7596
+P2_F64_FLOPS_PER_DIGEST = 8 * 5
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 WORKLOADS = {
     # name: (field, log_n, width, log_blowup, poseidon2 rounds_p)
@@ -42,7 +48,7 @@ WORKLOADS = {
     "gl_2^22x64_b2": ("gl", 22, 64, 2, 22),
 }
 DEFAULT = "bb_2^20x256_b1"
-SPECIAL = ("batch64",)
+SPECIAL = ("batch64", "serve")
 
 
 def load_p3hip():
@@ -99,15 +105,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--hash-engine", type=int, default=1, help="BabyBear Poseidon2: 0 uint32, 1 binary64")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning-file", default=None,
+                    help="load the LDE pass shapes from this file if it exists, else autotune and save them there")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.calibrate_copy:
+        return calibrate_copy()
     p3hip = load_p3hip()
     if args.workload == "batch64":
         return bench_batch(p3hip, args)
+    if args.workload == "serve":
+        return bench_serve(p3hip, args)
     field, log_n, width, lb, rp = WORKLOADS[args.workload]
     fid = p3hip.BABYBEAR if field == "bb" else p3hip.GOLDILOCKS
     n, N = 1 << log_n, 1 << (log_n + lb)
@@ -121,7 +135,13 @@ def main():
     trace = rng.integers(0, p, size=(n, width), dtype=dtype)
     m = ctx.upload(trace)
     del trace
-    tuned = None if args.no_autotune else ctx.autotune_lde(fid, log_n, width, lb, iters=2)
+    if args.tuning_file and os.path.exists(args.tuning_file):
+        ctx.tuning_load(args.tuning_file)
+        tuned = {"loaded": args.tuning_file}
+    else:
+        tuned = None if args.no_autotune else ctx.autotune_lde(fid, log_n, width, lb, iters=2)
+        if args.tuning_file and tuned is not None:
+            ctx.tuning_save(args.tuning_file)
 
     def step():
         lde = ctx.coset_lde(m, lb, shift)
@@ -154,8 +174,23 @@ def main():
     # LDE phase: algorithmic bytes = eb * w * (3n + N)
     lde_bytes = eb * width * (3 * n + N)
     lde_gbs = lde_bytes / (lde_ms / 1e3) / 1e9
-    # leaf hashing: perms per launch; VALU slots per permutation from the gfx950 ISA
+    # leaf hashing: one launch absorbs every LDE row
     perms = N * ((width + rate - 1) // rate)
+    hash_bytes = eb * width * N + eb * dig * N
+    traffic = load_traffic(args.workload)
+    if field == "bb" and args.hash_engine == 1:
+        flops = perms * P2_F64_FLOPS_PER_PERM + N * P2_F64_FLOPS_PER_DIGEST
+        ach = flops / (hash_ms / 1e3) / 1e12
+        roofline = {"bound": "valu", "kernel": "hash_rows_dense_f64_kernel (Poseidon2 leaf sponge, binary64)",
+                    "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                    "traffic": traffic.get("hash_rows_dense_f64_kernel"), "algorithmic_flops": flops,
+                    "algorithmic_bytes": hash_bytes, "hbm_GBps": hash_bytes / (hash_ms / 1e3) / 1e9,
+                    "avg_launch_ms": hash_ms}
+    else:
+        roofline = {"bound": "valu", "kernel": "hash_rows_dense_kernel (Poseidon2 leaf sponge, integer)",
+                    "achieved": perms / (hash_ms / 1e3) / 1e9, "unit": "Gperm/s", "peak": None, "frac": None,
+                    "traffic": traffic.get("hash_rows_dense_kernel"), "algorithmic_bytes": hash_bytes,
+                    "avg_launch_ms": hash_ms}
     out = {
         "metric": "committed field elems/s (LDE+Poseidon2 Merkle)",
         "value": value,
@@ -174,12 +209,10 @@ def main():
                                f"single-matrix MMCS root",
                    "rows": n, "width": width, "log_blowup": lb, "lde_rows": N},
         "phases_ms": {"lde": lde_ms, "leaf_hash": hash_ms, "compress": comp_ms},
-        "roofline": {"bound": "valu", "kernel": "hash_rows_dense (Poseidon2 leaf sponge)",
-                     "achieved": perms / (hash_ms / 1e3) / 1e9, "unit": "Gperm/s",
-                     "peak": None, "frac": None, "traffic": None},
+        "roofline": roofline,
         "roofline_lde": {"bound": "hbm", "achieved": lde_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": lde_gbs / HBM_PEAK_GBS, "traffic": None,
-                         "algorithmic_bytes": lde_bytes},
+                         "frac": lde_gbs / HBM_PEAK_GBS, "traffic": traffic.get("lde_phase"),
+                         "algorithmic_bytes": lde_bytes, "avg_phase_ms": lde_ms},
         "root": [int(x) for x in roots[0]],
         "hash_engine": "binary64" if args.hash_engine == 1 and field == "bb" else "uint32",
         "lde_tuning": tuned,
@@ -188,6 +221,29 @@ def main():
         out["cpu_baseline"] = cpu_baseline(field, width, lb, rp)
     print(json.dumps(out))
     ctx.close()
+
+
+def load_traffic(workload: str) -> dict:
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    (each pass calibrated against a 1 GiB copy), kept as a profile summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(workload, {})
+    except (OSError, ValueError):
+        return {}
+
+
+def calibrate_copy(nbytes: int = 1 << 30, reps: int = 3):
+    import torch
+
+    a = torch.randint(0, 1 << 30, (nbytes // 4,), dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.add(a, 1, out=b)  # one elementwise kernel: reads nbytes, writes nbytes
+    torch.cuda.synchronize()
+    assert torch.equal(a + 1, b)
+    print(json.dumps({"calibration": "elementwise copy (b = a + 1)", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "reps": reps}))
 
 
 def batch_traces(seed: int = 7, count: int = 64):
@@ -261,6 +317,116 @@ def bench_batch(p3hip, args):
     }
     print(json.dumps(out))
     plan.free()
+    ctx.close()
+
+
+def _pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))]
+
+
+def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 32):
+    """Open-loop Poisson commit requests (BabyBear, log_blowup 1, heights
+    2^[12,20], widths [8,128], 64 leaf openings each) at three arrival rates
+    plus a saturating burst.  The served form (continuous batching over 4
+    streams, admission under a fixed HBM budget, LDE and Merkle layers resident
+    until the openings are taken) runs against the plain per-request path (one
+    context, FIFO: upload, LDE, commit, root, openings, free) on the same
+    arrivals.  Latency is scheduled arrival -> root on the host."""
+    import threading
+
+    consts = p2_constants("bb", 13)
+    rng = np.random.default_rng(11)
+    shapes = [(int(rng.integers(12, 21)), int(rng.integers(8, 129))) for _ in range(pool)]
+    traces = [rng.integers(0, BB_P, size=(1 << h, w), dtype=np.uint32) for h, w in shapes]
+    order = rng.integers(0, pool, size=n_req)
+    elems = sum((2 << shapes[i][0]) * shapes[i][1] for i in order)
+    openings = [rng.integers(0, 2 << shapes[i][0], size=64) for i in order]
+
+    ctx = p3hip.Context(0)
+    ctx.set_poseidon2(p3hip.BABYBEAR, consts, 8, 13)
+
+    def plain_run(arrivals):
+        lat, start = [], time.perf_counter()
+        for k, (a, i) in enumerate(zip(arrivals, order)):
+            now = time.perf_counter() - start
+            if now < a:
+                time.sleep(a - now)
+            m = ctx.upload(traces[i])
+            lde = ctx.coset_lde(m, 1, 31)
+            tree = ctx.mmcs_commit([lde], 0)
+            tree.root()
+            lat.append((time.perf_counter() - start - a) * 1e3)
+            tree.open(openings[k])
+            tree.free(), lde.free(), m.free()
+        return lat, time.perf_counter() - start
+
+    def served_run(arrivals):
+        with p3hip.Server(consts, 8, 13, hbm_budget=budget_gib << 30, n_slots=4, max_batch=8) as sv:
+            ids, lag = [None] * n_req, [0.0] * n_req
+
+            def submitter():
+                for k, (a, i) in enumerate(zip(arrivals, order)):
+                    now = time.perf_counter() - st
+                    if now < a:
+                        time.sleep(a - now)
+                    lag[k] = (time.perf_counter() - st - a) * 1e3
+                    ids[k] = sv.submit(traces[i], 1, 31)
+
+            st = time.perf_counter()
+            th = threading.Thread(target=submitter)
+            th.start()
+            lat = []
+            for k in range(n_req):
+                while ids[k] is None:
+                    time.sleep(0.0002)
+                _, server_ms = sv.wait(ids[k])
+                lat.append(lag[k] + server_ms)
+                h, w = shapes[order[k]]
+                sv.open(ids[k], openings[k], w, h + 2)
+                sv.release(ids[k])
+            th.join()
+            span = time.perf_counter() - st
+            stats = sv.stats()
+            assert stats["completed"] == n_req and stats["reserved_bytes"] == 0
+            return lat, span, stats["batches"]
+
+    plain_run(np.zeros(n_req))  # warm caches and tuning on both paths
+    served_run(np.zeros(n_req))
+    lat_p, span_p = plain_run(np.zeros(n_req))
+    lat_s, span_s, nb = served_run(np.zeros(n_req))
+    res = {"budget_gib": budget_gib, "committed_elems": elems,
+           "burst": {"served": {"elems_per_s": elems / span_s, "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
+                                "batches": nb},
+                     "plain": {"elems_per_s": elems / span_p, "p50_ms": _pct(lat_p, 50), "p99_ms": _pct(lat_p, 99)}},
+           "rates": []}
+    svc = span_p / n_req  # plain per-request service time
+    for factor in (0.5, 1.0, 1.6):
+        lam = factor / svc
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=n_req))
+        lat_p, span_p = plain_run(arrivals)
+        lat_s, span_s, nb = served_run(arrivals)
+        res["rates"].append({
+            "arrival_rate_per_s": lam, "load_vs_plain_capacity": factor,
+            "served": {"p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99), "elems_per_s": elems / span_s,
+                       "batches": nb},
+            "plain": {"p50_ms": _pct(lat_p, 50), "p99_ms": _pct(lat_p, 99), "elems_per_s": elems / span_p},
+        })
+    top = res["rates"][-1]
+    out = {
+        "metric": "committed field elems/s (served, Poisson arrivals at 1.6x plain capacity)",
+        "value": top["served"]["elems_per_s"], "unit": "field elems/s",
+        "n_gpus": 1, "steps": n_req, "warmup": n_req, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded traces, Poisson arrivals)",
+        "config": {"workload": f"{n_req} Poisson commit requests over {pool} BabyBear traces, h in [12,20], "
+                               f"w in [8,128], log_blowup 1, 64 openings each; 4 slots, max batch 8, "
+                               f"{budget_gib} GiB HBM budget"},
+        "p50_ms": top["served"]["p50_ms"], "p99_ms": top["served"]["p99_ms"],
+        "vs_plain": top["served"]["elems_per_s"] / top["plain"]["elems_per_s"],
+        "serving": res,
+    }
+    print(json.dumps(out))
     ctx.close()
 
 

@@ -185,6 +185,36 @@ int p3h_batch_run(p3h_batch* b, void* cap_out);
 int p3h_batch_tree(const p3h_batch* b, const p3h_tree** out);
 int p3h_batch_free(p3h_batch* b);
 
+/* --------------------------------------------------------------- serving */
+/* A commit server for BabyBear traces: n_slots worker threads, each with its
+ * own context and HIP stream, take requests from one queue.  A request is
+ * admitted when its estimated footprint (trace + LDE + Merkle layers) fits the
+ * HBM budget next to the results still held; a worker takes every admissible
+ * queued request (up to max_batch) and runs their LDEs together, then each
+ * tree.  Results stay resident in HBM for openings until released.  Requests
+ * on different slots overlap on the device; a request's result does not
+ * depend on what else is in flight.  No counterpart in the reference.
+ * consts: BabyBear Poseidon2 constants as for p3h_poseidon2_set. */
+int p3h_serve_create(int device, uint64_t hbm_budget, uint32_t n_slots, uint32_t max_batch, const uint32_t* consts,
+                     uint32_t rounds_f, uint32_t rounds_p, p3h_serve** out);
+/* borrows the trace: the host rows must stay valid and unchanged until the
+ * request is done (p3h_serve_wait / p3h_serve_poll), when the upload has
+ * happened.  Returns P3H_EBUDGET if the request could never fit the budget. */
+int p3h_serve_submit(p3h_serve* sv, const uint32_t* trace, uint64_t height, uint32_t width, uint32_t added_bits,
+                     uint64_t shift, uint64_t* request_id);
+/* blocks until the request is committed; root_out: 8 words; latency: submit -> committed */
+int p3h_serve_wait(p3h_serve* sv, uint64_t request_id, uint32_t* root_out, double* latency_ms);
+int p3h_serve_poll(p3h_serve* sv, uint64_t request_id, int* done);
+/* openings from the resident tree (as p3h_tree_open) */
+int p3h_serve_open(p3h_serve* sv, uint64_t request_id, const uint64_t* indices, uint32_t n_idx, uint32_t* values_out,
+                   uint32_t* proof_out);
+/* drop the request's resident LDE and tree; frees its share of the budget */
+int p3h_serve_release(p3h_serve* sv, uint64_t request_id);
+int p3h_serve_stats(p3h_serve* sv, uint64_t* completed, uint64_t* batches, uint64_t* reserved_bytes,
+                    uint64_t* committed_elems);
+const char* p3h_serve_last_error(p3h_serve* sv);
+int p3h_serve_destroy(p3h_serve* sv);
+
 /* ------------------------------------------------------------ autotuner */
 /* Time the LDE of one shape under candidate pass shapes (columns per LDS tile,
  * stages per pass) and record the fastest for that (field, log_n, width,

@@ -10,6 +10,12 @@
 #include <algorithm>
 #include <cstdio>
 #include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -1520,6 +1526,313 @@ int p3h_tuning_load(p3h_ctx* ctx, const char* path) {
             ctx->tuned[std::make_tuple(field, L, w, k)] = NttTuning{lc, ms};
         std::fclose(f);
     });
+}
+
+}  // extern "C"
+
+// ============================================================ serving ===
+// A commit server: n_slots worker threads, each owning a context (its own HIP
+// stream and allocator), pull requests from one queue.  A worker takes every
+// queued request that fits the remaining HBM budget (up to max_batch), runs
+// their LDEs as one work-list launch group sequence, then each request's tree,
+// and publishes the roots.  Results (LDE + all Merkle layers) stay resident
+// until released, so openings are served from HBM.  Requests of different
+// slots run concurrently on the device.
+struct ServeReq {
+    uint64_t id;
+    const uint32_t* trace;        // caller's host rows, read once at upload
+    uint64_t height;
+    uint32_t width, k;
+    uint32_t shift;
+    uint64_t reserve;             // bytes held against the budget until release
+    std::chrono::steady_clock::time_point t_submit, t_done;
+    bool done = false;
+    int status = P3H_OK;
+    std::string err;
+    uint32_t root[8];
+    p3h_tree* tree = nullptr;
+    int slot = -1;
+};
+
+struct p3h_serve {
+    int device;
+    uint64_t budget;
+    uint64_t reserved = 0;
+    uint32_t max_batch;
+    std::vector<p3h_ctx*> slots;
+    std::vector<std::unique_ptr<std::mutex>> slot_mu;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServeReq*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<ServeReq>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    uint64_t completed = 0, batches = 0, committed_elems = 0;
+    std::string err;
+};
+
+namespace {
+
+size_t cls_bytes(size_t b) { return size_class(b); }
+
+uint64_t serve_footprint(uint64_t h, uint32_t w, uint32_t k) {
+    uint64_t N = h << k;
+    uint64_t tree = 0;
+    for (uint64_t c = N; c >= 1; c >>= 1) {
+        tree += cls_bytes(c * 32);
+        if (c == 1) break;
+    }
+    return cls_bytes(h * w * 4) + cls_bytes(N * w * 4) + tree + cls_bytes(N * 4) + 4096;
+}
+
+void serve_process(p3h_serve* sv, int slot, std::vector<ServeReq*>& batch) {
+    p3h_ctx* ctx = sv->slots[slot];
+    std::lock_guard<std::mutex> lk(*sv->slot_mu[slot]);
+    std::vector<p3h_mat*> traces(batch.size(), nullptr), ldes(batch.size(), nullptr);
+    std::string err;
+    int code = P3H_OK;
+    try {
+        std::vector<LdeJob<uint32_t>> jobs;
+        for (size_t i = 0; i < batch.size(); ++i) {
+            ServeReq* r = batch[i];
+            traces[i] = new_mat(ctx, P3H_BABYBEAR, r->height, r->width);
+            h2d(ctx, traces[i]->data<char>(), r->trace, r->height * r->width * 4);
+            r->trace = nullptr;
+            uint32_t L = log2_exact(r->height);
+            ldes[i] = new_mat(ctx, P3H_BABYBEAR, r->height << r->k, r->width);
+            ldes[i]->k = r->k;
+            ldes[i]->L = L;
+            jobs.push_back(LdeJob<uint32_t>{traces[i]->data<uint32_t>(), ldes[i]->data<uint32_t>(), L, r->width, r->k,
+                                            twiddles<HBB>(ctx, L), lde_twist<HBB>(ctx, L, r->k, r->shift),
+                                            tuning_for(ctx, P3H_BABYBEAR, L, r->width, r->k)});
+        }
+        auto groups = plan_lde_batch<BabyBear>(jobs);
+        std::vector<DevPtr> tables;
+        for (const auto& g : groups) {
+            DevPtr d = dalloc(ctx, g.work.size() * sizeof(NttWork<uint32_t>));
+            h2d(ctx, d->ptr, g.work.data(), g.work.size() * sizeof(NttWork<uint32_t>));
+            tables.push_back(d);
+        }
+        for (size_t g = 0; g < groups.size(); ++g)
+            launch_check(launch_lde_group<BabyBear>(groups[g], dptr<NttWork<uint32_t>>(tables[g]), ctx->stream),
+                         "serve lde");
+        for (size_t i = 0; i < batch.size(); ++i) {
+            const p3h_mat* one = ldes[i];
+            batch[i]->tree = do_mmcs_commit<HBB>(ctx, &one, 1, 0);
+        }
+        for (size_t i = 0; i < batch.size(); ++i)
+            hip_check(hipMemcpyAsync(batch[i]->root, batch[i]->tree->layers.back()->ptr, 32, hipMemcpyDeviceToHost,
+                                     ctx->stream),
+                      "root copy");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    } catch (const Err& e) {
+        code = e.code;
+        err = e.msg;
+    } catch (const std::exception& e) {
+        code = P3H_EDEVICE;
+        err = e.what();
+    }
+    for (auto* m : traces) mat_unref(m);
+    for (auto* m : ldes) mat_unref(m);  // trees hold their own references
+    auto now = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (auto* r : batch) {
+        r->t_done = now;
+        r->done = true;
+        r->status = code;
+        r->err = err;
+        r->slot = slot;
+        if (code != P3H_OK) {
+            if (r->tree) p3h_tree_free(r->tree);
+            r->tree = nullptr;
+            sv->reserved -= r->reserve;
+            r->reserve = 0;
+        } else {
+            sv->committed_elems += (r->height << r->k) * r->width;
+        }
+        ++sv->completed;
+    }
+    ++sv->batches;
+    sv->cv_done.notify_all();
+    sv->cv_work.notify_all();
+}
+
+void serve_worker(p3h_serve* sv, int slot) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (true) {
+        sv->cv_work.wait(lk, [&] {
+            return sv->stop || (!sv->queue.empty() && sv->reserved + sv->queue.front()->reserve <= sv->budget);
+        });
+        if (sv->stop) return;
+        std::vector<ServeReq*> batch;
+        while (!sv->queue.empty() && batch.size() < sv->max_batch &&
+               sv->reserved + sv->queue.front()->reserve <= sv->budget) {
+            ServeReq* r = sv->queue.front();
+            sv->queue.pop_front();
+            sv->reserved += r->reserve;
+            batch.push_back(r);
+        }
+        lk.unlock();
+        serve_process(sv, slot, batch);
+        lk.lock();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int p3h_serve_create(int device, uint64_t hbm_budget, uint32_t n_slots, uint32_t max_batch, const uint32_t* consts,
+                     uint32_t rounds_f, uint32_t rounds_p, p3h_serve** out) {
+    if (!out || !consts || n_slots == 0 || n_slots > 16 || max_batch == 0) return P3H_EINVAL;
+    auto* sv = new p3h_serve{};
+    sv->device = device;
+    sv->max_batch = max_batch;
+    for (uint32_t i = 0; i < n_slots; ++i) {
+        p3h_ctx* c = nullptr;
+        int rc = p3h_ctx_create(device, 0, &c);
+        if (rc == P3H_OK) rc = p3h_poseidon2_set(c, P3H_BABYBEAR, 16, rounds_f, rounds_p, consts);
+        if (rc != P3H_OK) {
+            for (auto* s : sv->slots) p3h_ctx_destroy(s);
+            if (c) p3h_ctx_destroy(c);
+            delete sv;
+            return rc;
+        }
+        sv->slots.push_back(c);
+        sv->slot_mu.emplace_back(new std::mutex);
+    }
+    sv->budget = hbm_budget ? hbm_budget : sv->slots[0]->budget;
+    for (uint32_t i = 0; i < n_slots; ++i) sv->workers.emplace_back(serve_worker, sv, int(i));
+    *out = sv;
+    return P3H_OK;
+}
+
+int p3h_serve_submit(p3h_serve* sv, const uint32_t* trace, uint64_t height, uint32_t width, uint32_t added_bits,
+                     uint64_t shift, uint64_t* id_out) {
+    if (!sv || !trace || !id_out || width == 0 || height == 0 || (height & (height - 1)) || shift >= BB_P ||
+        __builtin_ctzll(height) + added_bits > 27)
+        return P3H_EINVAL;
+    auto t0 = std::chrono::steady_clock::now();
+    auto r = std::make_unique<ServeReq>();
+    r->trace = trace;
+    r->height = height;
+    r->width = width;
+    r->k = added_bits;
+    r->shift = uint32_t(shift);
+    r->reserve = serve_footprint(height, width, added_bits);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (r->reserve > sv->budget) {
+        sv->err = "request footprint exceeds the server's HBM budget";
+        return P3H_EBUDGET;
+    }
+    r->id = sv->next_id++;
+    r->t_submit = t0;
+    *id_out = r->id;
+    sv->queue.push_back(r.get());
+    sv->reqs[r->id] = std::move(r);
+    sv->cv_work.notify_one();
+    return P3H_OK;
+}
+
+int p3h_serve_wait(p3h_serve* sv, uint64_t id, uint32_t* root_out, double* latency_ms) {
+    if (!sv) return P3H_EINVAL;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return P3H_EINVAL;
+    ServeReq* r = it->second.get();
+    sv->cv_done.wait(lk, [&] { return r->done; });
+    if (r->status != P3H_OK) {
+        sv->err = r->err;
+        return r->status;
+    }
+    if (root_out) std::memcpy(root_out, r->root, 32);
+    if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    return P3H_OK;
+}
+
+int p3h_serve_poll(p3h_serve* sv, uint64_t id, int* done) {
+    if (!sv || !done) return P3H_EINVAL;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return P3H_EINVAL;
+    *done = it->second->done ? 1 : 0;
+    return P3H_OK;
+}
+
+int p3h_serve_open(p3h_serve* sv, uint64_t id, const uint64_t* indices, uint32_t n, uint32_t* values,
+                   uint32_t* proof) {
+    if (!sv) return P3H_EINVAL;
+    ServeReq* r = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        if (it == sv->reqs.end()) return P3H_EINVAL;
+        r = it->second.get();
+        sv->cv_done.wait(lk, [&] { return r->done; });
+        if (r->status != P3H_OK || !r->tree) return P3H_ESTATE;
+    }
+    std::lock_guard<std::mutex> g(*sv->slot_mu[r->slot]);
+    int rc = p3h_tree_open(sv->slots[r->slot], r->tree, indices, n, values, proof);
+    if (rc != P3H_OK) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->err = p3h_last_error(sv->slots[r->slot]);
+    }
+    return rc;
+}
+
+int p3h_serve_release(p3h_serve* sv, uint64_t id) {
+    if (!sv) return P3H_EINVAL;
+    std::unique_ptr<ServeReq> r;
+    {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        if (it == sv->reqs.end()) return P3H_EINVAL;
+        sv->cv_done.wait(lk, [&] { return it->second->done; });
+        r = std::move(it->second);
+        sv->reqs.erase(it);
+    }
+    if (r->tree) {
+        std::lock_guard<std::mutex> g(*sv->slot_mu[r->slot]);
+        p3h_tree_free(r->tree);
+        // keep the slot's block cache within its share of the budget
+        p3h_ctx* c = sv->slots[r->slot];
+        if (c->cached > sv->budget / sv->slots.size()) trim(c);
+    }
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->reserved -= r->reserve;
+    sv->cv_work.notify_all();
+    return P3H_OK;
+}
+
+int p3h_serve_stats(p3h_serve* sv, uint64_t* completed, uint64_t* batches, uint64_t* reserved_bytes,
+                    uint64_t* committed_elems) {
+    if (!sv) return P3H_EINVAL;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (completed) *completed = sv->completed;
+    if (batches) *batches = sv->batches;
+    if (reserved_bytes) *reserved_bytes = sv->reserved;
+    if (committed_elems) *committed_elems = sv->committed_elems;
+    return P3H_OK;
+}
+
+const char* p3h_serve_last_error(p3h_serve* sv) { return sv ? sv->err.c_str() : "null server"; }
+
+int p3h_serve_destroy(p3h_serve* sv) {
+    if (!sv) return P3H_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    for (auto& t : sv->workers) t.join();
+    for (auto& kv : sv->reqs)
+        if (kv.second->tree) p3h_tree_free(kv.second->tree);
+    sv->reqs.clear();
+    for (auto* c : sv->slots) p3h_ctx_destroy(c);
+    delete sv;
+    return P3H_OK;
 }
 
 }  // extern "C"

@@ -94,6 +94,15 @@ _SIGS = {
                                 C.POINTER(C.c_double)]),
     "p3h_tuning_save": (_i32, [_vp, C.c_char_p]),
     "p3h_tuning_load": (_i32, [_vp, C.c_char_p]),
+    "p3h_serve_create": (_i32, [_i32, _u64, _u32, _u32, _vp, _u32, _u32, C.POINTER(_vp)]),
+    "p3h_serve_submit": (_i32, [_vp, _vp, _u64, _u32, _u32, _u64, C.POINTER(_u64)]),
+    "p3h_serve_wait": (_i32, [_vp, _u64, _vp, C.POINTER(C.c_double)]),
+    "p3h_serve_poll": (_i32, [_vp, _u64, C.POINTER(_i32)]),
+    "p3h_serve_open": (_i32, [_vp, _u64, _vp, _u32, _vp, _vp]),
+    "p3h_serve_release": (_i32, [_vp, _u64]),
+    "p3h_serve_stats": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
+    "p3h_serve_last_error": (C.c_char_p, [_vp]),
+    "p3h_serve_destroy": (_i32, [_vp]),
 }
 
 
@@ -445,6 +454,84 @@ class Fri:
                 self.free()
         except Exception:
             pass
+
+
+class Server:
+    """Commit server (p3h_serve_*): continuous batching over n_slots streams,
+    HBM-budget admission, resident results for openings."""
+
+    def __init__(self, consts, rounds_f: int = 8, rounds_p: int = 13, device: int = 0, hbm_budget: int = 0,
+                 n_slots: int = 4, max_batch: int = 8):
+        self.lib = load_library()
+        c = np.ascontiguousarray(consts, dtype=np.uint32)
+        h = _vp()
+        rc = self.lib.p3h_serve_create(device, hbm_budget, n_slots, max_batch, _ptr(c), rounds_f, rounds_p,
+                                       C.byref(h))
+        if rc != _OK:
+            raise RuntimeError(f"p3h_serve_create failed with status {rc}")
+        self.h = h
+        self._held = {}  # request id -> host trace the server reads until done
+
+    def _check(self, rc: int):
+        if rc == _OK:
+            return
+        msg = self.lib.p3h_serve_last_error(self.h).decode()
+        if rc == _EINVAL:
+            raise ValueError(msg)
+        if rc == _EBUDGET:
+            raise BudgetError(msg)
+        raise RuntimeError(f"[status {rc}] {msg}")
+
+    def submit(self, trace: np.ndarray, added_bits: int = 1, shift: int = 31) -> int:
+        t = np.ascontiguousarray(trace, dtype=np.uint32)
+        rid = _u64()
+        self._check(self.lib.p3h_serve_submit(self.h, _ptr(t), t.shape[0], t.shape[1], added_bits, shift,
+                                              C.byref(rid)))
+        self._held[rid.value] = t
+        return rid.value
+
+    def wait(self, rid: int):
+        """(root [8], latency_ms)"""
+        root = np.empty(8, dtype=np.uint32)
+        lat = C.c_double()
+        self._check(self.lib.p3h_serve_wait(self.h, rid, _ptr(root), C.byref(lat)))
+        self._held.pop(rid, None)
+        return root, lat.value
+
+    def done(self, rid: int) -> bool:
+        d = _i32()
+        self._check(self.lib.p3h_serve_poll(self.h, rid, C.byref(d)))
+        if d.value:
+            self._held.pop(rid, None)
+        return bool(d.value)
+
+    def open(self, rid: int, indices, width: int, n_layers: int):
+        idx = np.ascontiguousarray(indices, dtype=np.uint64).reshape(-1)
+        vals = np.empty((idx.size, width), dtype=np.uint32)
+        proof = np.empty((idx.size, n_layers - 1, 8), dtype=np.uint32)
+        self._check(self.lib.p3h_serve_open(self.h, rid, _ptr(idx), idx.size, _ptr(vals), _ptr(proof)))
+        return vals, proof
+
+    def release(self, rid: int):
+        self._check(self.lib.p3h_serve_release(self.h, rid))
+        self._held.pop(rid, None)
+
+    def stats(self):
+        a, b, c, d = _u64(), _u64(), _u64(), _u64()
+        self._check(self.lib.p3h_serve_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"completed": a.value, "batches": b.value, "reserved_bytes": c.value, "committed_elems": d.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.p3h_serve_destroy(self.h)  # joins the workers first
+            self.h = None
+            self._held.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # ----------------------------------------------------------- reference API --

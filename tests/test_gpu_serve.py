@@ -1,0 +1,62 @@
+"""Serving queue: every request's commitment and openings are the same alone
+(plain per-request API) and in the mix (continuous batching over several
+streams), and admission respects the HBM budget."""
+
+import numpy as np
+import pytest
+
+import p3hip
+from native import BB_P
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(seed, count, hmin=5, hmax=12):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        h, w = 1 << int(rng.integers(hmin, hmax + 1)), int(rng.integers(8, 48))
+        out.append(rng.integers(0, BB_P, size=(h, w), dtype=np.uint32))
+    return out
+
+
+def test_results_same_alone_and_in_the_mix(ctx, golden, orc):
+    reqs = _requests(5, 16)
+    with p3hip.Server(golden["consts_bb"], 8, 13, n_slots=3, max_batch=4) as sv:
+        ids = [sv.submit(t, 1, 31) for t in reqs]
+        rng = np.random.default_rng(9)
+        for rid, t in zip(ids, reqs):
+            root, lat = sv.wait(rid)
+            assert lat >= 0
+            lde = ctx.coset_lde(ctx.upload(t), 1, 31)
+            tree = ctx.mmcs_commit([lde], 0)
+            assert (root == tree.root()).all()
+            idx = rng.integers(0, 2 * t.shape[0], size=4)
+            v1, p1 = sv.open(rid, idx, t.shape[1], tree.n_layers)
+            v2, p2 = tree.open(idx)
+            assert (v1 == v2).all() and (p1 == p2).all()
+            sv.release(rid)
+        st = sv.stats()
+        assert st["completed"] == 16 and st["reserved_bytes"] == 0
+    # small requests also against the oracle
+    t = reqs[0]
+    want = orc.flat_commit("bb", golden["consts_bb"], 8, 13, orc.lde("bb", t, 1, 31))
+    with p3hip.Server(golden["consts_bb"], 8, 13, n_slots=1, max_batch=1) as sv:
+        rid = sv.submit(t, 1, 31)
+        assert (sv.wait(rid)[0] == want).all()
+
+
+def test_admission_under_a_small_budget(golden):
+    reqs = _requests(7, 8, 10, 10)  # equal sizes
+    one = 4 * (1 << 10) * 48 * 3 + (1 << 11) * 64 + (64 << 10)  # generous single-request footprint
+    with p3hip.Server(golden["consts_bb"], 8, 13, hbm_budget=3 * one, n_slots=2, max_batch=8) as sv:
+        ids = [sv.submit(t, 1, 31) for t in reqs]
+        roots = []
+        for rid in ids:
+            roots.append(sv.wait(rid)[0])
+            assert sv.stats()["reserved_bytes"] <= 3 * one
+            sv.release(rid)
+        assert sv.stats()["completed"] == 8
+    with p3hip.Server(golden["consts_bb"], 8, 13, hbm_budget=1 << 16, n_slots=1) as sv:
+        with pytest.raises(p3hip.BudgetError):
+            sv.submit(reqs[0], 1, 31)
