@@ -48,7 +48,7 @@ WORKLOADS = {
     "gl_2^22x64_b2": ("gl", 22, 64, 2, 22),
 }
 DEFAULT = "bb_2^20x256_b1"
-SPECIAL = ("batch64", "serve")
+SPECIAL = ("batch64", "serve", "fri_bb4_2^23")
 
 
 def load_p3hip():
@@ -72,8 +72,6 @@ def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
     2^15 rows x width, same blowup, LDE + single-matrix Poseidon2 commit, one
     thread.  Falls back to the oracle restatement when the reference build is
     absent."""
-    if field != "bb":
-        return None
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import native  # test infrastructure; used here only as the CPU baseline
 
@@ -81,18 +79,22 @@ def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
     kind = "reference"
     if lib is None:
         lib, kind = native.oracle(), "port"
-    log_n = 15
+    log_n = 15 if field == "bb" else 13
     n = 1 << log_n
     rng = np.random.default_rng(3)
-    trace = rng.integers(0, BB_P, size=(n, width), dtype=np.uint32)
-    consts = p2_constants("bb", rp)
+    if field == "bb":
+        trace = rng.integers(0, BB_P, size=(n, width), dtype=np.uint32)
+    else:
+        trace = rng.integers(0, GL_P, size=(n, width), dtype=np.uint64)
+    consts = p2_constants(field, rp)
     t0 = time.perf_counter()
-    lde = lib.lde("bb", trace, log_blowup, 31)
-    lib.flat_commit("bb", consts, 8, rp, lde)
+    lde = lib.lde(field, trace, log_blowup, 31 if field == "bb" else 7)
+    lib.flat_commit(field, consts, 8, rp, lde)
     dt = time.perf_counter() - t0
     elems = (n << log_blowup) * width
+    name = "BabyBear" if field == "bb" else "Goldilocks"
     return {"value": elems / dt, "unit": "committed field elems/s", "cores": 1, "kind": kind,
-            "sample": f"BabyBear 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
+            "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
                       f"{dt:.2f} s on 1 host thread"}
 
 
@@ -122,6 +124,8 @@ def main():
         return bench_batch(p3hip, args)
     if args.workload == "serve":
         return bench_serve(p3hip, args)
+    if args.workload == "fri_bb4_2^23":
+        return bench_fri(p3hip, args)
     field, log_n, width, lb, rp = WORKLOADS[args.workload]
     fid = p3hip.BABYBEAR if field == "bb" else p3hip.GOLDILOCKS
     n, N = 1 << log_n, 1 << (log_n + lb)
@@ -317,6 +321,72 @@ def bench_batch(p3hip, args):
     }
     print(json.dumps(out))
     plan.free()
+    ctx.close()
+
+
+def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3):
+    """FRI commit phase on a random BabyBear^4 codeword of 2^23 elements: arity 2,
+    log_blowup 1, final polynomial of 2^3 coefficients, seeded challenges; each
+    of the 19 layers (2^23 .. 2^5 elements) is committed with the single-matrix
+    Poseidon2-16 MMCS (2 elements = 8 BabyBear words per leaf), then folded."""
+    consts = p2_constants("bb", 13)
+    ctx = p3hip.Context(0)
+    ctx.set_poseidon2(p3hip.BABYBEAR, consts, 8, 13)
+    ctx.set_hash_engine(args.hash_engine)
+    rng = np.random.default_rng(5)
+    cw = rng.integers(0, BB_P, size=(1 << log_len, 4), dtype=np.uint32)
+    rounds = log_len - log_blowup - lfpl
+    betas = rng.integers(0, BB_P, size=(rounds, 4), dtype=np.uint32)
+    m = ctx.upload(cw)
+
+    def step():
+        roots, final, _ = ctx.fri_commit_phase(m, log_blowup, lfpl, 1, betas)
+        return roots, final
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    outs = [step() for _ in range(args.steps)]
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    phases = {ph: ctx.phase_stats(ph) for ph in ("fold", "leaf_hash", "compress")}
+    ctx.set_profiling(False)
+    assert all((o[0] == outs[0][0]).all() and (o[1] == outs[0][1]).all() for o in outs), "non-deterministic"
+    ms = dt * 1e3 / args.steps
+    layer_elems = sum(1 << l for l in range(lfpl + log_blowup + 1, log_len + 1))
+    fold_ms = phases["fold"][1] / args.steps
+    fold_bytes = sum(24 << l for l in range(lfpl + log_blowup + 1, log_len + 1))  # 16 B read + 8 B written per input
+    out = {
+        "metric": "committed BabyBear^4 elems/s (FRI commit phase)", "value": layer_elems / (ms / 1e3),
+        "unit": "ext field elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded codeword, challenges and Poseidon2 constants)",
+        "config": {"workload": f"FRI commit phase, BabyBear^4 codeword 2^{log_len}, arity 2, log_blowup {log_blowup}, "
+                               f"final poly 2^{lfpl}, {rounds} layer commits", "layer_elems": layer_elems},
+        "phases_ms": {"fold": fold_ms, "leaf_hash": phases["leaf_hash"][1] / args.steps,
+                      "compress": phases["compress"][1] / args.steps},
+        "roofline_fold": {"bound": "hbm", "achieved": fold_bytes / (fold_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": fold_bytes / (fold_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "algorithmic_bytes": fold_bytes, "launches": rounds},
+        "roots_first_last": [[int(x) for x in outs[0][0][0]], [int(x) for x in outs[0][0][-1]]],
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import native  # test infrastructure; used here only as the CPU baseline
+
+        lib, kind = native.reference(), "reference"
+        if lib is None:
+            lib, kind = native.oracle(), "port"
+        sl = 18
+        t0 = time.perf_counter()
+        lib.fri_commit(consts, 8, 13, cw[: 1 << sl], log_blowup, lfpl, 1, betas[: sl - log_blowup - lfpl])
+        cdt = time.perf_counter() - t0
+        se = sum(1 << l for l in range(lfpl + log_blowup + 1, sl + 1))
+        out["cpu_baseline"] = {"value": se / cdt, "unit": "ext field elems/s", "cores": 1, "kind": kind,
+                               "sample": f"codeword 2^{sl}, same parameters, {cdt:.2f} s on 1 host thread"}
+    print(json.dumps(out))
     ctx.close()
 
 

@@ -69,8 +69,10 @@ int p3h_phase_stats(const p3h_ctx* ctx, const char* phase, uint64_t* launches, d
  * 1 = exact binary64 on the FP64 VALU (default; fewer instructions per
  * permutation).  Both give identical digests. */
 int p3h_ctx_set_hash_engine(p3h_ctx* ctx, int engine);
-/* NTT pass shape: log2 columns per LDS tile and max radix-2 stages per pass (0 = default) */
-int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages);
+/* NTT pass shape: log2 columns per LDS tile, max radix-2 stages per pass, and
+ * for coset LDEs log2 of the column slab whose passes run back to back
+ * (0 = default / whole width) */
+int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages, uint32_t log_slab);
 
 /* ------------------------------------------------------------- Poseidon2 */
 /* Replaces the POD Poseidon2Cuda<F,W,RF,RP,D> (poseidon2_cuda.hpp:37-46) and the
@@ -217,12 +219,13 @@ int p3h_serve_destroy(p3h_serve* sv);
 
 /* ------------------------------------------------------------ autotuner */
 /* Time the LDE of one shape under candidate pass shapes (columns per LDS tile,
- * stages per pass) and record the fastest for that (field, log_n, width,
- * added_bits); later LDEs and batch plans of that shape use it.  No
- * counterpart in the reference. */
+ * stages per pass, column slab) and record the fastest for that (field,
+ * log_n, width, added_bits); later LDEs and batch plans of that shape use it
+ * (batch plans ignore the slab).  No counterpart in the reference. */
 int p3h_autotune_lde(p3h_ctx* ctx, int field, uint32_t log_n, uint32_t width, uint32_t added_bits,
-                     uint32_t iters, uint32_t* best_log_cols, uint32_t* best_max_stages, double* best_ms);
-/* the recorded table as text, one "field log_n width added log_cols max_stages" per line */
+                     uint32_t iters, uint32_t* best_log_cols, uint32_t* best_max_stages, uint32_t* best_log_slab,
+                     double* best_ms);
+/* the recorded table as text, one "field log_n width added log_cols max_stages log_slab" per line */
 int p3h_tuning_save(const p3h_ctx* ctx, const char* path);
 int p3h_tuning_load(p3h_ctx* ctx, const char* path);
 

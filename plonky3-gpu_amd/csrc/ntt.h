@@ -14,6 +14,7 @@ namespace p3h {
 struct NttTuning {
     uint32_t log_cols;    // log2 columns per tile (default 4)
     uint32_t max_stages;  // cap on radix-2 stages per LDS pass (default: tile-limited)
+    uint32_t log_slab;    // LDE only: run all passes on slabs of 2^log_slab columns in turn (0 = whole width)
 };
 
 // Arguments of one NTT pass over a row-major matrix (see ntt.hip).
@@ -27,7 +28,8 @@ struct NttArgs {
     uint64_t dst_region_rows;  // row offset between regions in dst
     uint32_t L;                // log2 transform size n
     uint32_t a;                // the pass covers index bits [a, a+S)
-    uint32_t w;                // matrix width (row stride in elements)
+    uint32_t w;                // columns transformed (from src/dst column 0)
+    uint32_t ld;               // row stride of src and dst in elements (>= w)
     uint32_t logC;             // columns per tile = 2^logC
     uint32_t n_col_tiles;
     uint32_t n_regions;        // cosets (LDE) or 1

@@ -183,6 +183,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     const uint32_t c0 = ct << logC;
     const uint64_t n = 1ull << A.L;
     const uint32_t w = A.w;
+    const uint64_t ld = A.ld;
 
     // position (within the length-n transform) of tile row t
     auto pos = [&](uint32_t t) -> uint64_t { return (uint64_t(hi) << (a + S)) | (uint64_t(t) << a) | lo0; };
@@ -193,18 +194,18 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         __syncthreads();
     }
 
-    const T* src = A.src + uint64_t(region) * A.src_region_rows * w;
-    T* dst = A.dst + uint64_t(region) * A.dst_region_rows * w;
+    const T* src = A.src + uint64_t(region) * A.src_region_rows * ld;
+    T* dst = A.dst + uint64_t(region) * A.dst_region_rows * ld;
     auto gload = [&](uint32_t c, uint32_t t) -> T {
         const uint32_t col = c0 + c;
         if (col >= w) return T(0);
         const uint64_t p = pos(t);
         if (MODE == INNER_DIT_GATHER) {
             const uint64_t row = bitrev(uint32_t(p), A.L);
-            T v = src[row * w + col];
+            T v = src[row * ld + col];
             return A.scale ? F::mul_c(v, A.scale[row]) : v;
         }
-        return src[p * w + col];
+        return src[p * ld + col];
     };
     auto gstore = [&](uint32_t c, uint32_t t, T v) {
         const uint32_t col = c0 + c;
@@ -212,9 +213,9 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         const uint64_t p = pos(t);
         if (MODE == INNER_DIF_SCATTER) {
             const uint64_t row = bitrev(uint32_t(p), A.L);
-            dst[row * w + col] = A.scale ? F::mul_c(v, A.scale[row]) : v;
+            dst[row * ld + col] = A.scale ? F::mul_c(v, A.scale[row]) : v;
         } else {
-            dst[p * w + col] = v;
+            dst[p * ld + col] = v;
         }
     };
     auto lload = [&](uint32_t c, uint32_t t) -> T { return lds[c * STRIDE + t]; };
@@ -242,10 +243,10 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         const T* twf = tws + TW_LDS;
         for (int r = int(A.n_regions) - 1; r >= 0; --r) {
             const T* twist = A.scale + uint64_t(r) * n;
-            T* dst_r = A.dst + uint64_t(r) * A.dst_region_rows * w;
+            T* dst_r = A.dst + uint64_t(r) * A.dst_region_rows * ld;
             auto store_r = [&](uint32_t c, uint32_t t, T v) {
                 const uint32_t col = c0 + c;
-                if (col < w) dst_r[pos(t) * w + col] = v;
+                if (col < w) dst_r[pos(t) * ld + col] = v;
             };
             if constexpr (R1 > 0) __syncthreads();  // previous coset's high round has read the tile
             if (active) {
@@ -284,26 +285,34 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     }
     }
 }
+// Workgroups stride over the pass's blocks when there are more blocks than the
+// grid holds (a dispatch counts at most 2^32 - 1 work-items).
 template <class F, int MODE, int S>
-__global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A) {
-    ntt_pass_body<F, MODE, S>(A, blockIdx.x);
+__global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A, uint32_t total) {
+    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+        if (b != blockIdx.x) __syncthreads();  // the previous block is done with the LDS tile
+        ntt_pass_body<F, MODE, S>(A, b);
+    }
 }
 
 // Work-list launch: many transforms (one per descriptor) that share MODE and S
 // run in one grid; block_start[i] is the first block of descriptor i
 // (ascending, block_start[count] = total).
 template <class F, int MODE, int S>
-__global__ __launch_bounds__(NTT_THREADS) void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count) {
-    // wave-uniform binary search for the descriptor owning this block
-    uint32_t lo = 0, hi = count;
-    const uint32_t b = blockIdx.x;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (work[mid].block_start <= b) lo = mid;
-        else hi = mid;
+__global__ __launch_bounds__(NTT_THREADS) void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count,
+                                                             uint32_t total) {
+    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+        if (b != blockIdx.x) __syncthreads();
+        // wave-uniform binary search for the descriptor owning this block
+        uint32_t lo = 0, hi = count;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (work[mid].block_start <= b) lo = mid;
+            else hi = mid;
+        }
+        const NttArgs<typename F::T> A = work[lo].args;
+        ntt_pass_body<F, MODE, S>(A, b - work[lo].block_start);
     }
-    const NttArgs<typename F::T> A = work[lo].args;
-    ntt_pass_body<F, MODE, S>(A, b - work[lo].block_start);
 }
 
 // table[i] = base^i * mult for i < count (multiply-constant form of each value)
@@ -365,7 +374,7 @@ __global__ void coset_to_natural_kernel(const T* in, T* out, uint32_t L, uint32_
 // outer passes split the rest as evenly as possible (top pass first).
 struct Plan {
     uint32_t s_inner;
-    uint32_t outer[4];
+    uint32_t outer[32];  // L <= 32 and every pass has at least one stage
     uint32_t n_outer;
 };
 
@@ -375,6 +384,7 @@ Plan make_plan(uint32_t L, const NttTuning& tune) {
     uint32_t smax = tile_log<T>() - 4;  // keep at least 16 columns per tile
     if (smax > MAX_STAGES) smax = MAX_STAGES;
     if (tune.max_stages && tune.max_stages < smax) smax = tune.max_stages;
+    if (smax == 0) smax = 1;
     Plan p{};
     p.s_inner = L < smax ? L : smax;
     uint32_t rest = L - p.s_inner;
@@ -411,14 +421,18 @@ uint64_t finalize_args(NttArgs<T>& A, int mode, uint32_t s, const NttTuning& tun
     return blocks;
 }
 
+// grid cap: 2^22 workgroups x 512 work-items = 2^31 work-items per dispatch
+constexpr uint64_t MAX_GRID = 1ull << 22;
+
 template <class F, int MODE, int S>
 void launch_s(const NttArgs<typename F::T>& A, uint64_t blocks, const NttWork<typename F::T>* work, uint32_t count,
               hipStream_t st) {
+    const uint32_t grid = uint32_t(blocks < MAX_GRID ? blocks : MAX_GRID);
     if (work)
-        hipLaunchKernelGGL((ntt_pass_list<F, MODE, S>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, work,
-                           count);
+        hipLaunchKernelGGL((ntt_pass_list<F, MODE, S>), dim3(grid), dim3(NTT_THREADS), 0, st, work, count,
+                           uint32_t(blocks));
     else
-        hipLaunchKernelGGL((ntt_pass<F, MODE, S>), dim3(uint32_t(blocks)), dim3(NTT_THREADS), 0, st, A);
+        hipLaunchKernelGGL((ntt_pass<F, MODE, S>), dim3(grid), dim3(NTT_THREADS), 0, st, A, uint32_t(blocks));
 }
 
 template <class F, int MODE>
@@ -457,17 +471,16 @@ void launch_pass(NttArgs<typename F::T> A, uint32_t s, const NttTuning& tune, hi
     launch_mode<F, MODE>(s, A, blocks, nullptr, 0, st);
 }
 
-}  // namespace
-
+// All passes of one LDE over columns [0, w) of row-stride-ld matrices.
 template <class F>
-int lde_device(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t w, uint32_t k,
-               const typename F::T* tw_inv_fwd, const typename F::T* twist, const NttTuning& tune,
-               hipStream_t st) {
+void lde_slab(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t w, uint32_t ld, uint32_t k,
+              const typename F::T* tw_inv_fwd, const typename F::T* twist, const NttTuning& tune, hipStream_t st) {
     Plan p = make_plan<F>(L, tune);
     const uint64_t n = 1ull << L;
     NttArgs<typename F::T> A{};
     A.L = L;
     A.w = w;
+    A.ld = ld;
     A.n_regions = 1;
     A.tw = tw_inv_fwd;  // inverse half first
     // 1. outer DIF passes over the high bits, top down: input -> region 0 of out
@@ -502,6 +515,21 @@ int lde_device(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t
         launch_pass<F, OUTER_DIT_FWD>(A, p.outer[i], tune, st);
         bot += p.outer[i];
     }
+}
+
+}  // namespace
+
+// With tune.log_slab set, the LDE runs slab by slab (2^log_slab columns each):
+// one slab's intermediate rows (2^(L+k) x slab) can stay in the device's
+// last-level cache across its passes instead of streaming every pass through
+// HBM.
+template <class F>
+int lde_device(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t w, uint32_t k,
+               const typename F::T* tw_inv_fwd, const typename F::T* twist, const NttTuning& tune,
+               hipStream_t st) {
+    const uint32_t slab = tune.log_slab ? (1u << tune.log_slab) : w;
+    for (uint32_t c0 = 0; c0 < w; c0 += slab)
+        lde_slab<F>(in + c0, out + c0, L, (w - c0 < slab ? w - c0 : slab), w, k, tw_inv_fwd, twist, tune, st);
     return int(hipGetLastError());
 }
 
@@ -514,6 +542,7 @@ int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F
     NttArgs<typename F::T> A{};
     A.L = L;
     A.w = w;
+    A.ld = w;
     A.n_regions = 1;
     if (kind == 0 || kind == 2) {
         // forward (optionally coset): gather rows in bit-reversed order with
@@ -600,6 +629,7 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
         NttArgs<T> A{};
         A.L = j.L;
         A.w = j.w;
+        A.ld = j.w;
         A.n_regions = 1;
         A.tw = j.tw;
         uint32_t top = j.L;

@@ -1033,12 +1033,12 @@ int p3h_phase_stats(const p3h_ctx* cctx, const char* phase, uint64_t* launches, 
     });
 }
 
-int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages) {
+int p3h_ctx_set_ntt_tuning(p3h_ctx* ctx, uint32_t log_cols, uint32_t max_stages, uint32_t log_slab) {
     return api(ctx, [&] {
-        if (log_cols > 6 || (max_stages && max_stages < 1) || max_stages > 13)
-            throw Err{P3H_EINVAL, "tuning out of range"};
+        if (log_cols > 10 || max_stages > 13 || log_slab > 16) throw Err{P3H_EINVAL, "tuning out of range"};
         ctx->tune.log_cols = log_cols;
         ctx->tune.max_stages = max_stages;
+        ctx->tune.log_slab = log_slab;
     });
 }
 
@@ -1451,7 +1451,7 @@ int p3h_mat_write(p3h_ctx* ctx, p3h_mat* m, const void* host) {
 
 // Time one LDE shape under candidate pass shapes and keep the fastest.
 int p3h_autotune_lde(p3h_ctx* ctx, int field, uint32_t log_n, uint32_t width, uint32_t added_bits, uint32_t iters,
-                     uint32_t* best_log_cols, uint32_t* best_max_stages, double* best_ms) {
+                     uint32_t* best_log_cols, uint32_t* best_max_stages, uint32_t* best_log_slab, double* best_ms) {
     return api(ctx, [&] {
         if (field != P3H_BABYBEAR && field != P3H_GOLDILOCKS) throw Err{P3H_EINVAL, "unknown field"};
         if (width == 0 || iters == 0) throw Err{P3H_EINVAL, "width and iters must be positive"};
@@ -1462,8 +1462,13 @@ int p3h_autotune_lde(p3h_ctx* ctx, int field, uint32_t log_n, uint32_t width, ui
         const uint32_t tile_log = field == P3H_BABYBEAR ? 14 : 13;
         const uint32_t smax = tile_log - 4;
         std::vector<NttTuning> cands;
-        for (uint32_t lc = 2; lc <= 7; ++lc)
-            for (uint32_t ms : {0u, smax - 1, smax - 2, smax - 3}) cands.push_back(NttTuning{lc, ms});
+        for (uint32_t sl : {0u, 4u, 5u, 6u}) {
+            if (sl && (1u << sl) >= width) continue;
+            for (uint32_t lc = 2; lc <= 7; ++lc) {
+                if (sl && lc > sl) continue;
+                for (uint32_t ms : {0u, smax - 1, smax - 2, smax - 3}) cands.push_back(NttTuning{lc, ms, sl});
+            }
+        }
         auto key = std::make_tuple(field, log_n, width, added_bits);
         NttTuning saved = ctx->tune;
         bool had = ctx->tuned.count(key) != 0;
@@ -1499,19 +1504,21 @@ int p3h_autotune_lde(p3h_ctx* ctx, int field, uint32_t log_n, uint32_t width, ui
         (void)prev;
         if (best_log_cols) *best_log_cols = bt.log_cols;
         if (best_max_stages) *best_max_stages = bt.max_stages;
+        if (best_log_slab) *best_log_slab = bt.log_slab;
         if (best_ms) *best_ms = best;
     });
 }
 
-// Tuning table as text lines "field log_n width added log_cols max_stages".
+// Tuning table as text lines "field log_n width added log_cols max_stages log_slab".
 int p3h_tuning_save(const p3h_ctx* ctx, const char* path) {
     auto* c = const_cast<p3h_ctx*>(ctx);
     return api(c, [&] {
         FILE* f = std::fopen(path, "w");
         if (!f) throw Err{P3H_EINVAL, std::string("cannot write ") + path};
         for (const auto& kv : c->tuned)
-            std::fprintf(f, "%d %u %u %u %u %u\n", std::get<0>(kv.first), std::get<1>(kv.first),
-                         std::get<2>(kv.first), std::get<3>(kv.first), kv.second.log_cols, kv.second.max_stages);
+            std::fprintf(f, "%d %u %u %u %u %u %u\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                         std::get<2>(kv.first), std::get<3>(kv.first), kv.second.log_cols, kv.second.max_stages,
+                         kv.second.log_slab);
         std::fclose(f);
     });
 }
@@ -1521,9 +1528,9 @@ int p3h_tuning_load(p3h_ctx* ctx, const char* path) {
         FILE* f = std::fopen(path, "r");
         if (!f) throw Err{P3H_EINVAL, std::string("cannot read ") + path};
         int field;
-        unsigned L, w, k, lc, ms;
-        while (std::fscanf(f, "%d %u %u %u %u %u", &field, &L, &w, &k, &lc, &ms) == 6)
-            ctx->tuned[std::make_tuple(field, L, w, k)] = NttTuning{lc, ms};
+        unsigned L, w, k, lc, ms, sl;
+        while (std::fscanf(f, "%d %u %u %u %u %u %u", &field, &L, &w, &k, &lc, &ms, &sl) == 7)
+            ctx->tuned[std::make_tuple(field, L, w, k)] = NttTuning{lc, ms, sl};
         std::fclose(f);
     });
 }

@@ -60,7 +60,7 @@ _SIGS = {
     "p3h_ctx_trim": (_i32, [_vp]),
     "p3h_ctx_set_profiling": (_i32, [_vp, _i32]),
     "p3h_phase_stats": (_i32, [_vp, C.c_char_p, C.POINTER(_u64), C.POINTER(C.c_double)]),
-    "p3h_ctx_set_ntt_tuning": (_i32, [_vp, _u32, _u32]),
+    "p3h_ctx_set_ntt_tuning": (_i32, [_vp, _u32, _u32, _u32]),
     "p3h_ctx_set_hash_engine": (_i32, [_vp, _i32]),
     "p3h_poseidon2_set": (_i32, [_vp, _i32, _u32, _u32, _u32, _vp]),
     "p3h_poseidon2_permute": (_i32, [_vp, _i32, _vp, _u64]),
@@ -91,7 +91,7 @@ _SIGS = {
     "p3h_batch_tree": (_i32, [_vp, C.POINTER(_vp)]),
     "p3h_batch_free": (_i32, [_vp]),
     "p3h_autotune_lde": (_i32, [_vp, _i32, _u32, _u32, _u32, _u32, C.POINTER(_u32), C.POINTER(_u32),
-                                C.POINTER(C.c_double)]),
+                                C.POINTER(_u32), C.POINTER(C.c_double)]),
     "p3h_tuning_save": (_i32, [_vp, C.c_char_p]),
     "p3h_tuning_load": (_i32, [_vp, C.c_char_p]),
     "p3h_serve_create": (_i32, [_i32, _u64, _u32, _u32, _vp, _u32, _u32, C.POINTER(_vp)]),
@@ -195,8 +195,8 @@ class Context:
         """BabyBear Poseidon2 engine: 0 = uint32 Montgomery, 1 = exact binary64."""
         self._check(self.lib.p3h_ctx_set_hash_engine(self.h, engine))
 
-    def set_ntt_tuning(self, log_cols: int = 0, max_stages: int = 0):
-        self._check(self.lib.p3h_ctx_set_ntt_tuning(self.h, log_cols, max_stages))
+    def set_ntt_tuning(self, log_cols: int = 0, max_stages: int = 0, log_slab: int = 0):
+        self._check(self.lib.p3h_ctx_set_ntt_tuning(self.h, log_cols, max_stages, log_slab))
 
     # -- Poseidon2 --------------------------------------------------------
     def set_poseidon2(self, field: int, consts: np.ndarray, rounds_f: int = 8, rounds_p: int = 13):
@@ -272,10 +272,10 @@ class Context:
         return BatchPlan(self, h, traces)
 
     def autotune_lde(self, field: int, log_n: int, width: int, added_bits: int, iters: int = 3):
-        lc, ms, t = _u32(), _u32(), C.c_double()
+        lc, ms, sl, t = _u32(), _u32(), _u32(), C.c_double()
         self._check(self.lib.p3h_autotune_lde(self.h, field, log_n, width, added_bits, iters, C.byref(lc),
-                                              C.byref(ms), C.byref(t)))
-        return {"log_cols": lc.value, "max_stages": ms.value, "ms": t.value}
+                                              C.byref(ms), C.byref(sl), C.byref(t)))
+        return {"log_cols": lc.value, "max_stages": ms.value, "log_slab": sl.value, "ms": t.value}
 
     def tuning_save(self, path: str):
         self._check(self.lib.p3h_tuning_save(self.h, path.encode()))

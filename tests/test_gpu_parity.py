@@ -279,6 +279,42 @@ def test_autotuned_lde_matches_oracle(ctx, orc):
     assert (ctx.coset_lde(ctx.upload(m), 1, 31).download() == orc.lde("bb", m, 1, 31)).all()
 
 
+@pytest.fixture
+def tuned_ctx(ctx):
+    yield ctx
+    ctx.set_ntt_tuning(0, 0, 0)
+
+
+@pytest.mark.parametrize("log_cols,max_stages,log_slab", [(2, 0, 4), (4, 3, 5), (3, 4, 4), (4, 0, 6)])
+def test_lde_column_slabs_vs_oracle(tuned_ctx, orc, log_cols, max_stages, log_slab):
+    """Slabbed LDEs (every pass of one column slab before the next slab), ragged
+    last slab, both fields, 1 and 2 added bits."""
+    ctx = tuned_ctx
+    ctx.set_ntt_tuning(log_cols, max_stages, log_slab)
+    for f, fid, shift, L, w in (("bb", BB, 31, 11, 37), ("bb", BB, 31, 13, 70), ("gl", GL, 7, 10, 21)):
+        m = rnd(L * 100 + w, (1 << L, w), f)
+        for k in (1, 2):
+            got = ctx.coset_lde(ctx.upload(m), k, shift).download()
+            assert (got == orc.lde(f, m, k, shift)).all(), (f, L, w, k)
+
+
+def test_lde_grid_stride_matches_default(tuned_ctx):
+    """A pass shape whose block count exceeds one dispatch's grid (4-column
+    tiles, 3-stage passes on a 2^20 x 256 trace: 2^24 blocks) gives the same
+    LDE as the default shape: equal Merkle roots of the 2^21 x 256 results."""
+    ctx = tuned_ctx
+    m = ctx.upload(rnd(77, (1 << 20, 256)))
+    roots = []
+    for tune in ((0, 0, 0), (2, 3, 0)):
+        ctx.set_ntt_tuning(*tune)
+        lde = ctx.coset_lde(m, 1, 31)
+        t = ctx.mmcs_commit([lde], 0)
+        roots.append(t.root())
+        t.free()
+        lde.free()
+    assert (roots[0] == roots[1]).all()
+
+
 # ---------------------------------------------- uint32 Montgomery engine --
 @pytest.fixture
 def ctx64(ctx):
