@@ -285,12 +285,26 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     }
     }
 }
+// Workgroups are handed to the 8 XCDs round-robin (workgroup g runs on XCD
+// g mod 8).  Renumbering so that each XCD gets one contiguous run of blocks
+// puts the column tiles of the same rows -- consecutive blocks -- on one XCD
+// at about the same time, where they share that XCD's L2 lines.  A bijection
+// of [0, grid).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t g, uint32_t grid) {
+    constexpr uint32_t XCDS = 8;
+    const uint32_t per = (grid + XCDS - 1) / XCDS;  // blocks on the fuller XCDs
+    const uint32_t full = grid - (per - 1) * XCDS;  // XCDs that get `per` blocks
+    const uint32_t x = g % XCDS, i = g / XCDS;
+    return x < full ? x * per + i : full * per + (x - full) * (per - 1) + i;
+}
+
 // Workgroups stride over the pass's blocks when there are more blocks than the
 // grid holds (a dispatch counts at most 2^32 - 1 work-items).
 template <class F, int MODE, int S>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A, uint32_t total) {
-    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
-        if (b != blockIdx.x) __syncthreads();  // the previous block is done with the LDS tile
+    const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x);
+    for (uint32_t b = b0; b < total; b += gridDim.x) {
+        if (b != b0) __syncthreads();  // the previous block is done with the LDS tile
         ntt_pass_body<F, MODE, S>(A, b);
     }
 }
@@ -301,8 +315,9 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A
 template <class F, int MODE, int S>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count,
                                                              uint32_t total) {
-    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
-        if (b != blockIdx.x) __syncthreads();
+    const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x);
+    for (uint32_t b = b0; b < total; b += gridDim.x) {
+        if (b != b0) __syncthreads();
         // wave-uniform binary search for the descriptor owning this block
         uint32_t lo = 0, hi = count;
         while (hi - lo > 1) {
