@@ -134,21 +134,93 @@ struct Group {
     }
 };
 
-// A full round: every group is loaded with ld(c, t), transformed, stored with
-// st(c, t, v).  Loads and stores are either the LDS tile or global memory.
-template <class F, int S, int R, int Q0, bool DIT, class Ld, class St>
-__device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC, Ld ld, St st) {
+// Tile access.  A group's positions tb + (m << Q0), m < 2^R, lie in one column,
+// so a group computes its bounds test and base address once and element m is
+// the base plus m times a uniform stride.
+
+// The LDS exchange tile, column-major with STRIDE words per column.
+template <class T, uint32_t STRIDE>
+struct LdsIO {
+    T* lds;
+    template <int R, int Q0>
+    __device__ __forceinline__ void load(uint32_t c, uint32_t tb, T* x) const {
+        const T* p = lds + c * STRIDE + tb;
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) x[m] = p[m << Q0];
+    }
+    template <int R, int Q0>
+    __device__ __forceinline__ void store(uint32_t c, uint32_t tb, const T* x) const {
+        T* p = lds + c * STRIDE + tb;
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) p[m << Q0] = x[m];
+    }
+};
+
+// Rows in global memory: tile position t, tile column c is at
+// base[t * tstride + c]; columns from `cols` on read as zero and are not
+// written (the last column tile of a matrix whose width is not a multiple of C).
+template <class T>
+struct GlobalIO {
+    const T* src;
+    T* dst;
+    uint64_t tstride;
+    uint32_t cols;
+    template <int R, int Q0>
+    __device__ __forceinline__ void load(uint32_t c, uint32_t tb, T* x) const {
+        if (c >= cols) {
+#pragma unroll
+            for (uint32_t m = 0; m < (1u << R); ++m) x[m] = T(0);
+            return;
+        }
+        const T* p = src + uint64_t(tb) * tstride + c;
+        const uint64_t step = tstride << Q0;
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) x[m] = p[m * step];
+    }
+    template <int R, int Q0>
+    __device__ __forceinline__ void store(uint32_t c, uint32_t tb, const T* x) const {
+        if (c >= cols) return;
+        T* p = dst + uint64_t(tb) * tstride + c;
+        const uint64_t step = tstride << Q0;
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) p[m * step] = x[m];
+    }
+};
+
+// Element-wise access through ld(c, t) / st(c, t, v), for the bit-reversed
+// gather and scatter of the plain transforms.
+template <class T, class Ld, class St>
+struct ElemIO {
+    Ld ldf;
+    St stf;
+    template <int R, int Q0>
+    __device__ __forceinline__ void load(uint32_t c, uint32_t tb, T* x) const {
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) x[m] = ldf(c, tb + (m << Q0));
+    }
+    template <int R, int Q0>
+    __device__ __forceinline__ void store(uint32_t c, uint32_t tb, const T* x) const {
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) stf(c, tb + (m << Q0), x[m]);
+    }
+};
+template <class T, class Ld, class St>
+__device__ __forceinline__ ElemIO<T, Ld, St> elem_io(Ld ld, St st) {
+    return ElemIO<T, Ld, St>{ld, st};
+}
+
+// A full round: every group is loaded from `in`, transformed, stored to `out`.
+template <class F, int S, int R, int Q0, bool DIT, class In, class Out>
+__device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC, const In& in, const Out& out) {
     using T = typename F::T;
     constexpr uint32_t M = 1u << R;
     const uint32_t groups = ((1u << S) >> R) << logC;
     for (uint32_t gi = threadIdx.x; gi < groups; gi += NTT_THREADS) {
         Group<R, Q0> g(gi, logC);
         T x[M];
-#pragma unroll
-        for (uint32_t m = 0; m < M; ++m) x[m] = ld(g.c, g.tb + (m << Q0));
+        in.template load<R, Q0>(g.c, g.tb, x);
         butterflies<F, R, Q0, DIT>(x, tws + (g.b & ((1u << Q0) - 1)));
-#pragma unroll
-        for (uint32_t m = 0; m < M; ++m) st(g.c, g.tb + (m << Q0), x[m]);
+        out.template store<R, Q0>(g.c, g.tb, x);
     }
 }
 
@@ -185,8 +257,9 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     const uint32_t w = A.w;
     const uint64_t ld = A.ld;
 
-    // position (within the length-n transform) of tile row t
-    auto pos = [&](uint32_t t) -> uint64_t { return (uint64_t(hi) << (a + S)) | (uint64_t(t) << a) | lo0; };
+    // tile position t is transform position row0 + (t << a)
+    const uint64_t row0 = (uint64_t(hi) << (a + S)) | lo0;
+    auto pos = [&](uint32_t t) -> uint64_t { return row0 | (uint64_t(t) << a); };
 
     if constexpr (S > 0) {
         build_twiddles(tws, A.tw, A.L, a, S, lo0);
@@ -196,38 +269,18 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
 
     const T* src = A.src + uint64_t(region) * A.src_region_rows * ld;
     T* dst = A.dst + uint64_t(region) * A.dst_region_rows * ld;
-    auto gload = [&](uint32_t c, uint32_t t) -> T {
-        const uint32_t col = c0 + c;
-        if (col >= w) return T(0);
-        const uint64_t p = pos(t);
-        if (MODE == INNER_DIT_GATHER) {
-            const uint64_t row = bitrev(uint32_t(p), A.L);
-            T v = src[row * ld + col];
-            return A.scale ? F::mul_c(v, A.scale[row]) : v;
-        }
-        return src[p * ld + col];
-    };
-    auto gstore = [&](uint32_t c, uint32_t t, T v) {
-        const uint32_t col = c0 + c;
-        if (col >= w) return;
-        const uint64_t p = pos(t);
-        if (MODE == INNER_DIF_SCATTER) {
-            const uint64_t row = bitrev(uint32_t(p), A.L);
-            dst[row * ld + col] = A.scale ? F::mul_c(v, A.scale[row]) : v;
-        } else {
-            dst[p * ld + col] = v;
-        }
-    };
-    auto lload = [&](uint32_t c, uint32_t t) -> T { return lds[c * STRIDE + t]; };
-    auto lstore = [&](uint32_t c, uint32_t t, T v) { lds[c * STRIDE + t] = v; };
+    const uint64_t tile_off = row0 * ld + c0;
+    const GlobalIO<T> gio{src + tile_off, dst + tile_off, ld << a, w - c0};
+    const LdsIO<T, STRIDE> lio{lds};
 
     if constexpr (MODE == INNER_LDE) {
         // DIF over all S bits, then per coset: twist, DIT over all S bits.
         // The DIF low round and the DIT low round group the same positions, so
         // the coefficients stay in that round's registers across the cosets.
         // The host keeps (2^S >> R0) * C <= NTT_THREADS: one low group per thread.
+        // Inner passes have a = 0: tile position t is row row0 + t.
         if constexpr (R1 > 0) {
-            round_io<F, S, R1, R0, false>(tws, logC, gload, lstore);
+            round_io<F, S, R1, R0, false>(tws, logC, gio, lio);
             __syncthreads();
         }
         constexpr uint32_t M = 1u << R0;
@@ -236,55 +289,85 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         Group<R0, 0> g(threadIdx.x, logC);
         T x[M];
         if (active) {
-#pragma unroll
-            for (uint32_t m = 0; m < M; ++m) x[m] = R1 > 0 ? lload(g.c, g.tb + m) : gload(g.c, g.tb + m);
+            if constexpr (R1 > 0) lio.template load<R0, 0>(g.c, g.tb, x);
+            else gio.template load<R0, 0>(g.c, g.tb, x);
             butterflies<F, R0, 0, false>(x, tws);
         }
         const T* twf = tws + TW_LDS;
         for (int r = int(A.n_regions) - 1; r >= 0; --r) {
-            const T* twist = A.scale + uint64_t(r) * n;
-            T* dst_r = A.dst + uint64_t(r) * A.dst_region_rows * ld;
-            auto store_r = [&](uint32_t c, uint32_t t, T v) {
-                const uint32_t col = c0 + c;
-                if (col < w) dst_r[pos(t) * ld + col] = v;
-            };
+            const T* twist = A.scale + uint64_t(r) * n + row0 + g.tb;
+            T* dst_r = A.dst + uint64_t(r) * A.dst_region_rows * ld + tile_off;
+            const GlobalIO<T> gio_r{nullptr, dst_r, ld, w - c0};
             if constexpr (R1 > 0) __syncthreads();  // previous coset's high round has read the tile
             if (active) {
                 T y[M];
 #pragma unroll
-                for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_c(x[m], twist[pos(g.tb + m)]);
+                for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_c(x[m], twist[m]);
                 butterflies<F, R0, 0, true>(y, twf);
-#pragma unroll
-                for (uint32_t m = 0; m < M; ++m) {
-                    if constexpr (R1 > 0) lstore(g.c, g.tb + m, y[m]);
-                    else store_r(g.c, g.tb + m, y[m]);
-                }
+                if constexpr (R1 > 0) lio.template store<R0, 0>(g.c, g.tb, y);
+                else gio_r.template store<R0, 0>(g.c, g.tb, y);
             }
             if constexpr (R1 > 0) {
                 __syncthreads();
-                round_io<F, S, R1, R0, true>(twf, logC, lload, store_r);
+                round_io<F, S, R1, R0, true>(twf, logC, lio, gio_r);
             }
         }
         return;
-    } else if constexpr (MODE == OUTER_DIT_FWD || MODE == INNER_DIT_GATHER) {
+    } else if constexpr (MODE == INNER_DIT_GATHER) {
+        // load rows bitrev(p) (times scale[row]) of the natural-order input
+        auto ldf = [&](uint32_t c, uint32_t t) -> T {
+            const uint32_t col = c0 + c;
+            if (col >= w) return T(0);
+            const uint64_t row = bitrev(uint32_t(pos(t)), A.L);
+            T v = src[row * ld + col];
+            return A.scale ? F::mul_c(v, A.scale[row]) : v;
+        };
+        auto stf = [&](uint32_t, uint32_t, T) {};
+        const auto gather = elem_io<T>(ldf, stf);
         if constexpr (R1 > 0) {
-            round_io<F, S, R0, 0, true>(tws, logC, gload, lstore);
+            round_io<F, S, R0, 0, true>(tws, logC, gather, lio);
             __syncthreads();
-            round_io<F, S, R1, R0, true>(tws, logC, lload, gstore);
+            round_io<F, S, R1, R0, true>(tws, logC, lio, gio);
         } else {
-            round_io<F, S, R0, 0, true>(tws, logC, gload, gstore);
+            round_io<F, S, R0, 0, true>(tws, logC, gather, gio);
         }
-    } else {
+    } else if constexpr (MODE == INNER_DIF_SCATTER) {
+        // store position p at row bitrev(p), times scale[row]
+        auto ldf = [&](uint32_t, uint32_t) -> T { return T(0); };
+        auto stf = [&](uint32_t c, uint32_t t, T v) {
+            const uint32_t col = c0 + c;
+            if (col >= w) return;
+            const uint64_t row = bitrev(uint32_t(pos(t)), A.L);
+            dst[row * ld + col] = A.scale ? F::mul_c(v, A.scale[row]) : v;
+        };
+        const auto scatter = elem_io<T>(ldf, stf);
         if constexpr (R1 > 0) {
-            round_io<F, S, R1, R0, false>(tws, logC, gload, lstore);
+            round_io<F, S, R1, R0, false>(tws, logC, gio, lio);
             __syncthreads();
-            round_io<F, S, R0, 0, false>(tws, logC, lload, gstore);
+            round_io<F, S, R0, 0, false>(tws, logC, lio, scatter);
         } else {
-            round_io<F, S, R0, 0, false>(tws, logC, gload, gstore);
+            round_io<F, S, R0, 0, false>(tws, logC, gio, scatter);
+        }
+    } else if constexpr (MODE == OUTER_DIT_FWD) {
+        if constexpr (R1 > 0) {
+            round_io<F, S, R0, 0, true>(tws, logC, gio, lio);
+            __syncthreads();
+            round_io<F, S, R1, R0, true>(tws, logC, lio, gio);
+        } else {
+            round_io<F, S, R0, 0, true>(tws, logC, gio, gio);
+        }
+    } else {  // OUTER_DIF_INV
+        if constexpr (R1 > 0) {
+            round_io<F, S, R1, R0, false>(tws, logC, gio, lio);
+            __syncthreads();
+            round_io<F, S, R0, 0, false>(tws, logC, lio, gio);
+        } else {
+            round_io<F, S, R0, 0, false>(tws, logC, gio, gio);
         }
     }
     }
 }
+
 // Workgroups are handed to the 8 XCDs round-robin (workgroup g runs on XCD
 // g mod 8).  Renumbering so that each XCD gets one contiguous run of blocks
 // puts the column tiles of the same rows -- consecutive blocks -- on one XCD
