@@ -265,6 +265,48 @@ __global__ __launch_bounds__(HASH_THREADS) void permute_f64_kernel(uint32_t* st,
     for (int q = 0; q < 16; ++q) st[i * 16 + q] = BBF64::canonical(s[q]);
 }
 
+// Lane-parallel kernels: 16 lanes per state (16 states per 256-thread
+// workgroup).  Used where a launch has few states, so that the latency of one
+// permutation, not the throughput, sets the time.
+constexpr int LANE_THREADS = 256;
+constexpr uint64_t LANE_MAX = 1ull << 15;  // at most this many states per launch go lane-parallel
+
+__global__ __launch_bounds__(LANE_THREADS) void compress_lanes_f64_kernel(const uint32_t* prev, uint64_t prev_count,
+                                                                          uint32_t* next, uint64_t next_count,
+                                                                          P2ParamsF64 p) {
+    const uint32_t lane = threadIdx.x & 15;
+    const uint64_t node = (uint64_t(blockIdx.x) * LANE_THREADS + threadIdx.x) >> 4;
+    double s = 0.0;
+    if (node < next_count) {
+        const uint64_t child = 2 * node + (lane >> 3);  // lanes 0-7 left digest, 8-15 right digest
+        if (child < prev_count) s = double(prev[child * 8 + (lane & 7)]);
+    }
+    s = Poseidon2F64Lanes::permute(s, p, lane);
+    if (node < next_count && lane < 8) next[node * 8 + lane] = BBF64::canonical(s);
+}
+
+__global__ __launch_bounds__(LANE_THREADS) void hash_rows_lanes_f64_kernel(RowSeg<uint32_t> seg, uint64_t rows,
+                                                                           uint32_t* out, P2ParamsF64 p) {
+    const uint32_t lane = threadIdx.x & 15;
+    const uint64_t j = (uint64_t(blockIdx.x) * LANE_THREADS + threadIdx.x) >> 4;
+    const bool live = j < rows;
+    const uint32_t* rowp = live ? seg.ptr + stored_row(seg, j) * seg.w : nullptr;
+    double s = 0.0;
+    for (uint32_t start = 0; start < seg.w; start += 8) {
+        if (live && lane < 8 && start + lane < seg.w) s = double(rowp[start + lane]);  // overwrite mode
+        s = Poseidon2F64Lanes::permute(s, p, lane);
+    }
+    if (live && lane < 8) out[j * 8 + lane] = BBF64::canonical(s);
+}
+
+__global__ __launch_bounds__(LANE_THREADS) void permute_lanes_f64_kernel(uint32_t* st, uint64_t n, P2ParamsF64 p) {
+    const uint32_t lane = threadIdx.x & 15;
+    const uint64_t i = (uint64_t(blockIdx.x) * LANE_THREADS + threadIdx.x) >> 4;
+    double s = i < n ? double(st[i * 16 + lane]) : 0.0;
+    s = Poseidon2F64Lanes::permute(s, p, lane);
+    if (i < n) st[i * 16 + lane] = BBF64::canonical(s);
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t threads) { return uint32_t((n + threads - 1) / threads); }
 
 }  // namespace
@@ -318,6 +360,11 @@ int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, 
 int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t total_w, const RowSeg<uint32_t>& seg0,
                          uint64_t rows, uint32_t* out, const P2ParamsF64& p, hipStream_t st) {
     if (rows == 0) return 0;
+    if (nsegs == 1 && seg0.ptr && seg0.w > 0 && rows <= LANE_MAX) {
+        hipLaunchKernelGGL(hash_rows_lanes_f64_kernel, dim3(grid_for(rows * 16, LANE_THREADS)), dim3(LANE_THREADS), 0,
+                           st, seg0, rows, out, p);
+        return int(hipGetLastError());
+    }
     bool dense = nsegs == 1 && seg0.ptr && seg0.w > 0 && seg0.w % 8 == 0 && (reinterpret_cast<uintptr_t>(seg0.ptr) & 15) == 0;
     if (dense)
         hipLaunchKernelGGL(hash_rows_dense_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st,
@@ -332,6 +379,11 @@ int compress_layer_f64_device(const uint32_t* prev, uint64_t prev_count, uint32_
                               const RowSeg<uint32_t>* inj, uint32_t ninj, uint32_t inj_w, const P2ParamsF64& p,
                               hipStream_t st) {
     if (next_count == 0) return 0;
+    if (ninj == 0 && next_count <= LANE_MAX) {
+        hipLaunchKernelGGL(compress_lanes_f64_kernel, dim3(grid_for(next_count * 16, LANE_THREADS)), dim3(LANE_THREADS),
+                           0, st, prev, prev_count, next, next_count, p);
+        return int(hipGetLastError());
+    }
     hipLaunchKernelGGL(compress_f64_kernel, dim3(grid_for(next_count, HASH_THREADS)), dim3(HASH_THREADS), 0, st, prev,
                        prev_count, next, next_count, ninj ? inj : nullptr, inj_w, p);
     return int(hipGetLastError());
@@ -339,6 +391,11 @@ int compress_layer_f64_device(const uint32_t* prev, uint64_t prev_count, uint32_
 
 int permute_f64_device(uint32_t* states, uint64_t n, const P2ParamsF64& p, hipStream_t st) {
     if (n == 0) return 0;
+    if (n <= LANE_MAX) {
+        hipLaunchKernelGGL(permute_lanes_f64_kernel, dim3(grid_for(n * 16, LANE_THREADS)), dim3(LANE_THREADS), 0, st,
+                           states, n, p);
+        return int(hipGetLastError());
+    }
     hipLaunchKernelGGL(permute_f64_kernel, dim3(grid_for(n, HASH_THREADS)), dim3(HASH_THREADS), 0, st, states, n, p);
     return int(hipGetLastError());
 }

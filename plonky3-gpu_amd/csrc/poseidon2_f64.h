@@ -77,8 +77,8 @@ struct Poseidon2F64 {
         double t0123 = t01 + t23;
         double t01123 = t0123 + x[1];
         double t01233 = t0123 + x[3];
-        double y3 = t01233 + (x[0] + x[0]);
-        double y1 = t01123 + (x[2] + x[2]);
+        double y3 = __builtin_fma(x[0], 2.0, t01233);  // exact: integers below 2^53
+        double y1 = __builtin_fma(x[2], 2.0, t01123);
         double y0 = t01123 + t01;
         double y2 = t01233 + t23;
         x[0] = y0;
@@ -119,6 +119,59 @@ struct Poseidon2F64 {
         for (uint32_t r = 0; r < p.half_full; ++r) full_round(s, p.term[r]);
 #pragma unroll
         for (int i = 0; i < 16; ++i) s[i] = BBF64::reduce(s[i]);
+    }
+};
+
+// The same permutation with one state spread over a 16-lane DPP row: lane i
+// holds s_i, so a wave64 carries 4 states.  The latency of one permutation is
+// about 900 dependent instructions instead of 5261, which is what the narrow
+// levels of a Merkle tree are bound by.  Cross-lane reads use DPP inside a
+// row: quad_perm for the 4x4 circulant, row_ror for sums across the row.  The
+// arithmetic is the exact integer arithmetic of Poseidon2F64 (same values,
+// same bounds; only the order of exact additions differs), so the canonical
+// output is identical.  Every lane of the row must execute it.
+struct Poseidon2F64Lanes {
+    template <int CTRL>
+    __device__ __forceinline__ static double dpp(double v) {
+        // every lane of the row is written (full masks, in-row sources), so no old value is needed
+        const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+        return __hiloint2double(hi, lo);
+    }
+    // quad_perm selectors: lane j of a quad reads lane j+1, j+2, j+3 (mod 4)
+    static constexpr int QP1 = 1 | (2 << 2) | (3 << 4) | (0 << 6);
+    static constexpr int QP2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
+    static constexpr int QP3 = 3 | (0 << 2) | (1 << 4) | (2 << 6);
+    static constexpr int ROR = 0x120;  // row_ror:n = ROR + n
+
+    // MDS-light: y_j = 2 x_j + 3 x_{j+1} + x_{j+2} + x_{j+3} inside each quad,
+    // then every lane adds the sum of its column over the 4 quads.
+    __device__ __forceinline__ static double mds_light(double x) {
+        const double x1 = dpp<QP1>(x), x2 = dpp<QP2>(x), x3 = dpp<QP3>(x);
+        const double sum = (x + x1) + (x2 + x3);
+        const double y = __builtin_fma(x1, 2.0, sum + x);
+        double t = y + dpp<ROR + 4>(y);
+        t = t + dpp<ROR + 8>(t);
+        return y + t;
+    }
+    // s_i <- reduce(2^i s_i + sum of the row)
+    __device__ __forceinline__ static double internal_layer(double s, double diag) {
+        double t = s + dpp<ROR + 8>(s);
+        t = t + dpp<ROR + 4>(t);
+        t = t + dpp<ROR + 2>(t);
+        t = t + dpp<ROR + 1>(t);
+        return BBF64::reduce(__builtin_fma(s, diag, t));
+    }
+    __device__ __forceinline__ static double permute(double s, const P2ParamsF64& p, uint32_t lane) {
+        const double diag = double(1u << lane);
+        s = mds_light(s);
+        for (uint32_t r = 0; r < p.half_full; ++r) s = mds_light(Poseidon2F64::sbox(s + p.init[r][lane]));
+        for (uint32_t r = 0; r < p.partial; ++r) {
+            const double t = Poseidon2F64::sbox(s + p.internal[r]);
+            s = internal_layer(lane == 0 ? t : s, diag);
+        }
+        for (uint32_t r = 0; r < p.half_full; ++r) s = mds_light(Poseidon2F64::sbox(s + p.term[r][lane]));
+        return BBF64::reduce(s);
     }
 };
 
