@@ -53,6 +53,11 @@ int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, 
 // contracts as the uint32 versions above.
 int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t total_w, const RowSeg<uint32_t>& seg0,
                          uint64_t rows, uint32_t* out, const P2ParamsF64& p, hipStream_t st);
+// Levels in_count -> counts[0] -> ... -> counts[nlev-1] (each ceil(prev / 2),
+// zero digest for a missing right child) in one launch; counts[0] <= 64,
+// nlev <= 8.  Binary64 engine, no injection.
+int compress_tail_f64_device(const uint32_t* in, uint64_t in_count, uint32_t* const* outs, const uint64_t* counts,
+                             uint32_t nlev, const P2ParamsF64& p, hipStream_t st);
 int compress_layer_f64_device(const uint32_t* prev, uint64_t prev_count, uint32_t* next, uint64_t next_count,
                               const RowSeg<uint32_t>* inj, uint32_t ninj, uint32_t inj_w, const P2ParamsF64& p,
                               hipStream_t st);

@@ -299,6 +299,44 @@ __global__ __launch_bounds__(LANE_THREADS) void hash_rows_lanes_f64_kernel(RowSe
     if (live && lane < 8) out[j * 8 + lane] = BBF64::canonical(s);
 }
 
+// The last levels of a tree in one workgroup of 64 lane-parallel states:
+// level l (l < nlev) reads level l-1 (level -1 is `in`, in_count digests),
+// writes count[l] = ceil(prev / 2) digests to out[l] and keeps them in LDS for
+// level l+1.  A missing right child is the zero digest.  count[0] <= 64.
+constexpr uint32_t TAIL_THREADS = 1024;
+constexpr uint32_t TAIL_MAX_LEVELS = 8;
+struct TailLevels {
+    const uint32_t* in;
+    uint64_t in_count;
+    uint32_t* out[TAIL_MAX_LEVELS];
+    uint32_t count[TAIL_MAX_LEVELS];
+    uint32_t nlev;
+};
+
+__global__ __launch_bounds__(TAIL_THREADS) void compress_tail_lanes_f64_kernel(TailLevels t, P2ParamsF64 p) {
+    __shared__ uint32_t buf[2][64 * 8];
+    const uint32_t lane = threadIdx.x & 15;
+    const uint32_t node = threadIdx.x >> 4;  // < 64
+    uint32_t prev_count = uint32_t(t.in_count);
+    for (uint32_t l = 0; l < t.nlev; ++l) {
+        const uint32_t cnt = t.count[l];
+        double s = 0.0;
+        if (node < cnt) {
+            const uint32_t child = 2 * node + (lane >> 3);
+            if (child < prev_count)
+                s = double(l == 0 ? t.in[uint64_t(child) * 8 + (lane & 7)] : buf[(l - 1) & 1][child * 8 + (lane & 7)]);
+        }
+        s = Poseidon2F64Lanes::permute(s, p, lane);
+        if (node < cnt && lane < 8) {
+            const uint32_t v = BBF64::canonical(s);
+            buf[l & 1][node * 8 + lane] = v;
+            t.out[l][node * 8 + lane] = v;
+        }
+        __syncthreads();
+        prev_count = cnt;
+    }
+}
+
 __global__ __launch_bounds__(LANE_THREADS) void permute_lanes_f64_kernel(uint32_t* st, uint64_t n, P2ParamsF64 p) {
     const uint32_t lane = threadIdx.x & 15;
     const uint64_t i = (uint64_t(blockIdx.x) * LANE_THREADS + threadIdx.x) >> 4;
@@ -386,6 +424,25 @@ int compress_layer_f64_device(const uint32_t* prev, uint64_t prev_count, uint32_
     }
     hipLaunchKernelGGL(compress_f64_kernel, dim3(grid_for(next_count, HASH_THREADS)), dim3(HASH_THREADS), 0, st, prev,
                        prev_count, next, next_count, ninj ? inj : nullptr, inj_w, p);
+    return int(hipGetLastError());
+}
+
+int compress_tail_f64_device(const uint32_t* in, uint64_t in_count, uint32_t* const* outs, const uint64_t* counts,
+                             uint32_t nlev, const P2ParamsF64& p, hipStream_t st) {
+    if (nlev == 0) return 0;
+    if (nlev > TAIL_MAX_LEVELS || counts[0] > 64 || in_count > 128) return int(hipErrorInvalidValue);
+    TailLevels t{};
+    t.in = in;
+    t.in_count = in_count;
+    t.nlev = nlev;
+    uint64_t prev = in_count;
+    for (uint32_t l = 0; l < nlev; ++l) {
+        if (counts[l] != (prev + 1) / 2) return int(hipErrorInvalidValue);
+        t.out[l] = outs[l];
+        t.count[l] = uint32_t(counts[l]);
+        prev = counts[l];
+    }
+    hipLaunchKernelGGL(compress_tail_lanes_f64_kernel, dim3(1), dim3(TAIL_THREADS), 0, st, t, p);
     return int(hipGetLastError());
 }
 

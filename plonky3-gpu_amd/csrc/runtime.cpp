@@ -573,7 +573,31 @@ p3h_tree* prepare_mmcs(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uin
         t->layers.push_back(leaves);
         t->counts.push_back(maxh);
         uint64_t level = maxh, cap = 1ull << cap_height;
+        const bool f64_tail = H::FIELD == P3H_BABYBEAR && ctx->hash_engine == 1;
         while (level > cap) {
+            // the last levels, when no matrix is left to inject, in one launch
+            const bool injections_left = next < n && mats[order[next]]->height >= cap;
+            if (f64_tail && level <= 128 && !injections_left) {
+                std::vector<uint64_t> counts;
+                std::vector<uint32_t*> outs;
+                for (uint64_t c = level >> 1; c >= cap; c >>= 1) {
+                    DevPtr nl = dalloc(ctx, c * D::DIG * sizeof(T));
+                    outs.push_back(reinterpret_cast<uint32_t*>(dptr<T>(nl)));
+                    counts.push_back(c);
+                    t->layers.push_back(nl);
+                    t->counts.push_back(c);
+                    if (c == cap) break;
+                }
+                const uint32_t* prev = reinterpret_cast<const uint32_t*>(dptr<T>(t->layers[t->layers.size() - 1 - outs.size()]));
+                const uint64_t pc = level;
+                const P2ParamsF64 p64t = ctx->bbp64;
+                out.emplace_back("compress", [=](p3h_ctx* c) {
+                    launch_check(compress_tail_f64_device(prev, pc, outs.data(), counts.data(), uint32_t(outs.size()),
+                                                          p64t, c->stream),
+                                 "compress");
+                });
+                break;
+            }
             level >>= 1;
             DevPtr nl = dalloc(ctx, level * D::DIG * sizeof(T));
             uint32_t wi = 0;
@@ -676,6 +700,25 @@ p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs
         t->counts.push_back(height);
         uint64_t cnt = height;
         while (cnt > 1) {
+            if (H::FIELD == P3H_BABYBEAR && ctx->hash_engine == 1 && cnt <= 128) {
+                // the last levels in one launch
+                std::vector<uint64_t> counts;
+                std::vector<uint32_t*> outs;
+                const uint32_t* prev = reinterpret_cast<const uint32_t*>(dptr<T>(t->layers.back()));
+                for (uint64_t c = (cnt + 1) / 2;; c = (c + 1) / 2) {
+                    DevPtr nl = dalloc(ctx, c * D::DIG * sizeof(T));
+                    outs.push_back(reinterpret_cast<uint32_t*>(dptr<T>(nl)));
+                    counts.push_back(c);
+                    t->layers.push_back(nl);
+                    t->counts.push_back(c);
+                    if (c == 1) break;
+                }
+                Phase ph(ctx, "compress");
+                launch_check(compress_tail_f64_device(prev, cnt, outs.data(), counts.data(), uint32_t(outs.size()),
+                                                      ctx->bbp64, ctx->stream),
+                             "compress");
+                break;
+            }
             uint64_t nxt = (cnt + 1) / 2;
             DevPtr nl = dalloc(ctx, nxt * D::DIG * sizeof(T));
             Phase ph(ctx, "compress");

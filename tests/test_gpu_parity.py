@@ -41,6 +41,16 @@ def test_permutation_many_states(ctx, golden, orc):
     assert (ctx.permute(GL, stg) == orc.permute("gl", golden["consts_gl"], RF, 22, stg)).all()
 
 
+def test_permutation_both_binary64_layouts(ctx, golden, orc):
+    """Up to 2^15 states run one state per 16-lane row; more run one state per
+    thread.  Both against the oracle, including all-(p-1) and all-zero states."""
+    for n in (7, 1 << 15, (1 << 15) + 5):
+        st = rnd(n, 16 * n)
+        st[:16] = BB_P - 1
+        st[16:32] = 0
+        assert (ctx.permute(BB, st) == orc.permute("bb", golden["consts_bb"], RF, 13, st)).all(), n
+
+
 # ------------------------------------------------------------------- DFT --
 def test_dft_golden(ctx, golden):
     dft = p3hip.Radix2DitHip(ctx)
