@@ -586,6 +586,108 @@ std::vector<E4> load_e4(const uint32_t* p, size_t n) {
     return v;
 }
 
+// DuplexChallenger<BabyBear, Poseidon2 width 16, 16, 8> (duplex_challenger.hpp:42-265):
+// observe pushes onto the input buffer (a full buffer of 8 triggers a
+// duplexing); sample duplexes when inputs are pending or no output is left and
+// pops the last output; a duplexing overwrites state[0..len) with the inputs,
+// permutes and exposes state[0..8) as outputs.
+struct Challenger {
+    const Perm<BB, 16>* perm;
+    std::array<uint32_t, 16> st{};
+    std::vector<uint32_t> in, out;
+
+    void duplex() {
+        std::copy(in.begin(), in.end(), st.begin());
+        in.clear();
+        perm->permute(st);
+        out.assign(st.begin(), st.begin() + 8);
+    }
+    void absorb(uint32_t v) {  // absorb_without_invalidation (:51-56)
+        in.push_back(v);
+        if (in.size() == 8) duplex();
+    }
+    void observe(uint32_t v) {  // :93-97
+        out.clear();
+        absorb(v % BB::P);
+    }
+    uint32_t sample() {  // :120-129
+        if (!in.empty() || out.empty()) duplex();
+        uint32_t v = out.back();
+        out.pop_back();
+        return v;
+    }
+    uint64_t sample_bits(size_t bits) {  // :137-148, 31 bits per sample
+        uint64_t r = 0;
+        size_t rem = bits;
+        while (rem > 0) {
+            uint32_t v = sample();
+            size_t chunk = std::min<size_t>(rem, 31);
+            r |= uint64_t(v & ((1u << chunk) - 1u)) << (bits - rem);
+            rem -= chunk;
+        }
+        return r;
+    }
+    void observe_witness(uint64_t w) {  // :58-62
+        out.clear();
+        absorb(uint32_t(w & 0xFFFFFFFFu) % BB::P);
+        absorb(uint32_t(w >> 32) % BB::P);
+    }
+    uint64_t grind(size_t bits) {  // :236-254: the smallest witness whose sample has `bits` zero low bits
+        const uint64_t mask = (uint64_t(1) << bits) - 1;
+        for (uint64_t w = 0;; ++w) {
+            Challenger a = *this;
+            a.observe_witness(w);
+            if ((a.sample() & mask) == 0) {
+                observe_witness(w);
+                sample();
+                return w;
+            }
+        }
+    }
+    bool check_witness(size_t bits, uint64_t w) {  // :262-268
+        observe_witness(w);
+        return (sample() & ((uint64_t(1) << bits) - 1)) == 0;
+    }
+};
+
+// Challenger script: op = code | arg << 8.  0 observe(arg), 1 sample,
+// 2 sample_bits(arg), 3 grind(arg), 4 check_witness(arg & 63, arg >> 6),
+// 5 sample_algebra_element<BabyBear4> (4 results).
+size_t run_script(Challenger& ch, const uint64_t* ops, size_t n_ops, uint64_t* out) {
+    size_t k = 0;
+    for (size_t i = 0; i < n_ops; ++i) {
+        const uint64_t code = ops[i] & 0xFF, arg = ops[i] >> 8;
+        switch (code) {
+            case 0: ch.observe(uint32_t(arg)); break;
+            case 1: out[k++] = ch.sample(); break;
+            case 2: out[k++] = ch.sample_bits(arg); break;
+            case 3: out[k++] = ch.grind(arg); break;
+            case 4: out[k++] = ch.check_witness(arg & 63, arg >> 6) ? 1 : 0; break;
+            case 5:
+                for (int q = 0; q < 4; ++q) out[k++] = ch.sample();
+                break;
+            default: throw std::invalid_argument("challenger op");
+        }
+    }
+    return k;
+}
+
+// Final polynomial of the FRI commit phase (fri_prover.hpp:147-178): truncate
+// to n = 2^lfpl, bit-reverse, naive inverse DFT over <omega_n>.
+std::vector<E4> final_poly(std::vector<E4> cur, size_t lfpl) {
+    size_t n = size_t(1) << lfpl;
+    cur.resize(n);
+    std::vector<E4> br(n), res(n);
+    for (size_t i = 0; i < n; ++i) br[rev_bits(i, lfpl)] = cur[i];
+    uint32_t winv = invf<BB>(BB::root(lfpl)), ninv = invf<BB>(uint32_t(n));
+    for (size_t k = 0; k < n; ++k) {
+        E4 s{0, 0, 0, 0};
+        for (size_t j = 0; j < n; ++j) s = e4_add(s, e4_scale(br[j], powf<BB>(winv, k * j)));
+        res[k] = e4_scale(s, ninv);
+    }
+    return res;
+}
+
 template <class Fn>
 int guarded(Fn&& f) {
     try {
@@ -877,6 +979,69 @@ int orc_bb_pcs_commit(const uint32_t* c, int rf, int rp, size_t n, const size_t*
         }
         auto t = flat_tree<BB>(perm, flat.data(), maxh, pw);
         std::copy(t.back()[0].begin(), t.back()[0].end(), root);
+        return 0;
+    });
+}
+
+// DuplexChallenger driven by a script (encoding in run_script); results in
+// order into out, their count into *n_out.
+int orc_bb_challenger_run(const uint32_t* c, int rf, int rp, const uint64_t* ops, size_t n_ops, uint64_t* out,
+                          size_t* n_out) {
+    return guarded([&] {
+        Perm<BB, 16> perm(c, rf, rp);
+        Challenger ch{&perm, {}, {}, {}};
+        *n_out = run_script(ch, ops, n_ops, out);
+        return 0;
+    });
+}
+
+// prove_fri commit phase (fri_prover.hpp:83-183) with the FriChallenger of the
+// reference's tests (tests/fri_test.cpp:99-124) over DuplexChallenger: after
+// `pre` observations, each round commits the layer, observes its root
+// (observe_merkle_cap), grinds pow_bits, samples beta (4 samples) and folds;
+// then the final polynomial's coefficients and every round's log arity are
+// observed.  post_out receives 8 further samples (the transcript state).
+int orc_bb4_fri_commit_chal(const uint32_t* c, int rf, int rp, const uint32_t* codeword, size_t log_len,
+                            size_t log_blowup, size_t log_final_poly_len, size_t max_log_arity, size_t pow_bits,
+                            const uint32_t* pre, size_t n_pre, uint32_t* roots_out, uint32_t* betas_out,
+                            uint64_t* witnesses_out, uint32_t* final_out, uint32_t* post_out, size_t* n_rounds) {
+    return guarded([&] {
+        Perm<BB, 16> perm(c, rf, rp);
+        Challenger ch{&perm, {}, {}, {}};
+        for (size_t i = 0; i < n_pre; ++i) ch.observe(pre[i]);
+        auto cur = load_e4(codeword, size_t(1) << log_len);
+        size_t lfh = log_blowup + log_final_poly_len, round = 0;
+        std::vector<size_t> arities;
+        while (cur.size() > (size_t(1) << lfh)) {
+            size_t lh = log2_strict(cur.size());
+            size_t la = std::min(lh - lfh, max_log_arity);
+            if (la == 0) break;
+            size_t arity = size_t(1) << la, rows = cur.size() / arity;
+            std::vector<uint32_t> flat(cur.size() * 4);
+            for (size_t i = 0; i < cur.size(); ++i)
+                for (int k = 0; k < 4; ++k) flat[4 * i + k] = cur[i][k];
+            auto t = flat_tree<BB>(perm, flat.data(), rows, 4 * arity);
+            for (int k = 0; k < 8; ++k) {
+                roots_out[8 * round + k] = t.back()[0][k];
+                ch.observe(t.back()[0][k]);
+            }
+            witnesses_out[round] = ch.grind(pow_bits);
+            E4 beta;
+            for (int q = 0; q < 4; ++q) beta[q] = ch.sample();
+            std::copy(beta.begin(), beta.end(), betas_out + 4 * round);
+            cur = fold(lh, la, beta, cur);
+            arities.push_back(la);
+            ++round;
+        }
+        auto fp = final_poly(cur, log_final_poly_len);
+        for (size_t k = 0; k < fp.size(); ++k)
+            for (int q = 0; q < 4; ++q) {
+                final_out[4 * k + q] = fp[k][q];
+                ch.observe(fp[k][q]);
+            }
+        for (size_t la : arities) ch.observe(uint32_t(la));
+        for (int k = 0; k < 8; ++k) post_out[k] = ch.sample();
+        *n_rounds = round;
         return 0;
     });
 }

@@ -95,6 +95,12 @@ int orc_bb4_fri_commit(const uint32_t* consts, int rf, int rp, const uint32_t* c
 
 /* PCS commit: LDE every matrix (added bits = log_blowup, shift per matrix),
  * concatenate rows cyclically, zero-pad the width to a power of two, flat commit */
+int orc_bb_challenger_run(const uint32_t* consts, int rf, int rp, const uint64_t* ops, size_t n_ops, uint64_t* out,
+                          size_t* n_out);
+int orc_bb4_fri_commit_chal(const uint32_t* consts, int rf, int rp, const uint32_t* codeword, size_t log_len,
+                            size_t log_blowup, size_t log_final_poly_len, size_t max_log_arity, size_t pow_bits,
+                            const uint32_t* pre, size_t n_pre, uint32_t* roots_out, uint32_t* betas_out,
+                            uint64_t* witnesses_out, uint32_t* final_out, uint32_t* post_out, size_t* n_rounds);
 int orc_bb_pcs_commit(const uint32_t* consts, int rf, int rp, size_t n_mats, const size_t* log_heights,
                       const size_t* widths, const uint32_t* shifts, const uint32_t* const* mats,
                       size_t log_blowup, uint32_t* root8);

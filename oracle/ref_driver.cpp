@@ -29,6 +29,7 @@
 #include "fri_params.hpp"
 #include "domain.hpp"
 #include "two_adic_fri_pcs.hpp"
+#include "duplex_challenger.hpp"
 #include "p3_util/util.hpp"
 
 #include <cstdint>
@@ -407,6 +408,101 @@ int ref_bb4_fri_commit(const uint32_t* c, int rf, int rp, const uint32_t* codewo
     });
 }
 
+
+using RefChallenger = p3_challenger::DuplexChallenger<BB, RefPerm<BB, 16>, 16, 8>;
+
+// The reference's DuplexChallenger driven by a script: op = code | arg << 8;
+// 0 observe(arg), 1 sample, 2 sample_bits(arg), 3 grind(arg),
+// 4 check_witness(arg & 63, arg >> 6), 5 sample_algebra_element<BabyBear4>.
+int ref_bb_challenger_run(const uint32_t* c, int rf, int rp, const uint64_t* ops, size_t n_ops, uint64_t* out,
+                          size_t* n_out) {
+    return guard([&] {
+        RefChallenger ch(make_perm<BB, uint32_t, 16>(c, rf, rp));
+        size_t k = 0;
+        for (size_t i = 0; i < n_ops; ++i) {
+            const uint64_t code = ops[i] & 0xFF, arg = ops[i] >> 8;
+            switch (code) {
+                case 0: ch.observe(BB(static_cast<uint32_t>(arg))); break;
+                case 1: out[k++] = ch.sample().value(); break;
+                case 2: out[k++] = ch.sample_bits(arg); break;
+                case 3: out[k++] = ch.grind(arg); break;
+                case 4: out[k++] = ch.check_witness(arg & 63, arg >> 6) ? 1 : 0; break;
+                case 5: {
+                    BB4 e = ch.sample_algebra_element<BB4>();
+                    for (int q = 0; q < 4; ++q) out[k++] = e.coeffs[q].value();
+                    break;
+                }
+                default: throw std::invalid_argument("challenger op");
+            }
+        }
+        *n_out = k;
+    });
+}
+
+// prove_fri's commit phase (src/include/fri_prover.hpp:83-190) with the
+// FriChallenger of src/tests/fri_test.cpp:99-124: per round observe_commitment
+// (observe_merkle_cap), grind(pow_bits), sample_challenge (4 samples), fold;
+// then observe_challenge for each final-polynomial coefficient and
+// observe_arity for each round.  pre: values observed before the loop;
+// post_out: 8 samples taken after the loop.
+int ref_bb4_fri_commit_chal(const uint32_t* c, int rf, int rp, const uint32_t* codeword, size_t log_len,
+                            size_t log_blowup, size_t log_final_poly_len, size_t max_log_arity, size_t pow_bits,
+                            const uint32_t* pre, size_t n_pre, uint32_t* roots_out, uint32_t* betas_out,
+                            uint64_t* witnesses_out, uint32_t* final_out, uint32_t* post_out, size_t* n_rounds) {
+    return guard([&] {
+        using S = Stack<BB>;
+        auto perm = make_perm<BB, uint32_t, 16>(c, rf, rp);
+        typename S::FlatMmcs inner{typename S::Hasher(perm), typename S::Compress(perm)};
+        p3_merkle::ExtensionMmcs<BB, BB4, typename S::FlatMmcs> mmcs(inner);
+        RefChallenger ch(perm);
+        for (size_t i = 0; i < n_pre; ++i) ch.observe(BB(pre[i]));
+        size_t len = size_t(1) << log_len;
+        std::vector<BB4> cur(len);
+        for (size_t i = 0; i < len; ++i)
+            cur[i] = BB4(std::array<BB, 4>{BB(codeword[4 * i]), BB(codeword[4 * i + 1]),
+                                           BB(codeword[4 * i + 2]), BB(codeword[4 * i + 3])});
+        size_t log_final_height = log_blowup + log_final_poly_len;
+        size_t round = 0;
+        std::vector<size_t> arities;
+        while (cur.size() > (size_t(1) << log_final_height)) {
+            size_t log_h = p3_util::log2_strict_usize(cur.size());
+            size_t la = p3_fri::compute_log_arity_for_round(log_h, false, 0, log_final_height, max_log_arity);
+            if (la == 0) break;
+            auto [commit, pd] = mmcs.commit_matrix(cur, size_t(1) << la);
+            for (int i = 0; i < 8; ++i) roots_out[8 * round + i] = commit[0][i].value();
+            ch.observe_merkle_cap(commit);
+            witnesses_out[round] = ch.grind(pow_bits);
+            BB4 beta = ch.sample_algebra_element<BB4>();
+            for (int q = 0; q < 4; ++q) betas_out[4 * round + q] = beta.coeffs[q].value();
+            cur = p3_fri::TwoAdicFriFolding<BB, BB4>::fold_matrix(log_h, la, beta, cur);
+            arities.push_back(la);
+            ++round;
+        }
+        size_t fpl = size_t(1) << log_final_poly_len;
+        cur.resize(fpl);
+        p3_util::reverse_slice_index_bits(cur);
+        size_t n = cur.size();
+        BB omega = BB::two_adic_generator(p3_util::log2_strict_usize(n));
+        BB omega_inv = omega.exp_u64(n - 1);
+        BB n_inv = BB(static_cast<uint32_t>(n)).inv();
+        BB wk = BB::one_val();
+        for (size_t k = 0; k < n; ++k) {
+            BB4 sum;
+            BB wkj = BB::one_val();
+            for (size_t j = 0; j < n; ++j) {
+                sum = sum + cur[j] * BB4(wkj);
+                wkj = wkj * wk;
+            }
+            BB4 ck = sum * BB4(n_inv);
+            for (int i = 0; i < 4; ++i) final_out[4 * k + i] = ck.coeffs[i].value();
+            ch.observe_algebra_element<BB4>(ck);
+            wk = wk * omega_inv;
+        }
+        for (size_t la : arities) ch.observe(BB(static_cast<uint32_t>(la)));
+        for (int k = 0; k < 8; ++k) post_out[k] = ch.sample().value();
+        *n_rounds = round;
+    });
+}
 
 // TwoAdicFriPcs::commit (src/include/two_adic_fri_pcs.hpp:207-266) with
 // Radix2Dit and the flat MMCS, as instantiated by src/tests/fri_test.cpp.

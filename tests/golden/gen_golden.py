@@ -22,6 +22,40 @@ sys.path.insert(0, os.path.dirname(HERE))
 from native import U32, U64, p2_consts, reference, seeded  # noqa: E402
 
 
+def challenger_scripts():
+    """Op lists (code, arg) in the encoding of oracle/oracle.cpp run_script."""
+    from native import CHECK_WITNESS, GRIND, OBSERVE, SAMPLE, SAMPLE_BITS, SAMPLE_EXT
+
+    rng = np.random.default_rng(77)
+    # 0: every input-buffer length, then samples across the output-buffer refill
+    s0 = []
+    for n_obs in range(18):
+        s0 += [(OBSERVE, int(v)) for v in rng.integers(0, 1 << 32, size=n_obs)]
+        s0 += [(SAMPLE, 0)] * int(n_obs % 10 + 1)
+    # 1: grinding from every input-buffer length (one or two permutations per witness)
+    s1 = []
+    for ell in range(8):
+        s1 += [(OBSERVE, int(v)) for v in rng.integers(0, 0x78000001, size=8 + ell)]
+        s1 += [(GRIND, int(ell % 7 + 1)), (SAMPLE, 0)]
+    # 2: a random mix, including check_witness on accepted and rejected witnesses
+    s2 = []
+    for _ in range(120):
+        r = int(rng.integers(0, 12))
+        if r < 5:
+            s2.append((OBSERVE, int(rng.integers(0, 1 << 32))))
+        elif r < 7:
+            s2.append((SAMPLE, 0))
+        elif r == 7:
+            s2.append((SAMPLE_BITS, int(rng.integers(1, 64))))
+        elif r == 8:
+            s2.append((SAMPLE_EXT, 0))
+        elif r == 9:
+            s2.append((GRIND, int(rng.integers(0, 11))))
+        else:
+            s2.append((CHECK_WITNESS, int(rng.integers(0, 4)) | (int(rng.integers(0, 40)) << 6)))
+    return [s0, s1, s2]
+
+
 def main():
     ref = reference()
     if ref is None:
@@ -137,6 +171,21 @@ def main():
         g[f"pcs_{case}_shifts"] = np.array(shifts, dtype=U32)
         g[f"pcs_{case}_lb"] = np.array([lb], dtype=U64)
         g[f"pcs_{case}_root"] = ref.pcs_commit(cb, 8, 13, mats, shifts, lb)
+
+    # DuplexChallenger (duplex_challenger.hpp:42-268) on scripted transcripts.
+    for i, ops in enumerate(challenger_scripts()):
+        g[f"chal_{i}_ops"] = np.array(ops, dtype=U64).reshape(-1, 2)
+        g[f"chal_{i}_out"] = ref.challenger_run(cb, 8, 13, ops)
+    # prove_fri's commit phase with the challenger in the loop (grinding per round)
+    for case, (log_len, lb, lfpl, mla, pow_bits, npre) in enumerate(
+            ((9, 1, 2, 1, 0, 3), (10, 1, 1, 2, 6, 0), (9, 2, 0, 3, 4, 11), (11, 1, 3, 1, 9, 8))):
+        cw = seeded(700 + case, (1 << log_len) * 4, "bb").reshape(-1, 4)
+        pre = seeded(710 + case, npre, "bb") if npre else np.zeros(0, dtype=U32)
+        r = ref.fri_commit_chal(cb, 8, 13, cw, lb, lfpl, mla, pow_bits, pre)
+        g[f"fchal_{case}_cw"], g[f"fchal_{case}_pre"] = cw, pre
+        g[f"fchal_{case}_params"] = np.array([log_len, lb, lfpl, mla, pow_bits], dtype=U64)
+        for k, v in r.items():
+            g[f"fchal_{case}_{k}"] = v
 
     out = os.path.join(HERE, "golden.npz")
     np.savez_compressed(out, **g)

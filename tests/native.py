@@ -204,6 +204,38 @@ class Native:
                  "fri_commit")
         return roots[: nr.value * 8].reshape(-1, 8), final.reshape(-1, 4)
 
+    # Challenger ------------------------------------------------------------
+    def challenger_run(self, consts, rf, rp, ops, max_out=4096):
+        """ops: list of (code, arg); see op_* helpers.  Returns the results."""
+        o = np.array([(int(c) & 0xFF) | (int(a) << 8) for c, a in ops], dtype=U64)
+        c = np.ascontiguousarray(consts, dtype=U32)
+        out = np.zeros(max_out, dtype=U64)
+        n = C.c_size_t(0)
+        self._rc(self.fn("bb_challenger_run")(_p(c), C.c_int(rf), C.c_int(rp), _p(o), C.c_size_t(o.size), _p(out),
+                                              C.byref(n)), "challenger_run")
+        return out[: n.value]
+
+    def fri_commit_chal(self, consts, rf, rp, codeword, log_blowup, log_final_poly_len, max_log_arity, pow_bits,
+                        pre=()):
+        cw = np.ascontiguousarray(codeword, dtype=U32).reshape(-1)
+        c = np.ascontiguousarray(consts, dtype=U32)
+        pr = np.ascontiguousarray(np.array(pre, dtype=U32).reshape(-1)) if len(pre) else np.zeros(1, dtype=U32)
+        log_len = (cw.size // 4).bit_length() - 1
+        roots = np.zeros(64 * 8, dtype=U32)
+        betas = np.zeros(64 * 4, dtype=U32)
+        wit = np.zeros(64, dtype=U64)
+        final = np.zeros(4 << log_final_poly_len, dtype=U32)
+        post = np.zeros(8, dtype=U32)
+        nr = C.c_size_t(0)
+        self._rc(self.fn("bb4_fri_commit_chal")(_p(c), C.c_int(rf), C.c_int(rp), _p(cw), C.c_size_t(log_len),
+                                                C.c_size_t(log_blowup), C.c_size_t(log_final_poly_len),
+                                                C.c_size_t(max_log_arity), C.c_size_t(pow_bits), _p(pr),
+                                                C.c_size_t(len(pre)), _p(roots), _p(betas), _p(wit), _p(final),
+                                                _p(post), C.byref(nr)), "fri_commit_chal")
+        r = nr.value
+        return {"roots": roots[: 8 * r].reshape(-1, 8), "betas": betas[: 4 * r].reshape(-1, 4), "witnesses": wit[:r],
+                "final": final.reshape(-1, 4), "post": post}
+
     def pcs_commit(self, consts, rf, rp, mats, shifts, log_blowup):
         ms = [np.ascontiguousarray(m, dtype=U32) for m in mats]
         n = len(ms)
@@ -216,6 +248,14 @@ class Native:
         self._rc(self.fn("bb_pcs_commit")(_p(c), C.c_int(rf), C.c_int(rp), C.c_size_t(n), lh, ws, _p(sh), ptrs,
                                           C.c_size_t(log_blowup), _p(root)), "pcs_commit")
         return root
+
+
+# challenger script op codes (oracle/oracle.cpp run_script)
+OBSERVE, SAMPLE, SAMPLE_BITS, GRIND, CHECK_WITNESS, SAMPLE_EXT = range(6)
+
+
+def check_witness_op(bits: int, witness: int):
+    return (CHECK_WITNESS, bits | (witness << 6))
 
 
 def oracle() -> Native:

@@ -17,6 +17,10 @@ What pins each oracle function (a plausible mistake in it fails at least one):
 | FRI fold                  | golden folds (arity 2/4/8); arity-2 closed form in pure Python integers    |
 | FRI commit phase          | golden layer roots and final polynomials                                   |
 | PCS commit                | golden roots from the reference's TwoAdicFriPcs::commit                    |
+| DuplexChallenger          | golden transcripts (every buffer length, grinding from each input length, |
+|                           | random mixes); grind = the least passing witness (brute force)            |
+| FRI commit + challenger   | golden roots, betas, witnesses, final poly, post-loop samples; the betas  |
+|                           | reproduce the roots through the supplied-beta commit phase                |
 
 When the reference build (oracle/_ref) is present, random cross-checks against
 it run as well.
@@ -240,6 +244,52 @@ def test_fri_commit_golden(orc, golden):
         assert (final == golden[f"fri_{case}_final"]).all(), case
 
 
+def _chal_ops(golden, i):
+    return [(int(c), int(a)) for c, a in golden[f"chal_{i}_ops"]]
+
+
+def test_challenger_golden(orc, golden):
+    i = 0
+    while f"chal_{i}_ops" in golden:
+        got = orc.challenger_run(golden["consts_bb"], RF, 13, _chal_ops(golden, i))
+        assert got.shape == golden[f"chal_{i}_out"].shape and (got == golden[f"chal_{i}_out"]).all(), i
+        i += 1
+    assert i == 3
+
+
+def test_grind_returns_the_least_passing_witness(orc, golden):
+    """grind(bits) from a given transcript returns w with check_witness(bits, w)
+    true and check_witness(bits, v) false for every v < w (each check on a
+    fresh copy of the transcript)."""
+    cb = golden["consts_bb"]
+    rng = np.random.default_rng(3)
+    for ell in range(8):  # every input-buffer length before the witness
+        prefix = [(native.OBSERVE, int(v)) for v in rng.integers(0, BB_P, size=16 + ell)]
+        bits = 5
+        (w,) = orc.challenger_run(cb, RF, 13, prefix + [(native.GRIND, bits)])
+        assert orc.challenger_run(cb, RF, 13, prefix + [native.check_witness_op(bits, int(w))])[0] == 1
+        for v in range(int(w)):
+            assert orc.challenger_run(cb, RF, 13, prefix + [native.check_witness_op(bits, v)])[0] == 0, (ell, v)
+        # after grinding, the transcript equals "observe witness, sample one"
+        after_grind = orc.challenger_run(cb, RF, 13, prefix + [(native.GRIND, bits)] + [(native.SAMPLE, 0)] * 3)
+        after_check = orc.challenger_run(cb, RF, 13, prefix + [native.check_witness_op(bits, int(w))] +
+                                         [(native.SAMPLE, 0)] * 3)
+        assert (after_grind[1:] == after_check[1:]).all(), ell
+
+
+def test_fri_commit_with_challenger_golden(orc, golden):
+    cb = golden["consts_bb"]
+    for case in range(4):
+        log_len, lb, lfpl, mla, pow_bits = (int(v) for v in golden[f"fchal_{case}_params"])
+        r = orc.fri_commit_chal(cb, RF, 13, golden[f"fchal_{case}_cw"], lb, lfpl, mla, pow_bits,
+                                golden[f"fchal_{case}_pre"])
+        for k in ("roots", "betas", "witnesses", "final", "post"):
+            assert (r[k] == golden[f"fchal_{case}_{k}"]).all(), (case, k)
+        # the sampled betas drive the supplied-beta commit phase to the same layers
+        roots, final = orc.fri_commit(cb, RF, 13, golden[f"fchal_{case}_cw"], lb, lfpl, mla, r["betas"])
+        assert (roots == r["roots"]).all() and (final == r["final"]).all(), case
+
+
 def test_pcs_commit_golden(orc, golden):
     for case in range(3):
         mats = _mats(golden, f"pcs_{case}")
@@ -265,3 +315,18 @@ def test_random_cross_check_against_reference(orc, ref):
             assert (x == y).all()
     m = rng.integers(0, BB_P, size=(128, 6), dtype=U32)
     assert (orc.lde("bb", m, 2, 31) == ref.lde("bb", m, 2, 31)).all()
+    for _ in range(5):
+        ops = []
+        for _ in range(50):
+            r = int(rng.integers(0, 10))
+            if r < 5:
+                ops.append((native.OBSERVE, int(rng.integers(0, 1 << 32))))
+            elif r < 7:
+                ops.append((native.SAMPLE, 0))
+            elif r == 7:
+                ops.append((native.SAMPLE_BITS, int(rng.integers(1, 64))))
+            elif r == 8:
+                ops.append((native.SAMPLE_EXT, 0))
+            else:
+                ops.append((native.GRIND, int(rng.integers(0, 8))))
+        assert (orc.challenger_run(cb, RF, 13, ops) == ref.challenger_run(cb, RF, 13, ops)).all()
