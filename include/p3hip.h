@@ -39,6 +39,7 @@ typedef struct p3h_tree p3h_tree; /* device-resident Merkle tree (prover data) *
 typedef struct p3h_fri p3h_fri;   /* device-resident FRI commit-phase data */
 typedef struct p3h_batch p3h_batch; /* planned batched commit */
 typedef struct p3h_serve p3h_serve; /* serving queue */
+typedef struct p3h_chal p3h_chal;   /* device-resident Fiat-Shamir transcript */
 
 enum { P3H_BABYBEAR = 0, P3H_GOLDILOCKS = 1 };
 
@@ -153,9 +154,48 @@ int p3h_fri_commit_phase(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blo
                          uint32_t log_final_poly_len, uint32_t max_log_arity, const uint32_t* betas,
                          uint32_t* roots_out, uint32_t* final_poly_out, uint32_t* n_rounds,
                          p3h_fri** keep);
+/* The same loop with the transcript in the loop, as prove_fri runs it
+ * (fri_prover.hpp:83-190) with the FriChallenger of the reference's tests
+ * (tests/fri_test.cpp:99-124): per round observe_commitment (the layer root),
+ * grind(pow_bits), sample_challenge (beta = 4 samples), fold; afterwards the
+ * final polynomial's coefficients and each round's log arity are observed.
+ * The transcript lives on the device; with pow_bits = 0 no round waits for
+ * the host.  betas_out: 4 words per round; witnesses_out: one per round. */
+int p3h_fri_commit_phase_chal(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup,
+                              uint32_t log_final_poly_len, uint32_t max_log_arity, p3h_chal* challenger,
+                              uint32_t pow_bits, uint32_t* roots_out, uint32_t* betas_out, uint64_t* witnesses_out,
+                              uint32_t* final_poly_out, uint32_t* n_rounds, p3h_fri** keep);
 /* query the kept FRI layers: layer r tree (borrowed pointer, valid until p3h_fri_free) */
 int p3h_fri_layer_tree(const p3h_fri* f, uint32_t round, const p3h_tree** out);
 int p3h_fri_free(p3h_fri* f);
+
+/* ------------------------------------------------------------ challenger */
+/* DuplexChallenger<BabyBear, Poseidon2 W16, 16, 8> (duplex_challenger.hpp:42-268)
+ * with the context's BabyBear Poseidon2 constants, its state in device memory.
+ * Calls run on the context's stream, in order. */
+int p3h_challenger_create(p3h_ctx* ctx, p3h_chal** out);
+/* copy constructor (duplex_challenger.hpp:80-82) */
+int p3h_challenger_clone(const p3h_chal* ch, p3h_chal** out);
+int p3h_challenger_free(p3h_chal* ch);
+/* observe_slice (duplex_challenger.hpp:100-113): n words, each reduced mod p
+ * as BabyBear(uint32_t) does; n = 0 changes nothing */
+int p3h_challenger_observe(p3h_chal* ch, const uint32_t* values, uint32_t n);
+/* observe_merkle_cap (duplex_challenger.hpp:206-220) of a BabyBear tree's cap
+ * (its root for a flat tree), read from device memory */
+int p3h_challenger_observe_cap(p3h_chal* ch, const p3h_tree* tree);
+/* n x sample() (duplex_challenger.hpp:120-129); sample_algebra_element<BabyBear4>
+ * is n = 4 (coefficients in order) */
+int p3h_challenger_sample(p3h_chal* ch, uint32_t n, uint32_t* out);
+/* sample_bits (duplex_challenger.hpp:137-148), bits <= 64 */
+int p3h_challenger_sample_bits(p3h_chal* ch, uint32_t bits, uint64_t* out);
+/* grind (duplex_challenger.hpp:236-254): the least witness whose following
+ * sample has `bits` zero low bits (bits < 31), searched on the GPU; then the
+ * witness is observed and one sample consumed */
+int p3h_challenger_grind(p3h_chal* ch, uint32_t bits, uint64_t* witness);
+/* check_witness (duplex_challenger.hpp:262-268) */
+int p3h_challenger_check_witness(p3h_chal* ch, uint32_t bits, uint64_t witness, int* ok);
+/* state for inspection: sponge[16], input[8], output[8], input length, output length */
+int p3h_challenger_state(p3h_chal* ch, uint32_t* out34);
 
 /* ------------------------------------------------------------------- PCS */
 /* TwoAdicFriPcs::commit (two_adic_fri_pcs.hpp:207-266): LDE every matrix with

@@ -1,0 +1,128 @@
+// Duplex-sponge transcript on the device (reference: DuplexChallenger,
+// duplex_challenger.hpp:42-268) and the grinding search.
+//
+// The transcript runs in one wavefront: the sponge state is spread over a
+// 16-lane row (Poseidon2F64Lanes), the input and output buffers sit in LDS,
+// and the buffer lengths are wave-uniform.  Semantics:
+//   observe(v): output buffer cleared; v appended to the input buffer; a full
+//               buffer (8) triggers a duplexing;
+//   duplexing:  state[0..len) = inputs, inputs cleared, permute, outputs =
+//               state[0..8);
+//   sample():   duplexing if inputs are pending or no output is left; pop the
+//               last output.
+//   observe_witness(w): output cleared; absorb w mod 2^32 and w >> 32 (each
+//               reduced mod p).
+
+#include "challenger.h"
+
+namespace p3h {
+
+namespace {
+
+constexpr uint32_t BB_PRIME = 0x78000001u;
+
+__global__ __launch_bounds__(64) void chal_run_kernel(DevChal* st, ChalProgram prog, P2ParamsF64 p) {
+    __shared__ uint32_t inb[8], outb[8];
+    const uint32_t lane = threadIdx.x;
+    double s = lane < 16 ? double(st->sponge[lane]) : 0.0;
+    if (lane < 8) {
+        inb[lane] = st->input[lane];
+        outb[lane] = st->output[lane];
+    }
+    uint32_t in_len = st->in_len, out_len = st->out_len;
+    __syncthreads();
+
+    auto duplex = [&]() {
+        if (lane < in_len) s = double(inb[lane]);
+        s = Poseidon2F64Lanes::permute(s, p, lane & 15);
+        const uint32_t v = BBF64::canonical(s);
+        s = double(v);
+        if (lane < 8) outb[lane] = v;
+        in_len = 0;
+        out_len = 8;
+        __syncthreads();
+    };
+    auto absorb = [&](uint32_t v) {
+        if (lane == 0) inb[in_len] = v % BB_PRIME;
+        __syncthreads();
+        if (++in_len == 8) duplex();
+    };
+
+    for (uint32_t i = 0; i < prog.n_ops; ++i) {
+        const ChalOp op = prog.op[i];
+        if (op.code == CH_OBSERVE) {
+            out_len = 0;
+            for (uint32_t j = 0; j < op.n; ++j) absorb(op.src[j]);
+        } else if (op.code == CH_OBSERVE_IMM) {
+            out_len = 0;
+            absorb(uint32_t(op.arg));
+        } else if (op.code == CH_WITNESS) {
+            out_len = 0;
+            absorb(uint32_t(op.arg & 0xFFFFFFFFu));
+            absorb(uint32_t(op.arg >> 32));
+        } else if (op.code == CH_SAMPLE) {
+            for (uint32_t j = 0; j < op.n; ++j) {
+                if (in_len > 0 || out_len == 0) duplex();
+                --out_len;
+                if (lane == 0) op.dst[j] = outb[out_len];
+            }
+        }
+    }
+    // write the state back: lane i < 16 its sponge word, lanes < 8 the buffers
+    if (lane < 16) st->sponge[lane] = BBF64::canonical(s);
+    if (lane < 8) {
+        st->input[lane] = inb[lane];
+        st->output[lane] = outb[lane];
+    }
+    if (lane == 0) {
+        st->in_len = in_len;
+        st->out_len = out_len;
+    }
+}
+
+// One thread per witness.  With l pending inputs, observe_witness + sample is
+// one permutation of [inputs, lo, hi, state[l+2..16)] when l <= 6 (for l = 6
+// the second word fills the buffer and the sample is taken from that
+// duplexing's outputs), and for l = 7 two: [inputs, lo, state[8..]] and then
+// [hi, state[1..]].  The sample is word 7 of the last permutation's output.
+__global__ __launch_bounds__(256) void grind_search_kernel(const DevChal* st, uint64_t base, uint32_t count,
+                                                            uint32_t bits, unsigned long long* found, P2ParamsF64 p) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint64_t w = base + t;
+    const uint32_t lo = uint32_t(w & 0xFFFFFFFFu) % BB_PRIME, hi = uint32_t(w >> 32) % BB_PRIME;
+    const uint32_t ell = st->in_len;
+    double s[16];
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t inp = i < 8 ? st->input[i] : 0u;
+        const uint32_t v = i < ell ? inp : (i == ell ? lo : (i == ell + 1 && ell < 7 ? hi : st->sponge[i]));
+        s[i] = double(v);
+    }
+    Poseidon2F64::permute(s, p);
+    if (ell == 7) {
+        s[0] = double(hi);
+        Poseidon2F64::permute(s, p);
+    }
+    const uint32_t v = BBF64::canonical(s[7]);
+    const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    if ((v & mask) == 0) atomicMin(found, (unsigned long long)w);
+}
+
+}  // namespace
+
+int chal_run_device(DevChal* st, const ChalProgram& prog, const P2ParamsF64& p, hipStream_t s) {
+    if (prog.n_ops > CHAL_MAX_OPS) return int(hipErrorInvalidValue);
+    hipLaunchKernelGGL(chal_run_kernel, dim3(1), dim3(64), 0, s, st, prog, p);
+    return int(hipGetLastError());
+}
+
+int grind_search_device(const DevChal* st, uint64_t base, uint32_t count, uint32_t bits,
+                        unsigned long long* found, const P2ParamsF64& p, hipStream_t s) {
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(grind_search_kernel, dim3((count + 255) / 256), dim3(256), 0, s, st, base, count, bits, found,
+                       p);
+    return int(hipGetLastError());
+}
+
+}  // namespace p3h

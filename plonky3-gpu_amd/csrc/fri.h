@@ -10,11 +10,12 @@ namespace p3h {
 
 // Fold a BabyBear^4 vector of 2^log_h elements (canonical, 4 words each) by
 // arity 2^log_arity into n_out = 2^(log_h - log_arity) elements.  beta is
-// canonical (4 words, host).  Arity 2 uses tinv_m (see fri.hip); other arities
-// use omega_h_m = omega_{2^log_h}, w_m = omega_{arity}, arity_inv_m = 1/arity,
-// all in Montgomery form.
+// canonical (4 words), on the host, or -- when beta_dev is not null -- read
+// from device memory by the kernel.  Arity 2 uses tinv_m (see fri.hip); other
+// arities use omega_h_m = omega_{2^log_h}, w_m = omega_{arity}, arity_inv_m =
+// 1/arity, all in Montgomery form.
 int fold_device(const uint32_t* in, uint32_t* out, uint64_t n_out, uint32_t log_h, uint32_t log_arity,
-                const uint32_t* beta, const uint32_t* tinv_m, uint32_t omega_h_m, uint32_t w_m,
-                uint32_t arity_inv_m, hipStream_t st);
+                const uint32_t* beta, const uint32_t* beta_dev, const uint32_t* tinv_m, uint32_t omega_h_m,
+                uint32_t w_m, uint32_t arity_inv_m, hipStream_t st);
 
 }  // namespace p3h

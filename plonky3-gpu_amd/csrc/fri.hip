@@ -97,10 +97,15 @@ __device__ __forceinline__ E4 e4_inv(const E4& a) {
     return e4_scale_m(t, BB::to_mont(ninv));
 }
 
+// beta_dev (nullable): canonical beta in device memory, used instead of beta_m
 __global__ void fold_arity2_kernel(const uint32_t* in, uint32_t* out, uint64_t n_out, E4 beta_m,
-                                   const uint32_t* tinv_m) {
+                                   const uint32_t* beta_dev, const uint32_t* tinv_m) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_out) return;
+    if (beta_dev) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) beta_m.c[q] = BB::to_mont(beta_dev[q]);
+    }
     constexpr uint32_t HALF_M = 0x07FFFFFFu;  // 2^-1 * 2^32 mod p
     const uint4* src = reinterpret_cast<const uint4*>(in) + 2 * i;
     uint4 a = src[0], b = src[1];
@@ -119,10 +124,14 @@ __device__ __forceinline__ uint32_t dbitrev(uint32_t x, uint32_t bits) {
 // L_j = prod_k (beta - t_k) / ((beta - t_j) * x^(a-1) * w^(-j) * a);
 // if beta equals some t_j the row returns e_j.
 __global__ void fold_general_kernel(const uint32_t* in, uint32_t* out, uint64_t n_out, uint32_t log_h,
-                                    uint32_t log_arity, E4 beta, uint32_t omega_h_m, uint32_t w_m,
-                                    uint32_t arity_inv_m) {
+                                    uint32_t log_arity, E4 beta, const uint32_t* beta_dev, uint32_t omega_h_m,
+                                    uint32_t w_m, uint32_t arity_inv_m) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_out) return;
+    if (beta_dev) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) beta.c[q] = beta_dev[q];
+    }
     const uint32_t arity = 1u << log_arity;
     const uint32_t* ev = in + 4 * uint64_t(i) * arity;
     uint32_t x_m = bb_pow_m(omega_h_m, dbitrev(uint32_t(i), log_h - log_arity));
@@ -160,19 +169,20 @@ __global__ void fold_general_kernel(const uint32_t* in, uint32_t* out, uint64_t 
 }  // namespace
 
 int fold_device(const uint32_t* in, uint32_t* out, uint64_t n_out, uint32_t log_h, uint32_t log_arity,
-                const uint32_t* beta, const uint32_t* tinv_m, uint32_t omega_h_m, uint32_t w_m,
-                uint32_t arity_inv_m, hipStream_t st) {
+                const uint32_t* beta, const uint32_t* beta_dev, const uint32_t* tinv_m, uint32_t omega_h_m,
+                uint32_t w_m, uint32_t arity_inv_m, hipStream_t st) {
     if (n_out == 0) return 0;
     uint32_t blocks = uint32_t((n_out + 255) / 256);
+    E4 b = {{0, 0, 0, 0}};
     if (log_arity == 1 && tinv_m) {
-        E4 bm;
-        for (int q = 0; q < 4; ++q) bm.c[q] = uint32_t((uint64_t(beta[q]) << 32) % BabyBear::P);
-        hipLaunchKernelGGL(fold_arity2_kernel, dim3(blocks), dim3(256), 0, st, in, out, n_out, bm, tinv_m);
+        if (!beta_dev)
+            for (int q = 0; q < 4; ++q) b.c[q] = uint32_t((uint64_t(beta[q]) << 32) % BabyBear::P);
+        hipLaunchKernelGGL(fold_arity2_kernel, dim3(blocks), dim3(256), 0, st, in, out, n_out, b, beta_dev, tinv_m);
     } else {
-        E4 b;
-        for (int q = 0; q < 4; ++q) b.c[q] = beta[q];
+        if (!beta_dev)
+            for (int q = 0; q < 4; ++q) b.c[q] = beta[q];
         hipLaunchKernelGGL(fold_general_kernel, dim3(blocks), dim3(256), 0, st, in, out, n_out, log_h, log_arity, b,
-                           omega_h_m, w_m, arity_inv_m);
+                           beta_dev, omega_h_m, w_m, arity_inv_m);
     }
     return int(hipGetLastError());
 }

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "fri.h"
+#include "challenger.h"
 #include "merkle.h"
 #include "ntt.h"
 
@@ -305,6 +306,14 @@ struct p3h_fri {
     p3h_ctx* ctx;
     std::vector<p3h_tree*> trees;
     std::vector<p3h_mat*> layers;
+};
+
+// Device-resident transcript (DevChal) of one context, plus a small device
+// buffer for samples and the grinding result.
+struct p3h_chal {
+    p3h_ctx* ctx;
+    DevPtr st;
+    DevPtr io;  // 64 sample words, then the 64-bit grinding minimum
 };
 
 namespace {
@@ -842,20 +851,24 @@ const uint32_t* fri_tinv(p3h_ctx* ctx, uint32_t LH) {
     return dptr<uint32_t>(buf);
 }
 
-p3h_mat* do_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uint32_t* beta) {
+// beta: canonical, host; or beta_dev (not null): canonical, device memory.
+p3h_mat* do_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uint32_t* beta,
+                 const uint32_t* beta_dev = nullptr) {
     if (cur->field != P3H_BABYBEAR || cur->width != 4 || cur->k != 0)
         throw Err{P3H_EINVAL, "fold input must be a natural-order (n x 4) BabyBear matrix of BabyBear^4 values"};
     uint32_t lh = log2_exact(cur->height);
     if (log_arity < 1 || log_arity > 3 || log_arity > lh) throw Err{P3H_EINVAL, "log_arity must be 1, 2 or 3"};
     if (lh > 27) throw Err{P3H_EINVAL, "height exceeds two-adicity"};
-    for (int q = 0; q < 4; ++q)
-        if (beta[q] >= BB_P) throw Err{P3H_EINVAL, "beta is not canonical"};
+    if (!beta_dev)
+        for (int q = 0; q < 4; ++q)
+            if (beta[q] >= BB_P) throw Err{P3H_EINVAL, "beta is not canonical"};
     uint64_t n_out = cur->height >> log_arity;
     p3h_mat* out = new_mat(ctx, P3H_BABYBEAR, n_out, 4);
     try {
         const uint32_t* tinv = log_arity == 1 && lh >= 1 ? fri_tinv(ctx, lh) : nullptr;
         Phase ph(ctx, "fold");
-        launch_check(fold_device(cur->data<uint32_t>(), out->data<uint32_t>(), n_out, lh, log_arity, beta, tinv,
+        launch_check(fold_device(cur->data<uint32_t>(), out->data<uint32_t>(), n_out, lh, log_arity, beta, beta_dev,
+                                 tinv,
                                  bb_mont(bb_root(lh)), bb_mont(bb_root(log_arity)),
                                  bb_mont(bb_pow(bb_inv(2), log_arity)), ctx->stream),
                      "fold");
@@ -864,6 +877,51 @@ p3h_mat* do_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uin
         throw;
     }
     return out;
+}
+
+// ------------------------------------------------------------ challenger ---
+DevChal* chal_state(p3h_chal* ch) { return dptr<DevChal>(ch->st); }
+uint32_t* chal_io(p3h_chal* ch) { return dptr<uint32_t>(ch->io); }
+unsigned long long* chal_found(p3h_chal* ch) { return reinterpret_cast<unsigned long long*>(chal_io(ch) + 64); }
+
+void chal_run(p3h_chal* ch, const ChalProgram& prog) {
+    p3h_ctx* ctx = ch->ctx;
+    if (!ctx->p2_set[P3H_BABYBEAR]) throw Err{P3H_ESTATE, "Poseidon2 constants for BabyBear not set"};
+    launch_check(chal_run_device(chal_state(ch), prog, ctx->bbp64, ctx->stream), "challenger");
+}
+
+ChalOp chal_op(uint32_t code, uint32_t n = 0, uint64_t arg = 0, const uint32_t* src = nullptr,
+               uint32_t* dst = nullptr) {
+    return ChalOp{code, n, arg, src, dst};
+}
+
+// The least witness whose following sample has `bits` zero low bits
+// (DuplexChallenger::grind, duplex_challenger.hpp:236-254), searched on the
+// device in batches of increasing witnesses; the transcript is not changed.
+uint64_t chal_search(p3h_chal* ch, uint32_t bits) {
+    p3h_ctx* ctx = ch->ctx;
+    if (bits >= 31) throw Err{P3H_EINVAL, "proof-of-work bits must be below 31"};
+    const unsigned long long none = ~0ull;
+    const uint32_t batch = uint32_t(std::min<uint64_t>(std::max<uint64_t>(4ull << bits, 1ull << 16), 1ull << 24));
+    for (uint64_t base = 0;; base += batch) {
+        h2d(ctx, chal_found(ch), &none, sizeof(none));
+        launch_check(grind_search_device(chal_state(ch), base, batch, bits, chal_found(ch), ctx->bbp64, ctx->stream),
+                     "grind");
+        unsigned long long w = none;
+        d2h(ctx, &w, chal_found(ch), sizeof(w));
+        if (w != none) return w;
+    }
+}
+
+// grind(bits): search, then observe the witness and consume one sample.
+uint64_t chal_grind(p3h_chal* ch, uint32_t bits) {
+    const uint64_t w = bits == 0 ? 0 : chal_search(ch, bits);  // with 0 bits every witness passes
+    ChalProgram prog{};
+    prog.op[0] = chal_op(CH_WITNESS, 0, w);
+    prog.op[1] = chal_op(CH_SAMPLE, 1, 0, nullptr, chal_io(ch));
+    prog.n_ops = 2;
+    chal_run(ch, prog);
+    return w;
 }
 
 p3h_mat* view_of(p3h_mat* base, uint64_t h, uint32_t w) {
@@ -999,6 +1057,122 @@ int api(p3h_ctx* ctx, Fn&& fn) {
 }  // namespace
 
 // ================================================================== C ABI ===
+namespace {
+
+// prove_fri's commit loop (fri_prover.hpp:83-190) for one input codeword.
+// Each round commits the current vector as a (len/arity) x (4*arity) matrix
+// (ExtensionMmcs over the flat MMCS) and folds it with that round's beta:
+// betas[4*round..] from the caller, or -- with a challenger -- the transcript
+// observes the root, grinds pow_bits and samples beta on the device, and the
+// fold reads beta from device memory, so no round waits for the host unless
+// pow_bits > 0.  The transcript then observes the final polynomial's
+// coefficients and every round's log arity (fri_prover.hpp:180-189).
+void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, uint32_t log_final_poly_len,
+                   uint32_t max_log_arity, const uint32_t* betas, p3h_chal* ch, uint32_t pow_bits,
+                   uint32_t* roots_out, uint32_t* betas_out, uint64_t* witnesses_out, uint32_t* final_poly_out,
+                   uint32_t* n_rounds, p3h_fri** keep) {
+    if (!codeword || !roots_out || !final_poly_out) throw Err{P3H_EINVAL, "null argument"};
+    if (ch && (!betas_out || !witnesses_out)) throw Err{P3H_EINVAL, "null argument"};
+    if (codeword->field != P3H_BABYBEAR || codeword->width != 4 || codeword->k != 0)
+        throw Err{P3H_EINVAL, "codeword must be a natural-order (n x 4) BabyBear matrix"};
+    if (max_log_arity < 1 || max_log_arity > 3) throw Err{P3H_EINVAL, "max_log_arity must be 1..3"};
+    if (ch && pow_bits >= 31) throw Err{P3H_EINVAL, "proof-of-work bits must be below 31"};
+    uint32_t lh0 = log2_exact(codeword->height);
+    uint32_t lfh = log_blowup + log_final_poly_len;
+    if (lh0 <= lfh) throw Err{P3H_EINVAL, "codeword must be longer than the final domain"};
+    auto* f = new p3h_fri{};
+    f->ctx = ctx;
+    try {
+        DevPtr betas_dev = ch ? dalloc(ctx, 4 * 32 * sizeof(uint32_t)) : DevPtr();
+        std::vector<uint32_t> arities;
+        p3h_mat* cur = const_cast<p3h_mat*>(codeword);
+        mat_ref(cur);
+        f->layers.push_back(cur);
+        uint32_t round = 0;
+        while (cur->height > (1ull << lfh)) {
+            uint32_t lh = log2_exact(cur->height);
+            uint32_t la = std::min(lh - lfh, max_log_arity);
+            uint32_t arity = 1u << la;
+            p3h_mat* view = view_of(cur, cur->height >> la, 4 * arity);
+            p3h_tree* t = nullptr;
+            try {
+                t = flat_tree<HBB>(ctx, {seg_of<HBB>(view)}, view->height, view->width, {view});
+            } catch (...) {
+                mat_unref(view);
+                throw;
+            }
+            mat_unref(view);  // the tree holds its own reference
+            f->trees.push_back(t);
+            const uint32_t* beta_dev = nullptr;
+            if (ch) {
+                uint32_t* bd = dptr<uint32_t>(betas_dev) + 4 * round;
+                const uint32_t* root = dptr<uint32_t>(t->layers.back());
+                ChalProgram pr{};
+                pr.op[pr.n_ops++] = chal_op(CH_OBSERVE, 8, 0, root);  // observe_commitment
+                uint64_t w = 0;
+                if (pow_bits > 0) {
+                    chal_run(ch, pr);
+                    w = chal_search(ch, pow_bits);
+                    pr.n_ops = 0;
+                }
+                pr.op[pr.n_ops++] = chal_op(CH_WITNESS, 0, w);                       // grind: witness ...
+                pr.op[pr.n_ops++] = chal_op(CH_SAMPLE, 1, 0, nullptr, chal_io(ch));  // ... and its sample
+                pr.op[pr.n_ops++] = chal_op(CH_SAMPLE, 4, 0, nullptr, bd);           // sample_challenge
+                chal_run(ch, pr);
+                witnesses_out[round] = w;
+                beta_dev = bd;
+            }
+            p3h_mat* nxt = do_fold(ctx, cur, la, ch ? nullptr : betas + 4 * round, beta_dev);
+            f->layers.push_back(nxt);
+            cur = nxt;
+            arities.push_back(la);
+            ++round;
+        }
+        for (uint32_t r = 0; r < round; ++r) d2h(ctx, roots_out + 8 * r, f->trees[r]->layers.back()->ptr, 32);
+        if (ch) d2h(ctx, betas_out, dptr<uint32_t>(betas_dev), 4 * round * sizeof(uint32_t));
+        // final polynomial: truncate to 2^log_final_poly_len, bit-reverse, inverse DFT (host, tiny)
+        uint32_t n = 1u << log_final_poly_len;
+        std::vector<uint32_t> last(uint64_t(cur->height) * 4);
+        d2h(ctx, last.data(), cur->data<uint32_t>(), last.size() * 4);
+        std::vector<uint32_t> br(4 * n);
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t j = 0;
+            for (uint32_t b = 0; b < log_final_poly_len; ++b) j |= ((i >> b) & 1) << (log_final_poly_len - 1 - b);
+            for (int q = 0; q < 4; ++q) br[4 * j + q] = last[4 * i + q];
+        }
+        uint32_t winv = bb_inv(bb_root(log_final_poly_len)), ninv = bb_inv(n);
+        for (uint32_t k = 0; k < n; ++k)
+            for (int q = 0; q < 4; ++q) {
+                uint64_t acc = 0;
+                for (uint32_t j = 0; j < n; ++j)
+                    acc = (acc + uint64_t(bb_mul(br[4 * j + q], bb_pow(winv, uint64_t(k) * j)))) % BB_P;
+                final_poly_out[4 * k + q] = bb_mul(uint32_t(acc), ninv);
+            }
+        if (ch) {
+            // observe_challenge for each coefficient, then observe_arity for each round
+            std::vector<uint32_t> obs(final_poly_out, final_poly_out + 4 * n);
+            obs.insert(obs.end(), arities.begin(), arities.end());
+            DevPtr d = dalloc(ctx, obs.size() * sizeof(uint32_t));
+            h2d(ctx, d->ptr, obs.data(), obs.size() * sizeof(uint32_t));
+            ChalProgram pr{};
+            pr.op[0] = chal_op(CH_OBSERVE, uint32_t(obs.size()), 0, dptr<uint32_t>(d));
+            pr.n_ops = 1;
+            chal_run(ch, pr);
+            hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        }
+        if (n_rounds) *n_rounds = round;
+    } catch (...) {
+        p3h_fri_free(f);
+        throw;
+    }
+    if (keep)
+        *keep = f;
+    else
+        p3h_fri_free(f);
+}
+
+}  // namespace
+
 extern "C" {
 
 int p3h_ctx_create(int device, uint64_t budget, p3h_ctx** out) {
@@ -1319,68 +1493,155 @@ int p3h_fri_commit_phase(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blo
                          uint32_t max_log_arity, const uint32_t* betas, uint32_t* roots_out,
                          uint32_t* final_poly_out, uint32_t* n_rounds, p3h_fri** keep) {
     return api(ctx, [&] {
-        if (!codeword || !betas || !roots_out || !final_poly_out) throw Err{P3H_EINVAL, "null argument"};
-        if (codeword->field != P3H_BABYBEAR || codeword->width != 4 || codeword->k != 0)
-            throw Err{P3H_EINVAL, "codeword must be a natural-order (n x 4) BabyBear matrix"};
-        if (max_log_arity < 1 || max_log_arity > 3) throw Err{P3H_EINVAL, "max_log_arity must be 1..3"};
-        uint32_t lh0 = log2_exact(codeword->height);
-        uint32_t lfh = log_blowup + log_final_poly_len;
-        if (lh0 <= lfh) throw Err{P3H_EINVAL, "codeword must be longer than the final domain"};
-        auto* f = new p3h_fri{};
-        f->ctx = ctx;
+        if (!betas) throw Err{P3H_EINVAL, "null argument"};
+        do_fri_commit(ctx, codeword, log_blowup, log_final_poly_len, max_log_arity, betas, nullptr, 0, roots_out,
+                      nullptr, nullptr, final_poly_out, n_rounds, keep);
+    });
+}
+
+int p3h_fri_commit_phase_chal(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup,
+                              uint32_t log_final_poly_len, uint32_t max_log_arity, p3h_chal* ch, uint32_t pow_bits,
+                              uint32_t* roots_out, uint32_t* betas_out, uint64_t* witnesses_out,
+                              uint32_t* final_poly_out, uint32_t* n_rounds, p3h_fri** keep) {
+    return api(ctx, [&] {
+        if (!ch || ch->ctx != ctx) throw Err{P3H_EINVAL, "challenger missing or of another context"};
+        do_fri_commit(ctx, codeword, log_blowup, log_final_poly_len, max_log_arity, nullptr, ch, pow_bits,
+                      roots_out, betas_out, witnesses_out, final_poly_out, n_rounds, keep);
+    });
+}
+
+// ------------------------------------------------------------ challenger ---
+int p3h_challenger_create(p3h_ctx* ctx, p3h_chal** out) {
+    return api(ctx, [&] {
+        if (!out) throw Err{P3H_EINVAL, "null argument"};
+        auto* ch = new p3h_chal{ctx, dalloc(ctx, sizeof(DevChal)), dalloc(ctx, 64 * sizeof(uint32_t) + 16)};
+        const DevChal zero{};
         try {
-            p3h_mat* cur = const_cast<p3h_mat*>(codeword);
-            mat_ref(cur);
-            f->layers.push_back(cur);
-            uint32_t round = 0;
-            while (cur->height > (1ull << lfh)) {
-                uint32_t lh = log2_exact(cur->height);
-                uint32_t la = std::min(lh - lfh, max_log_arity);
-                uint32_t arity = 1u << la;
-                p3h_mat* view = view_of(cur, cur->height >> la, 4 * arity);
-                p3h_tree* t = nullptr;
-                try {
-                    t = flat_tree<HBB>(ctx, {seg_of<HBB>(view)}, view->height, view->width, {view});
-                } catch (...) {
-                    mat_unref(view);
-                    throw;
-                }
-                mat_unref(view);  // the tree holds its own reference
-                f->trees.push_back(t);
-                p3h_mat* nxt = do_fold(ctx, cur, la, betas + 4 * round);
-                f->layers.push_back(nxt);
-                cur = nxt;
-                ++round;
-            }
-            for (uint32_t r = 0; r < round; ++r)
-                d2h(ctx, roots_out + 8 * r, f->trees[r]->layers.back()->ptr, 32);
-            // final polynomial: truncate to 2^log_final_poly_len, bit-reverse, inverse DFT (host, tiny)
-            uint32_t n = 1u << log_final_poly_len;
-            std::vector<uint32_t> last(uint64_t(cur->height) * 4);
-            d2h(ctx, last.data(), cur->data<uint32_t>(), last.size() * 4);
-            std::vector<uint32_t> br(4 * n);
-            for (uint32_t i = 0; i < n; ++i) {
-                uint32_t j = 0;
-                for (uint32_t b = 0; b < log_final_poly_len; ++b) j |= ((i >> b) & 1) << (log_final_poly_len - 1 - b);
-                for (int q = 0; q < 4; ++q) br[4 * j + q] = last[4 * i + q];
-            }
-            uint32_t winv = bb_inv(bb_root(log_final_poly_len)), ninv = bb_inv(n);
-            for (uint32_t k = 0; k < n; ++k)
-                for (int q = 0; q < 4; ++q) {
-                    uint64_t acc = 0;
-                    for (uint32_t j = 0; j < n; ++j)
-                        acc = (acc + uint64_t(bb_mul(br[4 * j + q], bb_pow(winv, uint64_t(k) * j)))) % BB_P;
-                    final_poly_out[4 * k + q] = bb_mul(uint32_t(acc), ninv);
-                }
-            if (n_rounds) *n_rounds = round;
+            h2d(ctx, ch->st->ptr, &zero, sizeof(zero));
         } catch (...) {
-            p3h_fri_free(f);
+            delete ch;
             throw;
         }
-        if (keep)
-            *keep = f;
-        else
-            p3h_fri_free(f);
+        *out = ch;
+    });
+}
+
+int p3h_challenger_clone(const p3h_chal* ch, p3h_chal** out) {
+    if (!ch) return P3H_EINVAL;
+    p3h_ctx* ctx = ch->ctx;
+    return api(ctx, [&] {
+        if (!out) throw Err{P3H_EINVAL, "null argument"};
+        auto* c = new p3h_chal{ctx, dalloc(ctx, sizeof(DevChal)), dalloc(ctx, 64 * sizeof(uint32_t) + 16)};
+        if (hipMemcpyAsync(c->st->ptr, ch->st->ptr, sizeof(DevChal), hipMemcpyDeviceToDevice, ctx->stream) !=
+            hipSuccess) {
+            delete c;
+            throw Err{P3H_EDEVICE, "challenger copy failed"};
+        }
+        *out = c;
+    });
+}
+
+int p3h_challenger_free(p3h_chal* ch) {
+    if (!ch) return P3H_EINVAL;
+    delete ch;
+    return P3H_OK;
+}
+
+int p3h_challenger_observe(p3h_chal* ch, const uint32_t* values, uint32_t n) {
+    if (!ch) return P3H_EINVAL;
+    return api(ch->ctx, [&] {
+        if (n == 0) return;  // an empty slice changes nothing (duplex_challenger.hpp:105-108)
+        if (!values) throw Err{P3H_EINVAL, "null argument"};
+        DevPtr d = dalloc(ch->ctx, uint64_t(n) * sizeof(uint32_t));
+        h2d(ch->ctx, d->ptr, values, uint64_t(n) * sizeof(uint32_t));
+        ChalProgram pr{};
+        pr.op[pr.n_ops++] = chal_op(CH_OBSERVE, n, 0, dptr<uint32_t>(d));
+        chal_run(ch, pr);
+    });
+}
+
+int p3h_challenger_observe_cap(p3h_chal* ch, const p3h_tree* t) {
+    if (!ch) return P3H_EINVAL;
+    return api(ch->ctx, [&] {
+        if (!t || t->field != P3H_BABYBEAR || t->ctx != ch->ctx)
+            throw Err{P3H_EINVAL, "tree missing, not BabyBear, or of another context"};
+        ChalProgram pr{};
+        pr.op[pr.n_ops++] = chal_op(CH_OBSERVE, uint32_t(t->counts.back() * 8), 0, dptr<uint32_t>(t->layers.back()));
+        chal_run(ch, pr);
+    });
+}
+
+int p3h_challenger_sample(p3h_chal* ch, uint32_t n, uint32_t* out) {
+    if (!ch) return P3H_EINVAL;
+    return api(ch->ctx, [&] {
+        if (n && !out) throw Err{P3H_EINVAL, "null argument"};
+        for (uint32_t off = 0; off < n; off += 64) {
+            const uint32_t m = std::min(64u, n - off);
+            ChalProgram pr{};
+            pr.op[pr.n_ops++] = chal_op(CH_SAMPLE, m, 0, nullptr, chal_io(ch));
+            chal_run(ch, pr);
+            d2h(ch->ctx, out + off, chal_io(ch), m * sizeof(uint32_t));
+        }
+    });
+}
+
+int p3h_challenger_sample_bits(p3h_chal* ch, uint32_t bits, uint64_t* out) {
+    if (!ch) return P3H_EINVAL;
+    return api(ch->ctx, [&] {
+        if (!out || bits > 64) throw Err{P3H_EINVAL, "bits must be 0..64"};
+        const uint32_t n = (bits + 30) / 31;
+        uint32_t v[3] = {0, 0, 0};
+        if (n) {
+            ChalProgram pr{};
+            pr.op[pr.n_ops++] = chal_op(CH_SAMPLE, n, 0, nullptr, chal_io(ch));
+            chal_run(ch, pr);
+            d2h(ch->ctx, v, chal_io(ch), n * sizeof(uint32_t));
+        }
+        uint64_t r = 0;
+        uint32_t rem = bits;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t chunk = std::min(rem, 31u);
+            r |= uint64_t(v[i] & ((1u << chunk) - 1u)) << (bits - rem);
+            rem -= chunk;
+        }
+        *out = r;
+    });
+}
+
+int p3h_challenger_grind(p3h_chal* ch, uint32_t bits, uint64_t* witness) {
+    if (!ch) return P3H_EINVAL;
+    return api(ch->ctx, [&] {
+        if (!witness) throw Err{P3H_EINVAL, "null argument"};
+        *witness = chal_grind(ch, bits);
+        hip_check(hipStreamSynchronize(ch->ctx->stream), "sync");
+    });
+}
+
+int p3h_challenger_check_witness(p3h_chal* ch, uint32_t bits, uint64_t witness, int* ok) {
+    if (!ch) return P3H_EINVAL;
+    return api(ch->ctx, [&] {
+        if (!ok || bits > 63) throw Err{P3H_EINVAL, "bits must be 0..63"};
+        ChalProgram pr{};
+        pr.op[pr.n_ops++] = chal_op(CH_WITNESS, 0, witness);
+        pr.op[pr.n_ops++] = chal_op(CH_SAMPLE, 1, 0, nullptr, chal_io(ch));
+        chal_run(ch, pr);
+        uint32_t v = 0;
+        d2h(ch->ctx, &v, chal_io(ch), sizeof(v));
+        *ok = (uint64_t(v) & ((1ull << bits) - 1)) == 0 ? 1 : 0;
+    });
+}
+
+int p3h_challenger_state(p3h_chal* ch, uint32_t* out34) {
+    if (!ch) return P3H_EINVAL;
+    return api(ch->ctx, [&] {
+        if (!out34) throw Err{P3H_EINVAL, "null argument"};
+        DevChal h{};
+        d2h(ch->ctx, &h, chal_state(ch), sizeof(h));
+        std::memcpy(out34, h.sponge, 16 * 4);
+        std::memcpy(out34 + 16, h.input, 8 * 4);
+        std::memcpy(out34 + 24, h.output, 8 * 4);
+        out34[32] = h.in_len;
+        out34[33] = h.out_len;
     });
 }
 

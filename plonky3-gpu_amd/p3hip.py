@@ -81,7 +81,19 @@ _SIGS = {
     "p3h_tree_free": (_i32, [_vp]),
     "p3h_fri_fold": (_i32, [_vp, _vp, _u32, _vp, C.POINTER(_vp)]),
     "p3h_fri_commit_phase": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, C.POINTER(_u32), C.POINTER(_vp)]),
+    "p3h_fri_commit_phase_chal": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp, _vp, C.POINTER(_u32),
+                                         C.POINTER(_vp)]),
     "p3h_fri_layer_tree": (_i32, [_vp, _u32, C.POINTER(_vp)]),
+    "p3h_challenger_create": (_i32, [_vp, C.POINTER(_vp)]),
+    "p3h_challenger_clone": (_i32, [_vp, C.POINTER(_vp)]),
+    "p3h_challenger_free": (_i32, [_vp]),
+    "p3h_challenger_observe": (_i32, [_vp, _vp, _u32]),
+    "p3h_challenger_observe_cap": (_i32, [_vp, _vp]),
+    "p3h_challenger_sample": (_i32, [_vp, _u32, _vp]),
+    "p3h_challenger_sample_bits": (_i32, [_vp, _u32, C.POINTER(_u64)]),
+    "p3h_challenger_grind": (_i32, [_vp, _u32, C.POINTER(_u64)]),
+    "p3h_challenger_check_witness": (_i32, [_vp, _u32, _u64, C.POINTER(_i32)]),
+    "p3h_challenger_state": (_i32, [_vp, _vp]),
     "p3h_fri_free": (_i32, [_vp]),
     "p3h_pcs_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u32, C.POINTER(_vp)]),
     "p3h_batch_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u64, _u32, C.POINTER(_vp)]),
@@ -304,6 +316,29 @@ class Context:
         fri = Fri(self, kp) if keep else None
         return roots[: nr.value], final, fri
 
+    def challenger(self) -> "Challenger":
+        return Challenger(self)
+
+    def fri_commit_phase_chal(self, codeword: "Mat", log_blowup: int, log_final_poly_len: int, max_log_arity: int,
+                              challenger: "Challenger", pow_bits: int = 0, keep: bool = False):
+        """prove_fri's commit loop with the transcript in the loop.  Returns a dict
+        with roots, betas, witnesses, final (and fri when keep)."""
+        roots = np.zeros((32, 8), dtype=np.uint32)
+        betas = np.zeros((32, 4), dtype=np.uint32)
+        wit = np.zeros(32, dtype=np.uint64)
+        final = np.zeros((1 << log_final_poly_len, 4), dtype=np.uint32)
+        nr = _u32()
+        kp = _vp()
+        self._check(self.lib.p3h_fri_commit_phase_chal(self.h, codeword.h, log_blowup, log_final_poly_len,
+                                                       max_log_arity, challenger.h, pow_bits, _ptr(roots),
+                                                       _ptr(betas), _ptr(wit), _ptr(final), C.byref(nr),
+                                                       C.byref(kp) if keep else None))
+        r = nr.value
+        out = {"roots": roots[:r], "betas": betas[:r], "witnesses": wit[:r], "final": final}
+        if keep:
+            out["fri"] = Fri(self, kp)
+        return out
+
 
 class Mat:
     """Device-resident matrix handle."""
@@ -422,6 +457,70 @@ class BatchPlan:
     def free(self):
         if self.h:
             self.ctx.lib.p3h_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Challenger:
+    """DuplexChallenger<BabyBear, Poseidon2 W16, 16, 8> with its state on the
+    device (p3h_challenger_*), using the context's BabyBear Poseidon2 constants."""
+
+    def __init__(self, ctx: "Context", h=None):
+        self.ctx = ctx
+        if h is None:
+            h = _vp()
+            ctx._check(ctx.lib.p3h_challenger_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def clone(self) -> "Challenger":
+        h = _vp()
+        self.ctx._check(self.ctx.lib.p3h_challenger_clone(self.h, C.byref(h)))
+        return Challenger(self.ctx, h)
+
+    def observe(self, values):
+        v = np.ascontiguousarray(np.atleast_1d(values), dtype=np.uint32).reshape(-1)
+        self.ctx._check(self.ctx.lib.p3h_challenger_observe(self.h, _ptr(v), v.size))
+
+    def observe_cap(self, tree: "Tree"):
+        self.ctx._check(self.ctx.lib.p3h_challenger_observe_cap(self.h, tree.h))
+
+    def sample(self, n: int = 1) -> np.ndarray:
+        out = np.zeros(max(1, n), dtype=np.uint32)
+        self.ctx._check(self.ctx.lib.p3h_challenger_sample(self.h, n, _ptr(out)))
+        return out[:n]
+
+    def sample_ext(self) -> np.ndarray:
+        return self.sample(4)
+
+    def sample_bits(self, bits: int) -> int:
+        r = _u64()
+        self.ctx._check(self.ctx.lib.p3h_challenger_sample_bits(self.h, bits, C.byref(r)))
+        return r.value
+
+    def grind(self, bits: int) -> int:
+        w = _u64()
+        self.ctx._check(self.ctx.lib.p3h_challenger_grind(self.h, bits, C.byref(w)))
+        return w.value
+
+    def check_witness(self, bits: int, witness: int) -> bool:
+        ok = _i32()
+        self.ctx._check(self.ctx.lib.p3h_challenger_check_witness(self.h, bits, witness, C.byref(ok)))
+        return bool(ok.value)
+
+    def state(self) -> np.ndarray:
+        out = np.zeros(34, dtype=np.uint32)
+        self.ctx._check(self.ctx.lib.p3h_challenger_state(self.h, _ptr(out)))
+        return out
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.p3h_challenger_free(self.h)
             self.h = None
 
     def __del__(self):
