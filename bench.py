@@ -48,7 +48,7 @@ WORKLOADS = {
     "gl_2^22x64_b2": ("gl", 22, 64, 2, 22),
 }
 DEFAULT = "bb_2^20x256_b1"
-SPECIAL = ("batch64", "serve", "fri_bb4_2^23")
+SPECIAL = ("batch64", "serve", "fri_bb4_2^23", "fri_bb4_2^23_chal", "grind_bb")
 
 
 def load_p3hip():
@@ -126,6 +126,10 @@ def main():
         return bench_serve(p3hip, args)
     if args.workload == "fri_bb4_2^23":
         return bench_fri(p3hip, args)
+    if args.workload == "fri_bb4_2^23_chal":
+        return bench_fri(p3hip, args, challenger=True)
+    if args.workload == "grind_bb":
+        return bench_grind(p3hip, args)
     field, log_n, width, lb, rp = WORKLOADS[args.workload]
     fid = p3hip.BABYBEAR if field == "bb" else p3hip.GOLDILOCKS
     n, N = 1 << log_n, 1 << (log_n + lb)
@@ -335,11 +339,14 @@ def bench_batch(p3hip, args):
     ctx.close()
 
 
-def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3):
+def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3, challenger: bool = False):
     """FRI commit phase on a random BabyBear^4 codeword of 2^23 elements: arity 2,
-    log_blowup 1, final polynomial of 2^3 coefficients, seeded challenges; each
-    of the 19 layers (2^23 .. 2^5 elements) is committed with the single-matrix
-    Poseidon2-16 MMCS (2 elements = 8 BabyBear words per leaf), then folded."""
+    log_blowup 1, final polynomial of 2^3 coefficients; each of the 19 layers
+    (2^23 .. 2^5 elements) is committed with the single-matrix Poseidon2-16 MMCS
+    (2 elements = 8 BabyBear words per leaf), then folded.  Challenges are
+    seeded, or -- with `challenger` -- drawn from the device-resident duplex
+    transcript in the loop (observe root, grind 0 bits, sample beta), as
+    prove_fri runs it."""
     consts = p2_constants("bb", 13)
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(p3hip.BABYBEAR, consts, 8, 13)
@@ -351,6 +358,12 @@ def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3
     m = ctx.upload(cw)
 
     def step():
+        if challenger:
+            ch = ctx.challenger()
+            ch.observe([1, 2, 3])
+            r = ctx.fri_commit_phase_chal(m, log_blowup, lfpl, 1, ch, 0)
+            ch.free()
+            return r["roots"], r["final"]
         roots, final, _ = ctx.fri_commit_phase(m, log_blowup, lfpl, 1, betas)
         return roots, final
 
@@ -375,7 +388,9 @@ def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (seeded codeword, challenges and Poseidon2 constants)",
         "config": {"workload": f"FRI commit phase, BabyBear^4 codeword 2^{log_len}, arity 2, log_blowup {log_blowup}, "
-                               f"final poly 2^{lfpl}, {rounds} layer commits", "layer_elems": layer_elems},
+                               f"final poly 2^{lfpl}, {rounds} layer commits, "
+                               + ("challenges from the duplex transcript" if challenger else "seeded challenges"),
+                   "layer_elems": layer_elems},
         "phases_ms": {"fold": fold_ms, "leaf_hash": phases["leaf_hash"][1] / args.steps,
                       "compress": phases["compress"][1] / args.steps},
         "roofline_fold": {"bound": "hbm", "achieved": fold_bytes / (fold_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
@@ -397,6 +412,56 @@ def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3
         se = sum(1 << l for l in range(lfpl + log_blowup + 1, sl + 1))
         out["cpu_baseline"] = {"value": se / cdt, "unit": "ext field elems/s", "cores": 1, "kind": kind,
                                "sample": f"codeword 2^{sl}, same parameters, {cdt:.2f} s on 1 host thread"}
+    print(json.dumps(out))
+    ctx.close()
+
+
+def bench_grind(p3hip, args, bits: int = 24, cpu_bits: int = 13):
+    """Proof-of-work search (DuplexChallenger::grind): from a seeded transcript,
+    the least witness whose following sample has `bits` zero low bits; value =
+    witnesses tried per second (the witness found + 1, over the wall time of
+    the grind call)."""
+    consts = p2_constants("bb", 13)
+    ctx = p3hip.Context(0)
+    ctx.set_poseidon2(p3hip.BABYBEAR, consts, 8, 13)
+    prefix = np.arange(1, 12, dtype=np.uint32) * 7919
+    runs = []
+    for rep in range(args.warmup + args.steps):
+        ch = ctx.challenger()
+        ch.observe(prefix)
+        ch.observe([rep])
+        t0 = time.perf_counter()
+        w = ch.grind(bits)
+        dt = time.perf_counter() - t0
+        ch.free()
+        if rep >= args.warmup:
+            runs.append((w, dt))
+    tries = sum(w + 1 for w, _ in runs)
+    total = sum(dt for _, dt in runs)
+    out = {
+        "metric": "proof-of-work witnesses tried/s (DuplexChallenger grind)", "value": tries / total,
+        "unit": "witnesses/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": total * 1e3 / len(runs), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic (seeded transcript prefixes)",
+        "config": {"workload": f"grind {bits} bits, BabyBear Poseidon2-16 duplex transcript (rate 8)",
+                   "witnesses": [int(w) for w, _ in runs]},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import native  # test infrastructure; used here only as the CPU baseline
+
+        lib, kind = native.reference(), "reference"
+        if lib is None:
+            lib, kind = native.oracle(), "port"
+        ctries, ctime = 0, 0.0
+        for rep in range(8):
+            ops = [(native.OBSERVE, int(v)) for v in prefix] + [(native.OBSERVE, rep), (native.GRIND, cpu_bits)]
+            t0 = time.perf_counter()
+            (w,) = lib.challenger_run(consts, 8, 13, ops)
+            ctime += time.perf_counter() - t0
+            ctries += int(w) + 1
+        out["cpu_baseline"] = {"value": ctries / ctime, "unit": "witnesses/s", "cores": 1, "kind": kind,
+                               "sample": f"8 grinds of {cpu_bits} bits, {ctries} witnesses in {ctime:.2f} s"}
     print(json.dumps(out))
     ctx.close()
 

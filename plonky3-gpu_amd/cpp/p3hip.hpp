@@ -7,6 +7,7 @@
 //   HipMerkleTreeMmcs<F, D>   ~ p3_merkle_tree::MerkleTreeMmcs (merkle_tree_mmcs.hpp:75-254)
 //   HipFlatMmcs<F, D>         ~ p3_merkle::MerkleTreeMmcs      (fri_merkle_tree_mmcs.hpp:29-168)
 //   hip_fold_matrix<EF>       ~ fold_matrix_cuda                (fri_fold_cuda.hpp:306-348)
+//   HipDuplexChallenger<F>    ~ p3_challenger::DuplexChallenger (duplex_challenger.hpp:42-268)
 //
 // Matrix types: any type with `values` (std::vector<F>), `width()` and
 // `height()` and a (std::vector<F>, size_t width) constructor -- the shape of
@@ -327,5 +328,86 @@ std::vector<EF> hip_fold_matrix(const Context& ctx, const EF& beta, size_t log_a
         for (int k = 0; k < 4; ++k) res[i].coeffs[k] = vals[4 * i + k];
     return res;
 }
+
+// ------------------------------------------------------------ challenger ---
+// DuplexChallenger<F, Perm, 16, 8> (duplex_challenger.hpp:42-268) for BabyBear
+// with the context's Poseidon2 constants, its state on the device.  The method
+// names are the reference's, so FriChallenger-style wrappers
+// (tests/fri_test.cpp:99-124) compile unchanged.
+template <class F>
+class HipDuplexChallenger {
+   public:
+    explicit HipDuplexChallenger(Context ctx) : ctx_(std::move(ctx)) {
+        p3h_chal* h = nullptr;
+        check(ctx_.get(), p3h_challenger_create(ctx_.get(), &h));
+        h_.reset(h, [](p3h_chal* p) { p3h_challenger_free(p); });
+    }
+    HipDuplexChallenger(const HipDuplexChallenger& o) : ctx_(o.ctx_) {  // deep copy, as the reference's
+        p3h_chal* h = nullptr;
+        check(ctx_.get(), p3h_challenger_clone(o.h_.get(), &h));
+        h_.reset(h, [](p3h_chal* p) { p3h_challenger_free(p); });
+    }
+    HipDuplexChallenger& operator=(const HipDuplexChallenger& o) {
+        if (this != &o) *this = HipDuplexChallenger(o);
+        return *this;
+    }
+    HipDuplexChallenger(HipDuplexChallenger&&) = default;
+    HipDuplexChallenger& operator=(HipDuplexChallenger&&) = default;
+
+    void observe(const F& v) {
+        uint32_t r = FieldRaw<F, uint32_t>::to(v);
+        check(ctx_.get(), p3h_challenger_observe(h_.get(), &r, 1));
+    }
+    void observe_slice(const std::vector<F>& vs) {
+        std::vector<uint32_t> r;
+        for (const F& v : vs) r.push_back(FieldRaw<F, uint32_t>::to(v));
+        check(ctx_.get(), p3h_challenger_observe(h_.get(), r.data(), uint32_t(r.size())));
+    }
+    template <class Hash>
+    void observe_merkle_cap(const std::vector<Hash>& cap) {
+        std::vector<uint32_t> r;
+        for (const Hash& h : cap)
+            for (const F& v : h) r.push_back(FieldRaw<F, uint32_t>::to(v));
+        check(ctx_.get(), p3h_challenger_observe(h_.get(), r.data(), uint32_t(r.size())));
+    }
+    template <class EF>
+    void observe_algebra_element(const EF& e) {
+        std::vector<F> v(e.coeffs.begin(), e.coeffs.end());
+        observe_slice(v);
+    }
+    F sample() {
+        uint32_t r = 0;
+        check(ctx_.get(), p3h_challenger_sample(h_.get(), 1, &r));
+        return FieldRaw<F, uint32_t>::from(r);
+    }
+    size_t sample_bits(size_t bits) {
+        uint64_t r = 0;
+        check(ctx_.get(), p3h_challenger_sample_bits(h_.get(), uint32_t(bits), &r));
+        return size_t(r);
+    }
+    template <class EF>
+    EF sample_algebra_element() {
+        uint32_t r[4];
+        check(ctx_.get(), p3h_challenger_sample(h_.get(), 4, r));
+        EF e{};
+        for (int i = 0; i < 4; ++i) e.coeffs[i] = FieldRaw<F, uint32_t>::from(r[i]);
+        return e;
+    }
+    uint64_t grind(size_t bits) {
+        uint64_t w = 0;
+        check(ctx_.get(), p3h_challenger_grind(h_.get(), uint32_t(bits), &w));
+        return w;
+    }
+    bool check_witness(size_t bits, uint64_t w) {
+        int ok = 0;
+        check(ctx_.get(), p3h_challenger_check_witness(h_.get(), uint32_t(bits), w, &ok));
+        return ok != 0;
+    }
+    p3h_chal* handle() const { return h_.get(); }
+
+   private:
+    Context ctx_;
+    std::shared_ptr<p3h_chal> h_;
+};
 
 }  // namespace p3hip

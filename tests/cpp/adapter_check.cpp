@@ -77,6 +77,15 @@ int main() {
     std::printf("fold0 %u %u %u %u\n", folded[0].coeffs[0].v, folded[0].coeffs[1].v, folded[0].coeffs[2].v,
                 folded[0].coeffs[3].v);
 
+    p3hip::HipDuplexChallenger<BB> chal(ctx);
+    chal.observe_slice(fill(9, 11));
+    chal.observe_merkle_cap(root);
+    BB4 z = chal.sample_algebra_element<BB4>();
+    auto copy = chal;  // deep copy: both continue from the same transcript
+    std::printf("chal_ext %u %u %u %u\n", z.coeffs[0].v, z.coeffs[1].v, z.coeffs[2].v, z.coeffs[3].v);
+    uint64_t w = chal.grind(6);
+    std::printf("chal_grind %llu %u %u\n", (unsigned long long)w, chal.sample().v, copy.sample().v);
+
     try {
         dft.coset_lde_batch(RowMajorMatrix<BB>(fill(8, 3 * 4), 4), 1, BB(31));
         std::printf("error_not_raised\n");

@@ -44,4 +44,13 @@ def test_cpp_adapters_match_oracle(orc):
     cw = native.seeded(7, 64 * 4).reshape(64, 4)
     f = orc.fold(6, 1, np.array([3, 1, 4, 1], dtype=np.uint32), cw)
     assert [int(x) for x in lines["fold0"]] == [int(x) for x in f[0]]
+    ops = [(native.OBSERVE, int(v)) for v in native.seeded(9, 11)]
+    ops += [(native.OBSERVE, int(v)) for v in orc.flat_commit("bb", consts, 8, 13, lde)]
+    ops += [(native.SAMPLE_EXT, 0), (native.GRIND, 6), (native.SAMPLE, 0)]
+    want = orc.challenger_run(consts, 8, 13, ops)
+    assert [int(x) for x in lines["chal_ext"]] == [int(x) for x in want[:4]]
+    assert [int(x) for x in lines["chal_grind"][:2]] == [int(want[4]), int(want[5])]
+    # the copy continues from the transcript as it was before grinding
+    fresh = orc.challenger_run(consts, 8, 13, ops[:-2] + [(native.SAMPLE, 0)])
+    assert int(lines["chal_grind"][2]) == int(fresh[4])
     assert "invalid_argument_raised" in lines
