@@ -11,6 +11,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <map>
 #include <array>
 #include <cstring>
 #include <numeric>
@@ -1040,6 +1041,188 @@ int orc_bb4_fri_commit_chal(const uint32_t* c, int rf, int rp, const uint32_t* c
                 ch.observe(fp[k][q]);
             }
         for (size_t la : arities) ch.observe(uint32_t(la));
+        for (int k = 0; k < 8; ++k) post_out[k] = ch.sample();
+        *n_rounds = round;
+        return 0;
+    });
+}
+
+// TwoAdicFriPcs::commit then ::open (two_adic_fri_pcs.hpp:207-455) with the
+// FriChallenger transcript of the reference's tests (tests/fri_test.cpp:99-124),
+// one batch of n matrices: the transcript observes `pre` and the commitment
+// root, then open samples alpha (two_adic_fri_pcs.hpp:277), evaluates every
+// matrix at its points by barycentric interpolation on its LDE coset
+// (interpolation.hpp:43-185), observes the opened values, builds one quotient
+// vector per LDE height (:335-395; bit-reversed, descending heights) and runs
+// prove_fri (fri_prover.hpp:47-280) over them: commit rounds with inputs mixed
+// in by beta^arity, final polynomial, query grinding, query indices
+// (sample_bits of the largest log height) and the input rows each query opens
+// (PcsQueryInputMmcs::open, two_adic_fri_pcs.hpp:103-121).
+// Outputs: root (8), opened [matrix][point][col] x 4, roots 8/round, commit
+// witnesses 1/round, final poly 4 x 2^lfpl, query witness, query indices,
+// query rows (per query: each height group's concatenated row, descending),
+// post 8 samples after the proof.
+int orc_bb_pcs_open(const uint32_t* c, int rf, int rp, size_t n, const size_t* log_hs, const size_t* ws,
+                    const uint32_t* shifts, const uint32_t* const* mats, size_t log_blowup, const size_t* n_points,
+                    const uint32_t* points, const uint32_t* pre, size_t n_pre, size_t lfpl, size_t mla,
+                    size_t num_queries, size_t commit_pow_bits, size_t query_pow_bits, uint32_t* root_out,
+                    uint32_t* opened_out, uint32_t* roots_out, uint64_t* cwit_out, uint32_t* final_out,
+                    uint64_t* qwit_out, uint64_t* qidx_out, uint32_t* qrows_out, uint32_t* post_out,
+                    size_t* n_rounds) {
+    return guarded([&] {
+        Perm<BB, 16> perm(c, rf, rp);
+        Challenger ch{&perm, {}, {}, {}};
+        for (size_t i = 0; i < n_pre; ++i) ch.observe(pre[i]);
+        // commit (the flat MMCS over the cyclically repeated, width-padded LDE rows)
+        std::vector<Mat<BB>> ldes;
+        size_t maxh = 0, tw = 0;
+        for (size_t m = 0; m < n; ++m) {
+            ldes.push_back(lde<BB>(mats[m], log_hs[m], ws[m], log_blowup, shifts[m] % BB::P));
+            maxh = std::max(maxh, ldes.back().h);
+            tw += ws[m];
+        }
+        size_t pw = 1;
+        while (pw < tw) pw <<= 1;
+        std::vector<uint32_t> flat;
+        for (size_t r = 0; r < maxh; ++r) {
+            for (auto& l : ldes) flat.insert(flat.end(), l.row(r % l.h), l.row(r % l.h) + l.w);
+            flat.insert(flat.end(), pw - tw, 0u);
+        }
+        auto t = flat_tree<BB>(perm, flat.data(), maxh, pw);
+        for (int k = 0; k < 8; ++k) {
+            root_out[k] = t.back()[0][k];
+            ch.observe(t.back()[0][k]);  // observe_commitment
+        }
+        // open
+        E4 alpha;
+        for (int q = 0; q < 4; ++q) alpha[q] = ch.sample();
+        std::vector<std::vector<std::vector<E4>>> opened(n);
+        size_t pt = 0;
+        for (size_t m = 0; m < n; ++m) {
+            const Mat<BB>& L = ldes[m];
+            const size_t N = L.h, lN = log2_strict(N);
+            const uint32_t g = shifts[m] % BB::P, om = BB::root(lN);
+            // diff_invs[i] = h_i / (N g^(N-1)) (interpolation.hpp:81-99)
+            const uint32_t common = invf<BB>(BB::mul(uint32_t(N % BB::P), powf<BB>(g, N - 1)));
+            for (size_t pi = 0; pi < n_points[m]; ++pi, ++pt) {
+                const E4 z{points[4 * pt] % BB::P, points[4 * pt + 1] % BB::P, points[4 * pt + 2] % BB::P,
+                           points[4 * pt + 3] % BB::P};
+                std::vector<E4> acc(L.w, E4{0, 0, 0, 0});
+                uint32_t h = 1;
+                for (size_t i = 0; i < N; ++i, h = BB::mul(h, om)) {
+                    const uint32_t x = BB::mul(g, h);
+                    const E4 wgt = e4_scale(e4_inv(e4_sub(z, e4_base(x))), BB::mul(h, common));
+                    for (size_t col = 0; col < L.w; ++col) acc[col] = e4_add(acc[col], e4_scale(wgt, L.v[i * L.w + col]));
+                }
+                const E4 van = e4_sub(e4_pow(z, N), e4_base(powf<BB>(g, N)));
+                std::vector<E4> vals(L.w);
+                for (size_t col = 0; col < L.w; ++col) vals[col] = e4_mul(van, acc[col]);
+                opened[m].push_back(vals);
+            }
+        }
+        size_t o = 0;
+        for (auto& mv : opened)
+            for (auto& pv : mv)
+                for (auto& v : pv)
+                    for (int q = 0; q < 4; ++q) {
+                        opened_out[o++] = v[q];
+                        ch.observe(v[q]);  // observe_challenge
+                    }
+        // quotients, one vector per LDE height, descending
+        std::map<size_t, std::vector<E4>, std::greater<size_t>> groups;
+        std::map<size_t, std::vector<size_t>, std::greater<size_t>> group_mats;
+        size_t apow = 0;
+        pt = 0;
+        for (size_t m = 0; m < n; ++m) {
+            const Mat<BB>& L = ldes[m];
+            const size_t N = L.h, lN = log2_strict(N);
+            auto& qv = groups[lN];
+            if (qv.empty()) qv.assign(N, E4{0, 0, 0, 0});
+            group_mats[lN].push_back(m);
+            const uint32_t g = shifts[m] % BB::P, om = BB::root(lN);
+            std::vector<E4> ap(n_points[m] * L.w);
+            E4 cur = e4_pow(alpha, apow);
+            for (auto& a : ap) {
+                a = cur;
+                cur = e4_mul(cur, alpha);
+            }
+            apow += ap.size();
+            for (size_t k = 0; k < N; ++k) {
+                const E4 x = e4_base(BB::mul(g, powf<BB>(om, k)));
+                for (size_t pi = 0; pi < n_points[m]; ++pi) {
+                    const size_t pp = pt + pi;
+                    const E4 z{points[4 * pp] % BB::P, points[4 * pp + 1] % BB::P, points[4 * pp + 2] % BB::P,
+                               points[4 * pp + 3] % BB::P};
+                    const E4 dinv = e4_inv(e4_sub(z, x));
+                    for (size_t col = 0; col < L.w; ++col) {
+                        const E4 num = e4_sub(opened[m][pi][col], e4_base(L.v[k * L.w + col]));
+                        qv[k] = e4_add(qv[k], e4_mul(e4_mul(ap[pi * L.w + col], num), dinv));
+                    }
+                }
+            }
+            pt += n_points[m];
+        }
+        std::vector<std::vector<E4>> inputs;
+        for (auto& [lh, qv] : groups) {
+            std::vector<E4> br(qv.size());
+            for (size_t i = 0; i < qv.size(); ++i) br[i] = qv[rev_bits(i, lh)];
+            inputs.push_back(br);
+        }
+        // prove_fri
+        const size_t lfh = log_blowup + lfpl;
+        if (log2_strict(inputs.back().size()) <= lfh) throw std::invalid_argument("input too small for FRI");
+        std::vector<E4> cur = inputs[0];
+        size_t next_in = 1, round = 0;
+        std::vector<size_t> arities;
+        std::vector<std::vector<E4>> layers;
+        while (cur.size() > (size_t(1) << lfh)) {
+            const size_t lh = log2_strict(cur.size());
+            const bool has_next = next_in < inputs.size();
+            const size_t target = has_next ? log2_strict(inputs[next_in].size()) : lfh;
+            if (lh <= target) break;
+            const size_t la = std::min(lh - target, mla);
+            const size_t arity = size_t(1) << la, rows = cur.size() / arity;
+            std::vector<uint32_t> fl(cur.size() * 4);
+            for (size_t i = 0; i < cur.size(); ++i)
+                for (int q = 0; q < 4; ++q) fl[4 * i + q] = cur[i][q];
+            auto tr = flat_tree<BB>(perm, fl.data(), rows, 4 * arity);
+            for (int k = 0; k < 8; ++k) {
+                roots_out[8 * round + k] = tr.back()[0][k];
+                ch.observe(tr.back()[0][k]);
+            }
+            cwit_out[round] = ch.grind(commit_pow_bits);
+            E4 beta;
+            for (int q = 0; q < 4; ++q) beta[q] = ch.sample();
+            cur = fold(lh, la, beta, cur);
+            if (has_next && inputs[next_in].size() == cur.size()) {
+                E4 bp = beta;
+                for (size_t s2 = 0; s2 < la; ++s2) bp = e4_mul(bp, bp);
+                for (size_t k = 0; k < cur.size(); ++k) cur[k] = e4_add(cur[k], e4_mul(bp, inputs[next_in][k]));
+                ++next_in;
+            }
+            arities.push_back(la);
+            ++round;
+        }
+        auto fp = final_poly(cur, lfpl);
+        for (size_t k = 0; k < fp.size(); ++k)
+            for (int q = 0; q < 4; ++q) {
+                final_out[4 * k + q] = fp[k][q];
+                ch.observe(fp[k][q]);
+            }
+        for (size_t la : arities) ch.observe(uint32_t(la));
+        *qwit_out = ch.grind(query_pow_bits);
+        size_t log_max = lfh;
+        for (size_t la : arities) log_max += la;
+        size_t qr = 0;
+        for (size_t q = 0; q < num_queries; ++q) {
+            const uint64_t idx = ch.sample_bits(log_max);
+            qidx_out[q] = idx;
+            for (auto& [lh, ms] : group_mats) {
+                const size_t k = rev_bits(idx >> (log_max - lh), lh);
+                for (size_t m : ms)
+                    for (size_t col = 0; col < ldes[m].w; ++col) qrows_out[qr++] = ldes[m].v[k * ldes[m].w + col];
+            }
+        }
         for (int k = 0; k < 8; ++k) post_out[k] = ch.sample();
         *n_rounds = round;
         return 0;

@@ -101,6 +101,13 @@ int orc_bb4_fri_commit_chal(const uint32_t* consts, int rf, int rp, const uint32
                             size_t log_blowup, size_t log_final_poly_len, size_t max_log_arity, size_t pow_bits,
                             const uint32_t* pre, size_t n_pre, uint32_t* roots_out, uint32_t* betas_out,
                             uint64_t* witnesses_out, uint32_t* final_out, uint32_t* post_out, size_t* n_rounds);
+int orc_bb_pcs_open(const uint32_t* consts, int rf, int rp, size_t n, const size_t* log_hs, const size_t* ws,
+                    const uint32_t* shifts, const uint32_t* const* mats, size_t log_blowup, const size_t* n_points,
+                    const uint32_t* points, const uint32_t* pre, size_t n_pre, size_t lfpl, size_t mla,
+                    size_t num_queries, size_t commit_pow_bits, size_t query_pow_bits, uint32_t* root_out,
+                    uint32_t* opened_out, uint32_t* roots_out, uint64_t* cwit_out, uint32_t* final_out,
+                    uint64_t* qwit_out, uint64_t* qidx_out, uint32_t* qrows_out, uint32_t* post_out,
+                    size_t* n_rounds);
 int orc_bb_pcs_commit(const uint32_t* consts, int rf, int rp, size_t n_mats, const size_t* log_heights,
                       const size_t* widths, const uint32_t* shifts, const uint32_t* const* mats,
                       size_t log_blowup, uint32_t* root8);

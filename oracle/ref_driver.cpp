@@ -409,7 +409,26 @@ int ref_bb4_fri_commit(const uint32_t* c, int rf, int rp, const uint32_t* codewo
 }
 
 
+}  // extern "C"
+
 using RefChallenger = p3_challenger::DuplexChallenger<BB, RefPerm<BB, 16>, 16, 8>;
+
+// Transcript adapter with the methods prove_fri and TwoAdicFriPcs::open call,
+// of the same shape as the FriChallenger in src/tests/fri_test.cpp:99-124.
+struct PcsTranscript {
+    RefChallenger inner;
+    template <class Commit>
+    void observe_commitment(const Commit& cm) { inner.observe_merkle_cap(cm); }
+    template <class EF = BB4>
+    EF sample_challenge() { return inner.template sample_algebra_element<EF>(); }
+    template <class EF>
+    void observe_challenge(const EF& v) { inner.template observe_algebra_element<EF>(v); }
+    void observe_arity(size_t la) { inner.observe(BB(static_cast<uint32_t>(la))); }
+    uint64_t grind(size_t bits) { return inner.grind(bits); }
+    size_t sample_bits(size_t bits) { return inner.sample_bits(bits); }
+};
+
+extern "C" {
 
 // The reference's DuplexChallenger driven by a script: op = code | arg << 8;
 // 0 observe(arg), 1 sample, 2 sample_bits(arg), 3 grind(arg),
@@ -526,6 +545,68 @@ int ref_bb_pcs_commit(const uint32_t* c, int rf, int rp, size_t n, const size_t*
         }
         auto [commit, pd] = pcs.commit(std::move(evals));
         for (int i = 0; i < 8; ++i) root[i] = commit[0][i].value();
+    });
+}
+
+// TwoAdicFriPcs::commit + ::open (src/include/two_adic_fri_pcs.hpp:207-455)
+// as src/tests/fri_test.cpp instantiates it, with a transcript adapter of the
+// same shape as that test's FriChallenger; outputs as orc_bb_pcs_open.
+int ref_bb_pcs_open(const uint32_t* c, int rf, int rp, size_t n, const size_t* log_hs, const size_t* ws,
+                    const uint32_t* shifts, const uint32_t* const* mats, size_t log_blowup, const size_t* n_points,
+                    const uint32_t* points, const uint32_t* pre, size_t n_pre, size_t lfpl, size_t mla,
+                    size_t num_queries, size_t commit_pow_bits, size_t query_pow_bits, uint32_t* root_out,
+                    uint32_t* opened_out, uint32_t* roots_out, uint64_t* cwit_out, uint32_t* final_out,
+                    uint64_t* qwit_out, uint64_t* qidx_out, uint32_t* qrows_out, uint32_t* post_out,
+                    size_t* n_rounds) {
+    return guard([&] {
+        using S = Stack<BB>;
+        using EFMmcs = p3_merkle::ExtensionMmcs<BB, BB4, typename S::FlatMmcs>;
+        using Pcs = p3_fri::TwoAdicFriPcs<BB, BB4, p3_dft::Radix2Dit<BB>, typename S::FlatMmcs, EFMmcs>;
+        auto perm = make_perm<BB, uint32_t, 16>(c, rf, rp);
+        typename S::FlatMmcs inner{typename S::Hasher(perm), typename S::Compress(perm)};
+        EFMmcs ef_mmcs(inner);
+        p3_fri::FriParameters<EFMmcs> params{log_blowup, lfpl, mla, num_queries, commit_pow_bits, query_pow_bits,
+                                             ef_mmcs};
+        Pcs pcs(p3_dft::Radix2Dit<BB>{}, inner, params);
+        std::vector<std::pair<typename Pcs::Domain, p3_matrix::RowMajorMatrix<BB>>> evals;
+        for (size_t m = 0; m < n; ++m) {
+            typename Pcs::Domain d;
+            d.log_n = log_hs[m];
+            d.shift = BB(shifts[m]);
+            evals.emplace_back(d, load_mat<BB>(mats[m], size_t(1) << log_hs[m], ws[m]));
+        }
+        auto [commit, pd] = pcs.commit(std::move(evals));
+        for (int i = 0; i < 8; ++i) root_out[i] = commit[0][i].value();
+        PcsTranscript tr{RefChallenger(perm)};
+        for (size_t i = 0; i < n_pre; ++i) tr.inner.observe(BB(pre[i]));
+        tr.observe_commitment(commit);
+        std::vector<std::vector<BB4>> pts(n);
+        size_t k = 0;
+        for (size_t m = 0; m < n; ++m)
+            for (size_t i = 0; i < n_points[m]; ++i, ++k)
+                pts[m].push_back(BB4(std::array<BB, 4>{BB(points[4 * k]), BB(points[4 * k + 1]), BB(points[4 * k + 2]),
+                                                       BB(points[4 * k + 3])}));
+        std::vector<std::pair<const typename Pcs::PcsProverData*, std::vector<std::vector<BB4>>>> od;
+        od.emplace_back(&pd, pts);
+        auto [opened, proof] = pcs.open(std::move(od), tr);
+        size_t o = 0;
+        for (auto& mv : opened[0])
+            for (auto& pv : mv)
+                for (auto& v : pv)
+                    for (int q = 0; q < 4; ++q) opened_out[o++] = v.coeffs[q].value();
+        for (size_t r = 0; r < proof.commit_phase_commits.size(); ++r)
+            for (int i = 0; i < 8; ++i) roots_out[8 * r + i] = proof.commit_phase_commits[r][0][i].value();
+        for (size_t r = 0; r < proof.commit_pow_witnesses.size(); ++r) cwit_out[r] = proof.commit_pow_witnesses[r];
+        for (size_t i = 0; i < proof.final_poly.size(); ++i)
+            for (int q = 0; q < 4; ++q) final_out[4 * i + q] = proof.final_poly[i].coeffs[q].value();
+        *qwit_out = proof.query_pow_witness;
+        size_t qr = 0;
+        for (size_t q = 0; q < proof.query_proofs.size(); ++q) {
+            qidx_out[q] = proof.query_proofs[q].input_proof.row_index;
+            for (const BB& v : proof.query_proofs[q].input_proof.row_values) qrows_out[qr++] = v.value();
+        }
+        for (int i = 0; i < 8; ++i) post_out[i] = tr.inner.sample().value();
+        *n_rounds = proof.commit_phase_commits.size();
     });
 }
 

@@ -187,6 +187,23 @@ def main():
         for k, v in r.items():
             g[f"fchal_{case}_{k}"] = v
 
+    # TwoAdicFriPcs::commit + open with the transcript (opened values, FRI proof parts)
+    for case, (shapes, shifts, lb, npts, lfpl, mla, nq, cpow, qpow) in enumerate((
+        ([(16, 3), (32, 4), (16, 1)], [31, 1, 31], 1, [1, 2, 1], 2, 1, 4, 0, 0),
+        ([(64, 5)], [31], 1, [2], 1, 2, 6, 3, 4),
+        ([(32, 2), (8, 3), (64, 1)], [7, 31, 1], 2, [1, 1, 3], 0, 3, 3, 2, 1),
+    )):
+        mats = [seeded(800 + 10 * case + j, h * w, "bb").reshape(h, w) for j, (h, w) in enumerate(shapes)]
+        pts = [seeded(850 + 10 * case + j, 4 * k, "bb").reshape(k, 4) for j, k in enumerate(npts)]
+        pre = seeded(890 + case, 3, "bb")
+        r = ref.pcs_open(cb, 8, 13, mats, shifts, lb, pts, pre, lfpl, mla, nq, cpow, qpow)
+        for j, mat in enumerate(mats):
+            g[f"popen_{case}_m{j}"], g[f"popen_{case}_p{j}"] = mat, pts[j]
+        g[f"popen_{case}_shifts"], g[f"popen_{case}_pre"] = np.array(shifts, dtype=U32), pre
+        g[f"popen_{case}_params"] = np.array([lb, lfpl, mla, nq, cpow, qpow], dtype=U64)
+        for k, v in r.items():
+            g[f"popen_{case}_{k}"] = v
+
     out = os.path.join(HERE, "golden.npz")
     np.savez_compressed(out, **g)
     print(f"wrote {out}: {len(g)} arrays, {os.path.getsize(out)} bytes")

@@ -236,6 +236,45 @@ class Native:
         return {"roots": roots[: 8 * r].reshape(-1, 8), "betas": betas[: 4 * r].reshape(-1, 4), "witnesses": wit[:r],
                 "final": final.reshape(-1, 4), "post": post}
 
+    def pcs_open(self, consts, rf, rp, mats, shifts, log_blowup, points, pre, lfpl, mla, num_queries,
+                 commit_pow_bits, query_pow_bits):
+        """TwoAdicFriPcs commit + open (one batch); points: list per matrix of
+        BabyBear^4 values (4 words each).  Returns a dict of the proof parts."""
+        ms = [np.ascontiguousarray(m, dtype=U32) for m in mats]
+        n = len(ms)
+        lhs = (C.c_size_t * n)(*[m.shape[0].bit_length() - 1 for m in ms])
+        ws = (C.c_size_t * n)(*[m.shape[1] for m in ms])
+        npts = (C.c_size_t * n)(*[len(p) for p in points])
+        ptrs = (C.c_void_p * n)(*[m.ctypes.data for m in ms])
+        sh = np.ascontiguousarray(shifts, dtype=U32)
+        c = np.ascontiguousarray(consts, dtype=U32)
+        flat_pts = np.ascontiguousarray(np.concatenate([np.array(p, dtype=U32).reshape(-1) for p in points]
+                                                       + [np.zeros(4, dtype=U32)]))
+        pr = np.ascontiguousarray(np.array(pre, dtype=U32).reshape(-1)) if len(pre) else np.zeros(1, dtype=U32)
+        n_open = sum(len(p) * m.shape[1] for p, m in zip(points, ms))
+        root = np.zeros(8, dtype=U32)
+        opened = np.zeros(max(1, 4 * n_open), dtype=U32)
+        roots = np.zeros(64 * 8, dtype=U32)
+        cwit = np.zeros(64, dtype=U64)
+        final = np.zeros(4 << lfpl, dtype=U32)
+        qwit = np.zeros(1, dtype=U64)
+        qidx = np.zeros(max(1, num_queries), dtype=U64)
+        tw = sum(m.shape[1] for m in ms)
+        qrows = np.zeros(max(1, num_queries * tw), dtype=U32)
+        post = np.zeros(8, dtype=U32)
+        nr = C.c_size_t(0)
+        self._rc(self.fn("bb_pcs_open")(_p(c), C.c_int(rf), C.c_int(rp), C.c_size_t(n), lhs, ws, _p(sh), ptrs,
+                                        C.c_size_t(log_blowup), npts, _p(flat_pts), _p(pr), C.c_size_t(len(pre)),
+                                        C.c_size_t(lfpl), C.c_size_t(mla), C.c_size_t(num_queries),
+                                        C.c_size_t(commit_pow_bits), C.c_size_t(query_pow_bits), _p(root),
+                                        _p(opened), _p(roots), _p(cwit), _p(final), _p(qwit), _p(qidx), _p(qrows),
+                                        _p(post), C.byref(nr)), "pcs_open")
+        r = nr.value
+        return {"root": root, "opened": opened[: 4 * n_open].reshape(-1, 4), "roots": roots[: 8 * r].reshape(-1, 8),
+                "commit_witnesses": cwit[:r], "final": final.reshape(-1, 4), "query_witness": qwit,
+                "query_indices": qidx[:num_queries], "query_rows": qrows[: num_queries * tw].reshape(num_queries, tw),
+                "post": post}
+
     def pcs_commit(self, consts, rf, rp, mats, shifts, log_blowup):
         ms = [np.ascontiguousarray(m, dtype=U32) for m in mats]
         n = len(ms)

@@ -21,6 +21,10 @@ What pins each oracle function (a plausible mistake in it fails at least one):
 |                           | random mixes); grind = the least passing witness (brute force)            |
 | FRI commit + challenger   | golden roots, betas, witnesses, final poly, post-loop samples; the betas  |
 |                           | reproduce the roots through the supplied-beta commit phase                |
+| PCS open                  | golden opened values and FRI proof parts from the reference's             |
+|                           | TwoAdicFriPcs::open; opened values = direct evaluation sum_j a_j z^j of   |
+|                           | the interpolant (a = idft of the trace column), independent of the        |
+|                           | barycentric formula                                                       |
 
 When the reference build (oracle/_ref) is present, random cross-checks against
 it run as well.
@@ -288,6 +292,46 @@ def test_fri_commit_with_challenger_golden(orc, golden):
         # the sampled betas drive the supplied-beta commit phase to the same layers
         roots, final = orc.fri_commit(cb, RF, 13, golden[f"fchal_{case}_cw"], lb, lfpl, mla, r["betas"])
         assert (roots == r["roots"]).all() and (final == r["final"]).all(), case
+
+
+def _popen_case(golden, case):
+    mats = _mats(golden, f"popen_{case}")
+    pts, j = [], 0
+    while f"popen_{case}_p{j}" in golden:
+        pts.append(golden[f"popen_{case}_p{j}"])
+        j += 1
+    lb, lfpl, mla, nq, cpow, qpow = (int(v) for v in golden[f"popen_{case}_params"])
+    return mats, pts, lb, lfpl, mla, nq, cpow, qpow
+
+
+def _e4_mul(a, b):  # BabyBear^4 = F_p[X]/(X^4 - 11), pure Python integers
+    r = [0] * 7
+    for i in range(4):
+        for j in range(4):
+            r[i + j] += int(a[i]) * int(b[j])
+    return [(r[i] + 11 * (r[i + 4] if i + 4 < 7 else 0)) % BB_P for i in range(4)]
+
+
+def test_pcs_open_golden_and_direct_evaluation(orc, golden):
+    cb = golden["consts_bb"]
+    for case in range(3):
+        mats, pts, lb, lfpl, mla, nq, cpow, qpow = _popen_case(golden, case)
+        r = orc.pcs_open(cb, RF, 13, mats, golden[f"popen_{case}_shifts"], lb, pts, golden[f"popen_{case}_pre"],
+                         lfpl, mla, nq, cpow, qpow)
+        for k, v in r.items():
+            assert v.shape == golden[f"popen_{case}_{k}"].shape and (v == golden[f"popen_{case}_{k}"]).all(), (case, k)
+        # opened value = P(z) evaluated from the coefficients of the trace's interpolant
+        o = 0
+        for mat, pp in zip(mats, pts):
+            coeffs = orc.dft("bb", 1, mat)
+            for z in pp:
+                for col in range(mat.shape[1]):
+                    acc, zp = [0, 0, 0, 0], [1, 0, 0, 0]
+                    for a in coeffs[:, col]:
+                        acc = [(x + int(a) * y) % BB_P for x, y in zip(acc, zp)]
+                        zp = _e4_mul(zp, z)
+                    assert acc == [int(x) for x in r["opened"][o]], (case, col)
+                    o += 1
 
 
 def test_pcs_commit_golden(orc, golden):
