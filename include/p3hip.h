@@ -205,6 +205,28 @@ int p3h_challenger_state(p3h_chal* ch, uint32_t* out34);
 int p3h_pcs_commit(p3h_ctx* ctx, const p3h_mat* const* mats, const uint64_t* shifts, uint32_t n_mats,
                    uint32_t log_blowup, p3h_tree** out);
 
+/* TwoAdicFriPcs::open (two_adic_fri_pcs.hpp:271-455) for one batch: `tree` is
+ * the result of p3h_pcs_commit (its LDEs), shifts the matrices' domain
+ * shifts, n_points[m] BabyBear^4 points for matrix m (4 words each,
+ * concatenated).  On the device: alpha is sampled from the transcript, every
+ * matrix is evaluated at its points by barycentric interpolation on its LDE
+ * coset (interpolation.hpp:152-185), the transcript observes the values, one
+ * quotient vector per LDE height is accumulated (two_adic_fri_pcs.hpp:335-381)
+ * and prove_fri runs over them (fri_prover.hpp:47-280): commit rounds with the
+ * smaller inputs mixed in, final polynomial, query proof of work, query
+ * indices and the input rows each query opens (PcsQueryInputMmcs::open,
+ * two_adic_fri_pcs.hpp:103-121).
+ * opened_out: [matrix][point][column] x 4 words; roots_out: 8 per round;
+ * query_rows_out: per query, each height group's rows (descending heights,
+ * matrices in commit order), total width words; keep (optional): the FRI
+ * layers, for the commit-phase openings of the queries. */
+int p3h_pcs_open(p3h_ctx* ctx, const p3h_tree* tree, const uint64_t* shifts, const uint32_t* n_points,
+                 const uint32_t* points, p3h_chal* challenger, uint32_t log_final_poly_len, uint32_t max_log_arity,
+                 uint32_t num_queries, uint32_t commit_pow_bits, uint32_t query_pow_bits, uint32_t* opened_out,
+                 uint32_t* roots_out, uint64_t* commit_witnesses_out, uint32_t* final_poly_out,
+                 uint64_t* query_witness_out, uint64_t* query_indices_out, uint32_t* query_rows_out,
+                 uint32_t* n_rounds, p3h_fri** keep);
+
 /* --------------------------------------------------------------- batched */
 /* Commit many BabyBear traces as one multi-height MMCS: every trace gets its
  * own LDE (added bits per trace, common shift) and the LDEs are committed

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <atomic>
 #include <chrono>
@@ -27,6 +28,7 @@
 
 #include "fri.h"
 #include "challenger.h"
+#include "pcs.h"
 #include "merkle.h"
 #include "ntt.h"
 
@@ -51,6 +53,37 @@ uint32_t bb_pow(uint32_t a, uint64_t e) {
 uint32_t bb_inv(uint32_t a) { return bb_pow(a, BB_P - 2); }
 uint32_t bb_mont(uint32_t a) { return uint32_t((uint64_t(a) << 32) % BB_P); }  // a * 2^32 mod p
 uint32_t bb_root(uint32_t bits) { return bb_pow(440564289u, uint64_t(1) << (27 - bits)); }
+
+// BabyBear^4 = F_p[a]/(a^4 - 11) on the host (small per-call constants only).
+using HE4 = std::array<uint32_t, 4>;
+HE4 he4_add(const HE4& a, const HE4& b) {
+    HE4 r;
+    for (int i = 0; i < 4; ++i) r[i] = uint32_t((uint64_t(a[i]) + b[i]) % BB_P);
+    return r;
+}
+HE4 he4_sub(const HE4& a, const HE4& b) {
+    HE4 r;
+    for (int i = 0; i < 4; ++i) r[i] = uint32_t((uint64_t(a[i]) + BB_P - b[i]) % BB_P);
+    return r;
+}
+HE4 he4_mul(const HE4& a, const HE4& b) {
+    uint64_t t[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) t[i + j] = (t[i + j] + uint64_t(a[i]) * b[j]) % BB_P;
+    HE4 r;
+    for (int i = 0; i < 4; ++i) r[i] = uint32_t((t[i] + 11 * (i + 4 < 7 ? t[i + 4] : 0)) % BB_P);
+    return r;
+}
+HE4 he4_pow(HE4 a, uint64_t e) {
+    HE4 r = {1, 0, 0, 0};
+    while (e) {
+        if (e & 1) r = he4_mul(r, a);
+        a = he4_mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+HE4 he4_mont(const HE4& a) { return HE4{bb_mont(a[0]), bb_mont(a[1]), bb_mont(a[2]), bb_mont(a[3])}; }
 
 uint64_t gl_mul(uint64_t a, uint64_t b) { return uint64_t((unsigned __int128)a * b % GL_P); }
 uint64_t gl_pow(uint64_t a, uint64_t e) {
@@ -1067,10 +1100,14 @@ namespace {
 // fold reads beta from device memory, so no round waits for the host unless
 // pow_bits > 0.  The transcript then observes the final polynomial's
 // coefficients and every round's log arity (fri_prover.hpp:180-189).
+// `extra`: further inputs of strictly decreasing heights below the
+// codeword's; each is added, times beta^arity, to the folded vector of its
+// height (fri_prover.hpp:123-135), and each round's arity stops at the next
+// input's height (compute_log_arity_for_round, fri_params.hpp:36-57).
 void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, uint32_t log_final_poly_len,
                    uint32_t max_log_arity, const uint32_t* betas, p3h_chal* ch, uint32_t pow_bits,
                    uint32_t* roots_out, uint32_t* betas_out, uint64_t* witnesses_out, uint32_t* final_poly_out,
-                   uint32_t* n_rounds, p3h_fri** keep) {
+                   uint32_t* n_rounds, p3h_fri** keep, const std::vector<p3h_mat*>& extra = {}) {
     if (!codeword || !roots_out || !final_poly_out) throw Err{P3H_EINVAL, "null argument"};
     if (ch && (!betas_out || !witnesses_out)) throw Err{P3H_EINVAL, "null argument"};
     if (codeword->field != P3H_BABYBEAR || codeword->width != 4 || codeword->k != 0)
@@ -1080,6 +1117,13 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
     uint32_t lh0 = log2_exact(codeword->height);
     uint32_t lfh = log_blowup + log_final_poly_len;
     if (lh0 <= lfh) throw Err{P3H_EINVAL, "codeword must be longer than the final domain"};
+    uint64_t prev_h = codeword->height;
+    for (const p3h_mat* e : extra) {
+        if (e->field != P3H_BABYBEAR || e->width != 4 || e->k != 0 || e->height >= prev_h ||
+            log2_exact(e->height) <= lfh)
+            throw Err{P3H_EINVAL, "FRI inputs must be natural (n x 4) matrices of decreasing heights above the final domain"};
+        prev_h = e->height;
+    }
     auto* f = new p3h_fri{};
     f->ctx = ctx;
     try {
@@ -1089,9 +1133,13 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
         mat_ref(cur);
         f->layers.push_back(cur);
         uint32_t round = 0;
+        size_t next_in = 0;
         while (cur->height > (1ull << lfh)) {
             uint32_t lh = log2_exact(cur->height);
-            uint32_t la = std::min(lh - lfh, max_log_arity);
+            const bool has_next = next_in < extra.size();
+            const uint32_t target = has_next ? log2_exact(extra[next_in]->height) : lfh;
+            if (lh <= target) break;
+            uint32_t la = std::min(lh - target, max_log_arity);
             uint32_t arity = 1u << la;
             p3h_mat* view = view_of(cur, cur->height >> la, 4 * arity);
             p3h_tree* t = nullptr;
@@ -1124,6 +1172,12 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
             }
             p3h_mat* nxt = do_fold(ctx, cur, la, ch ? nullptr : betas + 4 * round, beta_dev);
             f->layers.push_back(nxt);
+            if (has_next && extra[next_in]->height == nxt->height) {
+                launch_check(fri_mix_device(nxt->data<uint32_t>(), extra[next_in]->data<uint32_t>(), nxt->height,
+                                            ch ? nullptr : betas + 4 * round, beta_dev, la, ctx->stream),
+                             "fri mix");
+                ++next_in;
+            }
             cur = nxt;
             arities.push_back(la);
             ++round;
@@ -1642,6 +1696,228 @@ int p3h_challenger_state(p3h_chal* ch, uint32_t* out34) {
         std::memcpy(out34 + 24, h.output, 8 * 4);
         out34[32] = h.in_len;
         out34[33] = h.out_len;
+    });
+}
+
+// ------------------------------------------------------------- PCS open ---
+int p3h_pcs_open(p3h_ctx* ctx, const p3h_tree* tree, const uint64_t* shifts, const uint32_t* n_points,
+                 const uint32_t* points, p3h_chal* ch, uint32_t log_final_poly_len, uint32_t max_log_arity,
+                 uint32_t num_queries, uint32_t commit_pow_bits, uint32_t query_pow_bits, uint32_t* opened_out,
+                 uint32_t* roots_out, uint64_t* commit_witnesses_out, uint32_t* final_poly_out,
+                 uint64_t* query_witness_out, uint64_t* query_indices_out, uint32_t* query_rows_out,
+                 uint32_t* n_rounds, p3h_fri** keep) {
+    return api(ctx, [&] {
+        if (!tree || !shifts || !n_points || !ch || !roots_out || !commit_witnesses_out || !final_poly_out ||
+            !query_witness_out || (num_queries && (!query_indices_out || !query_rows_out)))
+            throw Err{P3H_EINVAL, "null argument"};
+        if (ch->ctx != ctx || tree->ctx != ctx) throw Err{P3H_EINVAL, "challenger or tree of another context"};
+        if (!tree->flat || tree->field != P3H_BABYBEAR || tree->mats.empty())
+            throw Err{P3H_EINVAL, "expects the tree of p3h_pcs_commit"};
+        const auto& mats = tree->mats;
+        const uint32_t nm = uint32_t(mats.size());
+        const uint32_t kb = mats[0]->k;
+        uint64_t total_open = 0, total_w = 0;
+        for (uint32_t m = 0; m < nm; ++m) {
+            if (mats[m]->field != P3H_BABYBEAR || mats[m]->k != kb)
+                throw Err{P3H_EINVAL, "PCS matrices must be BabyBear LDEs of one blowup"};
+            if (shifts[m] % BB_P == 0) throw Err{P3H_EINVAL, "coset shift must be non-zero"};
+            total_open += uint64_t(n_points[m]) * mats[m]->width;
+            total_w += mats[m]->width;
+        }
+        if (total_open && (!points || !opened_out)) throw Err{P3H_EINVAL, "null argument"};
+        auto lde_view = [&](const p3h_mat* m) {
+            const uint32_t lN = log2_exact(m->height);
+            return LdeView{m->data<uint32_t>(), lN - m->k, m->k, m->width};
+        };
+        // alpha (two_adic_fri_pcs.hpp:277)
+        ChalProgram pr{};
+        pr.op[pr.n_ops++] = chal_op(CH_SAMPLE, 4, 0, nullptr, chal_io(ch));
+        chal_run(ch, pr);
+        HE4 alpha;
+        d2h(ctx, alpha.data(), chal_io(ch), 16);
+        // opened values: barycentric interpolation on each LDE coset
+        DevPtr opened_dev = dalloc(ctx, std::max<uint64_t>(1, total_open) * 16);
+        std::vector<uint32_t> opened_host(total_open * 4);
+        {
+            Phase ph(ctx, "pcs_open_values");
+            uint64_t off = 0, pt = 0;
+            for (uint32_t m = 0; m < nm; ++m) {
+                const p3h_mat* L = mats[m];
+                const uint64_t N = L->height;
+                const uint32_t lN = log2_exact(N), w = L->width, g = uint32_t(shifts[m] % BB_P);
+                const uint32_t omega_m = bb_mont(bb_root(lN));
+                const uint32_t common_m = bb_mont(bb_inv(bb_mul(uint32_t(N % BB_P), bb_pow(g, N - 1))));
+                const uint32_t rows_per_block = uint32_t(std::max<uint64_t>(64, N / 2048));
+                const uint32_t n_blocks = uint32_t((N + rows_per_block - 1) / rows_per_block);
+                for (uint32_t p0 = 0; p0 < n_points[m]; p0 += PCS_MAX_POINTS) {
+                    const uint32_t P = std::min(PCS_MAX_POINTS, n_points[m] - p0);
+                    std::vector<uint32_t> z(4 * P), van(4 * P);
+                    for (uint32_t p = 0; p < P; ++p) {
+                        HE4 zz;
+                        for (int q = 0; q < 4; ++q) zz[q] = points[4 * (pt + p0 + p) + q] % BB_P;
+                        const HE4 v = he4_sub(he4_pow(zz, N), HE4{bb_pow(g, N), 0, 0, 0});
+                        for (int q = 0; q < 4; ++q) {
+                            z[4 * p + q] = zz[q];
+                            van[4 * p + q] = v[q];
+                        }
+                    }
+                    DevPtr zd = dalloc(ctx, z.size() * 4), vd = dalloc(ctx, van.size() * 4);
+                    h2d(ctx, zd->ptr, z.data(), z.size() * 4);
+                    h2d(ctx, vd->ptr, van.data(), van.size() * 4);
+                    DevPtr u = dalloc(ctx, N * P * 16);
+                    DevPtr part = dalloc(ctx, uint64_t(n_blocks) * P * w * 16);
+                    const LdeView v = lde_view(L);
+                    launch_check(pcs_bary_weights_device(v, g, omega_m, common_m, dptr<uint32_t>(zd), P,
+                                                         dptr<uint32_t>(u), ctx->stream),
+                                 "pcs weights");
+                    launch_check(pcs_bary_dot_device(v, dptr<uint32_t>(u), P, rows_per_block, dptr<uint32_t>(part),
+                                                     ctx->stream),
+                                 "pcs dot");
+                    launch_check(pcs_bary_finish_device(dptr<uint32_t>(part), n_blocks, P, w, dptr<uint32_t>(vd),
+                                                        dptr<uint32_t>(opened_dev) + 4 * off, ctx->stream),
+                                 "pcs finish");
+                    off += uint64_t(P) * w;
+                }
+                pt += n_points[m];
+            }
+            if (total_open) d2h(ctx, opened_host.data(), dptr<uint32_t>(opened_dev), total_open * 16);
+        }
+        if (total_open) std::memcpy(opened_out, opened_host.data(), total_open * 16);
+        // the transcript observes every opened value (observe_challenge)
+        for (uint64_t o = 0; o < total_open * 4; o += 0xFFFF0000u) {
+            ChalProgram po{};
+            po.op[po.n_ops++] = chal_op(CH_OBSERVE, uint32_t(std::min<uint64_t>(0xFFFF0000u, total_open * 4 - o)), 0,
+                                        dptr<uint32_t>(opened_dev) + o);
+            chal_run(ch, po);
+        }
+        // quotients, one vector per LDE height (descending), bit-reversed
+        std::map<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> groups;
+        for (uint32_t m = 0; m < nm; ++m) groups[log2_exact(mats[m]->height)].push_back(m);
+        std::vector<uint64_t> aoff(nm), ooff(nm);
+        {
+            uint64_t a = 0;
+            for (uint32_t m = 0; m < nm; ++m) {
+                aoff[m] = a;
+                ooff[m] = a;  // opened values are laid out in the same [matrix][point][col] order
+                a += uint64_t(n_points[m]) * mats[m]->width;
+            }
+        }
+        std::vector<p3h_mat*> inputs;
+        try {
+            Phase ph(ctx, "pcs_quotients");
+            for (auto& [lN, ms] : groups) {
+                p3h_mat* q = new_mat(ctx, P3H_BABYBEAR, 1ull << lN, 4);
+                inputs.push_back(q);
+                hip_check(hipMemsetAsync(q->data<uint32_t>(), 0, (16ull << lN), ctx->stream), "memset");
+                for (uint32_t m : ms) {
+                    const p3h_mat* L = mats[m];
+                    const uint32_t w = L->width, P = n_points[m], g = uint32_t(shifts[m] % BB_P);
+                    if (P == 0) continue;
+                    // alpha^c (Montgomery), S_p = sum_c alpha^c f_p[c], alpha^(a + p w) (Montgomery)
+                    std::vector<uint32_t> apow(4 * w), z(4 * P), sv(4 * P), ao(4 * P);
+                    std::vector<HE4> ap(w);
+                    HE4 cur = {1, 0, 0, 0};
+                    for (uint32_t c = 0; c < w; ++c) {
+                        ap[c] = cur;
+                        const HE4 cm = he4_mont(cur);
+                        for (int qq = 0; qq < 4; ++qq) apow[4 * c + qq] = cm[qq];
+                        cur = he4_mul(cur, alpha);
+                    }
+                    uint64_t pbase = 0;
+                    for (uint32_t mm = 0; mm < m; ++mm) pbase += n_points[mm];
+                    for (uint32_t p = 0; p < P; ++p) {
+                        HE4 acc = {0, 0, 0, 0};
+                        for (uint32_t c = 0; c < w; ++c) {
+                            const uint32_t* f = &opened_host[4 * (ooff[m] + uint64_t(p) * w + c)];
+                            acc = he4_add(acc, he4_mul(ap[c], HE4{f[0], f[1], f[2], f[3]}));
+                        }
+                        const HE4 a = he4_mont(he4_pow(alpha, aoff[m] + uint64_t(p) * w));
+                        for (int qq = 0; qq < 4; ++qq) {
+                            z[4 * p + qq] = points[4 * (pbase + p) + qq] % BB_P;
+                            sv[4 * p + qq] = acc[qq];
+                            ao[4 * p + qq] = a[qq];
+                        }
+                    }
+                    DevPtr ad = dalloc(ctx, apow.size() * 4), zd = dalloc(ctx, z.size() * 4),
+                           sd = dalloc(ctx, sv.size() * 4), od = dalloc(ctx, ao.size() * 4);
+                    h2d(ctx, ad->ptr, apow.data(), apow.size() * 4);
+                    h2d(ctx, zd->ptr, z.data(), z.size() * 4);
+                    h2d(ctx, sd->ptr, sv.data(), sv.size() * 4);
+                    h2d(ctx, od->ptr, ao.data(), ao.size() * 4);
+                    launch_check(pcs_quotient_device(lde_view(L), g, bb_mont(bb_root(lN)), dptr<uint32_t>(ad),
+                                                     dptr<uint32_t>(zd), dptr<uint32_t>(sd), dptr<uint32_t>(od), P,
+                                                     q->data<uint32_t>(), ctx->stream),
+                                 "pcs quotient");
+                    hip_check(hipStreamSynchronize(ctx->stream), "sync");  // the small tables above are freed next
+                }
+            }
+            // prove_fri over the quotient vectors, then its query phase
+            p3h_fri* f = nullptr;
+            std::vector<uint32_t> betas(4 * 32);
+            std::vector<p3h_mat*> extra(inputs.begin() + 1, inputs.end());
+            do_fri_commit(ctx, inputs[0], kb, log_final_poly_len, max_log_arity, nullptr, ch, commit_pow_bits,
+                          roots_out, betas.data(), commit_witnesses_out, final_poly_out, n_rounds, &f, extra);
+            std::unique_ptr<p3h_fri, int (*)(p3h_fri*)> fguard(f, p3h_fri_free);
+            *query_witness_out = chal_grind(ch, query_pow_bits);
+            const uint32_t log_max = log2_exact(inputs[0]->height);
+            const uint32_t per = (log_max + 30) / 31;
+            std::vector<uint32_t> smp(uint64_t(num_queries) * per);
+            for (uint64_t off = 0; off < smp.size(); off += 64) {
+                const uint32_t cnt = uint32_t(std::min<uint64_t>(64, smp.size() - off));
+                ChalProgram ps{};
+                ps.op[ps.n_ops++] = chal_op(CH_SAMPLE, cnt, 0, nullptr, chal_io(ch));
+                chal_run(ch, ps);
+                d2h(ctx, smp.data() + off, chal_io(ch), cnt * 4);
+            }
+            for (uint32_t q = 0; q < num_queries; ++q) {  // sample_bits (duplex_challenger.hpp:137-148)
+                uint64_t r = 0;
+                uint32_t rem = log_max;
+                for (uint32_t i = 0; i < per; ++i) {
+                    const uint32_t chunk = std::min(rem, 31u);
+                    r |= uint64_t(smp[q * per + i] & ((1u << chunk) - 1u)) << (log_max - rem);
+                    rem -= chunk;
+                }
+                query_indices_out[q] = r;
+            }
+            // PcsQueryInputMmcs::open: per height group the concatenated LDE rows at
+            // natural index bitrev(index >> (log_max - log_h), log_h)
+            if (num_queries) {
+                uint64_t col0 = 0;
+                DevPtr rows_d = dalloc(ctx, uint64_t(num_queries) * 8);
+                for (auto& [lN, ms] : groups) {
+                    std::vector<RowSeg<uint32_t>> segs;
+                    uint32_t gw = 0;
+                    for (uint32_t m : ms) {
+                        segs.push_back(seg_of<HBB>(mats[m]));
+                        gw += mats[m]->width;
+                    }
+                    std::vector<uint64_t> rows(num_queries);
+                    for (uint32_t q = 0; q < num_queries; ++q) {
+                        const uint64_t red = query_indices_out[q] >> (log_max - lN);
+                        uint64_t k = 0;
+                        for (uint32_t b = 0; b < lN; ++b) k |= ((red >> b) & 1ull) << (lN - 1 - b);
+                        rows[q] = k;
+                    }
+                    DevPtr sd = upload_segs<HBB>(ctx, segs);
+                    h2d(ctx, rows_d->ptr, rows.data(), rows.size() * 8);
+                    DevPtr out = dalloc(ctx, uint64_t(num_queries) * gw * 4);
+                    launch_check(gather_rows_device<uint32_t>(dptr<RowSeg<uint32_t>>(sd), uint32_t(segs.size()), gw,
+                                                              dptr<uint64_t>(rows_d), num_queries,
+                                                              dptr<uint32_t>(out), ctx->stream),
+                                 "query rows");
+                    std::vector<uint32_t> host(uint64_t(num_queries) * gw);
+                    d2h(ctx, host.data(), dptr<uint32_t>(out), host.size() * 4);
+                    for (uint32_t q = 0; q < num_queries; ++q)
+                        std::memcpy(query_rows_out + uint64_t(q) * total_w + col0, &host[uint64_t(q) * gw], gw * 4);
+                    col0 += gw;
+                }
+            }
+            if (keep) *keep = fguard.release();
+        } catch (...) {
+            for (auto* q : inputs) mat_unref(q);
+            throw;
+        }
+        for (auto* q : inputs) mat_unref(q);
     });
 }
 

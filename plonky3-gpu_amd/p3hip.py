@@ -96,6 +96,8 @@ _SIGS = {
     "p3h_challenger_state": (_i32, [_vp, _vp]),
     "p3h_fri_free": (_i32, [_vp]),
     "p3h_pcs_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u32, C.POINTER(_vp)]),
+    "p3h_pcs_open": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp,
+                            _vp, _vp, C.POINTER(_u32), C.POINTER(_vp)]),
     "p3h_batch_commit": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u64, _u32, C.POINTER(_vp)]),
     "p3h_mat_write": (_i32, [_vp, _vp, _vp]),
     "p3h_batch_plan": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u64, _u32, _i32, C.POINTER(_vp)]),
@@ -266,7 +268,44 @@ class Context:
         sh = np.ascontiguousarray(shifts, dtype=np.uint64)
         h = _vp()
         self._check(self.lib.p3h_pcs_commit(self.h, arr, _ptr(sh), len(mats), log_blowup, C.byref(h)))
-        return Tree(self, h, BABYBEAR)
+        t = Tree(self, h, BABYBEAR)
+        t.mat_widths = [m.width for m in mats]  # the committed matrices, for pcs_open's output sizes
+        return t
+
+    def pcs_open(self, tree: "Tree", shifts, points, challenger: "Challenger", log_final_poly_len: int,
+                 max_log_arity: int, num_queries: int, commit_pow_bits: int = 0, query_pow_bits: int = 0,
+                 keep: bool = False):
+        """TwoAdicFriPcs::open over the LDEs of a p3h_pcs_commit tree; points: per
+        matrix an (n x 4) array of BabyBear^4 values.  Returns a dict."""
+        n = len(points)
+        sh = np.ascontiguousarray(shifts, dtype=np.uint64)
+        npts = np.array([len(p) for p in points], dtype=np.uint32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.uint32).reshape(-1) for p in points]
+                                                   + [np.zeros(4, dtype=np.uint32)]))
+        widths = tree.mat_widths
+        n_open = int(sum(int(k) * w for k, w in zip(npts, widths)))
+        tw = int(sum(widths))
+        opened = np.zeros((max(1, n_open), 4), dtype=np.uint32)
+        roots = np.zeros((32, 8), dtype=np.uint32)
+        cwit = np.zeros(32, dtype=np.uint64)
+        final = np.zeros((1 << log_final_poly_len, 4), dtype=np.uint32)
+        qwit = np.zeros(1, dtype=np.uint64)
+        qidx = np.zeros(max(1, num_queries), dtype=np.uint64)
+        qrows = np.zeros((max(1, num_queries), tw), dtype=np.uint32)
+        nr = _u32()
+        kp = _vp()
+        assert n == len(widths), "one point list per committed matrix"
+        self._check(self.lib.p3h_pcs_open(self.h, tree.h, _ptr(sh), _ptr(npts), _ptr(flat), challenger.h,
+                                          log_final_poly_len, max_log_arity, num_queries, commit_pow_bits,
+                                          query_pow_bits, _ptr(opened), _ptr(roots), _ptr(cwit), _ptr(final),
+                                          _ptr(qwit), _ptr(qidx), _ptr(qrows), C.byref(nr),
+                                          C.byref(kp) if keep else None))
+        r = nr.value
+        out = {"opened": opened[:n_open], "roots": roots[:r], "commit_witnesses": cwit[:r], "final": final,
+               "query_witness": qwit, "query_indices": qidx[:num_queries], "query_rows": qrows[:num_queries]}
+        if keep:
+            out["fri"] = Fri(self, kp)
+        return out
 
     def batch_commit(self, traces, added_bits, shift: int, cap_height: int = 0) -> "Tree":
         arr = (_vp * len(traces))(*[m.h for m in traces])
