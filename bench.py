@@ -48,7 +48,7 @@ WORKLOADS = {
     "gl_2^22x64_b2": ("gl", 22, 64, 2, 22),
 }
 DEFAULT = "bb_2^20x256_b1"
-SPECIAL = ("batch64", "serve", "fri_bb4_2^23", "fri_bb4_2^23_chal", "grind_bb")
+SPECIAL = ("batch64", "serve", "fri_bb4_2^23", "fri_bb4_2^23_chal", "grind_bb", "pcs_open_bb_2^20x256")
 
 
 def load_p3hip():
@@ -130,6 +130,8 @@ def main():
         return bench_fri(p3hip, args, challenger=True)
     if args.workload == "grind_bb":
         return bench_grind(p3hip, args)
+    if args.workload == "pcs_open_bb_2^20x256":
+        return bench_pcs_open(p3hip, args)
     field, log_n, width, lb, rp = WORKLOADS[args.workload]
     fid = p3hip.BABYBEAR if field == "bb" else p3hip.GOLDILOCKS
     n, N = 1 << log_n, 1 << (log_n + lb)
@@ -462,6 +464,75 @@ def bench_grind(p3hip, args, bits: int = 24, cpu_bits: int = 13):
             ctries += int(w) + 1
         out["cpu_baseline"] = {"value": ctries / ctime, "unit": "witnesses/s", "cores": 1, "kind": kind,
                                "sample": f"8 grinds of {cpu_bits} bits, {ctries} witnesses in {ctime:.2f} s"}
+    print(json.dumps(out))
+    ctx.close()
+
+
+def bench_pcs_open(p3hip, args, log_n: int = 20, width: int = 256, n_points: int = 2, lfpl: int = 3,
+                   num_queries: int = 100, query_pow_bits: int = 16):
+    """A PCS round: TwoAdicFriPcs::commit of a 2^20 x 256 BabyBear trace
+    (log_blowup 1) and ::open at 2 BabyBear^4 points -- barycentric opened
+    values, quotient vector, prove_fri (arity 2, final poly 2^3), 16-bit query
+    proof of work, 100 query indices and rows -- with the duplex transcript on
+    the device.  value = committed-and-opened LDE elements per second."""
+    consts = p2_constants("bb", 13)
+    ctx = p3hip.Context(0)
+    ctx.set_poseidon2(p3hip.BABYBEAR, consts, 8, 13)
+    rng = np.random.default_rng(12)
+    trace = ctx.upload(rng.integers(0, BB_P, size=(1 << log_n, width), dtype=np.uint32))
+    pts = [rng.integers(0, BB_P, size=(n_points, 4), dtype=np.uint32)]
+    if not args.no_autotune:
+        ctx.autotune_lde(p3hip.BABYBEAR, log_n, width, 1, iters=2)
+
+    def step():
+        tree = ctx.pcs_commit([trace], [31], 1)
+        ch = ctx.challenger()
+        ch.observe_cap(tree)
+        r = ctx.pcs_open(tree, [31], pts, ch, lfpl, 1, num_queries, 0, query_pow_bits)
+        ch.free()
+        tree.free()
+        return r
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    outs = [step() for _ in range(args.steps)]
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    phases = {ph: ctx.phase_stats(ph)[1] / args.steps
+              for ph in ("lde", "leaf_hash", "compress", "pcs_open_values", "pcs_quotients", "fold", "fri_queries")}
+    ctx.set_profiling(False)
+    assert all((o["roots"] == outs[0]["roots"]).all() for o in outs), "non-deterministic"
+    elems = (2 << log_n) * width
+    ms = dt * 1e3 / args.steps
+    out = {
+        "metric": "PCS commit + open, LDE elems/s", "value": elems / (ms / 1e3), "unit": "field elems/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic (seeded trace and points)",
+        "config": {"workload": f"TwoAdicFriPcs commit + open, BabyBear 2^{log_n} x {width}, log_blowup 1, "
+                               f"{n_points} points, FRI arity 2, final poly 2^{lfpl}, {num_queries} queries, "
+                               f"query PoW {query_pow_bits} bits"},
+        "phases_ms": phases,
+        "rounds": len(outs[0]["roots"]),
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import native  # test infrastructure; used here only as the CPU baseline
+
+        lib, kind = native.reference(), "reference"
+        if lib is None:
+            lib, kind = native.oracle(), "port"
+        sl = 12
+        m = rng.integers(0, BB_P, size=(1 << sl, width), dtype=np.uint32)
+        t0 = time.perf_counter()
+        lib.pcs_open(consts, 8, 13, [m], [31], 1, pts, [], lfpl, 1, num_queries, 0, 8)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ((2 << sl) * width) / cdt, "unit": "field elems/s", "cores": 1,
+                               "kind": kind,
+                               "sample": f"commit + open of 2^{sl} x {width}, same parameters except 8-bit query "
+                                         f"PoW, {cdt:.2f} s on 1 host thread"}
     print(json.dumps(out))
     ctx.close()
 

@@ -31,9 +31,10 @@ int pcs_bary_dot_device(const LdeView& v, const uint32_t* u_m, uint32_t P, uint3
 int pcs_bary_finish_device(const uint32_t* partial, uint32_t n_blocks, uint32_t P, uint32_t w, const uint32_t* van,
                            uint32_t* out, hipStream_t st);
 // q[bitrev(j)] += sum_p aoff_p * (s_p - sum_c apow_c * lde[j][c]) / (z_p - shift * omega_N^j)
+// ab: scratch of 8 words per LDE row.
 int pcs_quotient_device(const LdeView& v, uint32_t shift, uint32_t omega_m, const uint32_t* apow_m,
-                        const uint32_t* z, const uint32_t* s, const uint32_t* aoff_m, uint32_t P, uint32_t* q,
-                        hipStream_t st);
+                        const uint32_t* z, const uint32_t* s, const uint32_t* aoff_m, uint32_t P, uint32_t* ab,
+                        uint32_t* q, hipStream_t st);
 // cur[i] += beta^(2^log_arity) * next[i] for i < n (BabyBear^4 vectors); beta
 // from beta_dev (device) when not null, else from beta_host.
 int fri_mix_device(uint32_t* cur, const uint32_t* next, uint64_t n, const uint32_t* beta_host,

@@ -18,6 +18,8 @@
 // Sums of Montgomery products are accumulated lazily in 64-bit integers and
 // reduced once.
 
+#include <algorithm>
+
 #include "ext4.h"
 #include "pcs.h"
 
@@ -80,53 +82,123 @@ __global__ void bary_dot_kernel(LdeView v, const uint32_t* u_m, uint32_t P, uint
             partial[((uint64_t(blockIdx.x) * P + p) * v.w + c) * 4 + q] = reduce64(acc[p][q]);
 }
 
-__global__ void bary_finish_kernel(const uint32_t* partial, uint32_t n_blocks, uint32_t P, uint32_t w,
-                                   const uint32_t* van, uint32_t* out) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P * w) return;
+// one workgroup per output (p, c): the threads stride over the row blocks'
+// partials and reduce in LDS
+__global__ __launch_bounds__(256) void bary_finish_kernel(const uint32_t* partial, uint32_t n_blocks, uint32_t P,
+                                                          uint32_t w, const uint32_t* van, uint32_t* out) {
+    __shared__ uint64_t red[4][256];
+    const uint32_t t = blockIdx.x;  // = p * w + c
     const uint32_t p = t / w;
     uint64_t acc[4] = {0, 0, 0, 0};
-    for (uint32_t b = 0; b < n_blocks; ++b)
+    for (uint32_t b = threadIdx.x; b < n_blocks; b += 256)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] += partial[(uint64_t(b) * P * w + t) * 4 + q];
-    const E4 sum = {{reduce64(acc[0]), reduce64(acc[1]), reduce64(acc[2]), reduce64(acc[3])}};
-    const E4 vz = {{van[4 * p], van[4 * p + 1], van[4 * p + 2], van[4 * p + 3]}};
-    const E4 r = e4_mul_m(sum, e4_to_m(vz));
 #pragma unroll
-    for (int q = 0; q < 4; ++q) out[uint64_t(t) * 4 + q] = r.c[q];
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const E4 sum = {{reduce64(red[0][0]), reduce64(red[1][0]), reduce64(red[2][0]), reduce64(red[3][0])}};
+        const E4 vz = {{van[4 * p], van[4 * p + 1], van[4 * p + 2], van[4 * p + 3]}};
+        const E4 r = e4_mul_m(sum, e4_to_m(vz));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[uint64_t(t) * 4 + q] = r.c[q];
+    }
 }
 
-__global__ void quotient_kernel(LdeView v, uint32_t shift, uint32_t omega_m, const uint32_t* apow_m,
-                                const uint32_t* z, const uint32_t* s_c, const uint32_t* aoff_m, uint32_t P,
-                                uint32_t* qv) {
+// Quotient, pass 1 (one thread per row): with w_p = aoff_p / (z_p - x_j),
+// A_j = sum_p w_p S_p and B_j = sum_p w_p, so that q_j = A_j - R_j B_j.
+__global__ void quotient_weights_kernel(LdeView v, uint32_t shift, uint32_t omega_m, const uint32_t* z,
+                                        const uint32_t* s_c, const uint32_t* aoff_m, uint32_t P, uint32_t* ab) {
     const uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint32_t lN = v.L + v.k;
-    if (s >= (1ull << lN)) return;
+    if (s >= (1ull << (v.L + v.k))) return;
     const uint32_t j = lde_row_of(s, v.L, v.k);
-    // R = sum_c alpha^c * lde[j][c]
-    uint64_t acc[4] = {0, 0, 0, 0};
-    const uint32_t* row = v.data + s * v.w;
-    for (uint32_t c = 0; c < v.w; ++c) {
-        const uint32_t x = row[c];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] += BB::mont_lazy(x, apow_m[4 * c + q]);
-    }
-    const E4 R = {{reduce64(acc[0]), reduce64(acc[1]), reduce64(acc[2]), reduce64(acc[3])}};
     const uint32_t h = BB::from_mont(bb_pow_m(omega_m, j));
     const uint32_t x = BB::from_mont(BB::mont(BB::to_mont(h), BB::to_mont(shift)));
-    E4 sum = {{0, 0, 0, 0}};
+    E4 A = {{0, 0, 0, 0}}, B = {{0, 0, 0, 0}};
     for (uint32_t p = 0; p < P; ++p) {
         E4 d = {{z[4 * p], z[4 * p + 1], z[4 * p + 2], z[4 * p + 3]}};
         d.c[0] = BB::sub(d.c[0], x);
-        const E4 sp = {{s_c[4 * p], s_c[4 * p + 1], s_c[4 * p + 2], s_c[4 * p + 3]}};
         const E4 ao = {{aoff_m[4 * p], aoff_m[4 * p + 1], aoff_m[4 * p + 2], aoff_m[4 * p + 3]}};
-        const E4 t = e4_mul_m(e4_mul_m(e4_sub(sp, R), ao), e4_to_m(e4_inv(d)));
-        sum = e4_add(sum, t);
+        const E4 w = e4_mul_m(e4_inv(d), ao);
+        const E4 sp = {{s_c[4 * p], s_c[4 * p + 1], s_c[4 * p + 2], s_c[4 * p + 3]}};
+        A = e4_add(A, e4_mul_m(sp, e4_to_m(w)));
+        B = e4_add(B, w);
     }
-    const uint64_t pos = lN == 0 ? 0 : (__brev(j) >> (32 - lN));
-    uint32_t* o = qv + pos * 4;
+    const E4 Bm = e4_to_m(B);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) o[q] = BB::add(o[q], sum.c[q]);
+    for (int q = 0; q < 4; ++q) {
+        ab[s * 8 + q] = A.c[q];
+        ab[s * 8 + 4 + q] = Bm.c[q];
+    }
+}
+
+// x mod p for x < 2^64: x = hi 2^32 + lo, hi 2^32 = mont(hi, 2^64 mod p).
+__device__ __forceinline__ uint32_t reduce_u64(uint64_t x) {
+    const uint32_t hi = uint32_t(x >> 32), lo = uint32_t(x);
+    uint32_t l = lo;
+    l = l >= BB::P ? l - BB::P : l;
+    l = l >= BB::P ? l - BB::P : l;
+    return BB::add(BB::mont(hi, BB::R2_MOD_P), l);
+}
+
+// Quotient, pass 2: 16 lanes per row read the row with 16-byte loads (16
+// lanes x 4 columns per step), multiply by alpha^c from an LDS table and sum
+// across the 16 lanes; lane 0 of the row adds A_j - R_j B_j at bitrev(j).
+constexpr uint32_t QR_THREADS = 256;
+constexpr uint32_t QR_LDS_COLS = 4096;
+__global__ __launch_bounds__(QR_THREADS) void quotient_reduce_kernel(LdeView v, const uint32_t* apow_m,
+                                                                     const uint32_t* ab, uint32_t* qv) {
+    __shared__ uint32_t at[QR_LDS_COLS * 4];
+    const bool in_lds = v.w <= QR_LDS_COLS;
+    if (in_lds)
+        for (uint32_t t = threadIdx.x; t < v.w * 4; t += QR_THREADS) at[t] = apow_m[t];
+    __syncthreads();
+    const uint32_t* ap = in_lds ? at : apow_m;
+    const uint32_t lN = v.L + v.k;
+    const uint64_t N = 1ull << lN;
+    const uint32_t sub = threadIdx.x & 15;
+    const bool vec = (v.w % 4) == 0 && (reinterpret_cast<uintptr_t>(v.data) & 15) == 0;
+    for (uint64_t s = (uint64_t(blockIdx.x) * QR_THREADS + threadIdx.x) >> 4; s < N;
+         s += (uint64_t(gridDim.x) * QR_THREADS) >> 4) {
+        const uint32_t* row = v.data + s * v.w;
+        uint64_t acc[4] = {0, 0, 0, 0};
+        if (vec) {
+            for (uint32_t c = sub * 4; c < v.w; c += 64) {
+                const uint4 x = *reinterpret_cast<const uint4*>(row + c);
+                const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] += BB::mont_lazy(xs[e], ap[4 * (c + e) + q]);
+            }
+        } else {
+            for (uint32_t c = sub; c < v.w; c += 16) {
+                const uint32_t x = row[c];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] += BB::mont_lazy(x, ap[4 * c + q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            for (int off = 8; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 16);
+        if (sub == 0) {
+            const E4 R = {{reduce_u64(acc[0]), reduce_u64(acc[1]), reduce_u64(acc[2]), reduce_u64(acc[3])}};
+            const uint32_t* a = ab + s * 8;
+            const E4 A = {{a[0], a[1], a[2], a[3]}}, Bm = {{a[4], a[5], a[6], a[7]}};
+            const E4 t = e4_sub(A, e4_mul_m(R, Bm));
+            const uint32_t j = lde_row_of(s, v.L, v.k);
+            const uint64_t pos = lN == 0 ? 0 : (__brev(j) >> (32 - lN));
+            uint32_t* o = qv + pos * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = BB::add(o[q], t.c[q]);
+        }
+    }
 }
 
 __global__ void fri_mix_kernel(uint32_t* cur, const uint32_t* next, uint64_t n, E4 beta, const uint32_t* beta_dev,
@@ -171,17 +243,19 @@ int pcs_bary_dot_device(const LdeView& v, const uint32_t* u_m, uint32_t P, uint3
 
 int pcs_bary_finish_device(const uint32_t* partial, uint32_t n_blocks, uint32_t P, uint32_t w, const uint32_t* van,
                            uint32_t* out, hipStream_t st) {
-    hipLaunchKernelGGL(bary_finish_kernel, dim3(blocks_for(uint64_t(P) * w, 256)), dim3(256), 0, st, partial,
-                       n_blocks, P, w, van, out);
+    if (uint64_t(P) * w == 0) return 0;
+    hipLaunchKernelGGL(bary_finish_kernel, dim3(P * w), dim3(256), 0, st, partial, n_blocks, P, w, van, out);
     return int(hipGetLastError());
 }
 
 int pcs_quotient_device(const LdeView& v, uint32_t shift, uint32_t omega_m, const uint32_t* apow_m,
-                        const uint32_t* z, const uint32_t* s, const uint32_t* aoff_m, uint32_t P, uint32_t* q,
-                        hipStream_t st) {
+                        const uint32_t* z, const uint32_t* s, const uint32_t* aoff_m, uint32_t P, uint32_t* ab,
+                        uint32_t* q, hipStream_t st) {
     const uint64_t N = 1ull << (v.L + v.k);
-    hipLaunchKernelGGL(quotient_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, v, shift, omega_m, apow_m, z, s,
-                       aoff_m, P, q);
+    hipLaunchKernelGGL(quotient_weights_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, v, shift, omega_m, z, s,
+                       aoff_m, P, ab);
+    const uint64_t grid = std::min<uint64_t>(blocks_for(N * 16, QR_THREADS), 1u << 16);
+    hipLaunchKernelGGL(quotient_reduce_kernel, dim3(uint32_t(grid)), dim3(QR_THREADS), 0, st, v, apow_m, ab, q);
     return int(hipGetLastError());
 }
 

@@ -1804,7 +1804,7 @@ int p3h_pcs_open(p3h_ctx* ctx, const p3h_tree* tree, const uint64_t* shifts, con
         }
         std::vector<p3h_mat*> inputs;
         try {
-            Phase ph(ctx, "pcs_quotients");
+            std::unique_ptr<Phase> ph(new Phase(ctx, "pcs_quotients"));
             for (auto& [lN, ms] : groups) {
                 p3h_mat* q = new_mat(ctx, P3H_BABYBEAR, 1ull << lN, 4);
                 inputs.push_back(q);
@@ -1839,18 +1839,20 @@ int p3h_pcs_open(p3h_ctx* ctx, const p3h_tree* tree, const uint64_t* shifts, con
                         }
                     }
                     DevPtr ad = dalloc(ctx, apow.size() * 4), zd = dalloc(ctx, z.size() * 4),
-                           sd = dalloc(ctx, sv.size() * 4), od = dalloc(ctx, ao.size() * 4);
+                           sd = dalloc(ctx, sv.size() * 4), od = dalloc(ctx, ao.size() * 4),
+                           abd = dalloc(ctx, L->height * 32);
                     h2d(ctx, ad->ptr, apow.data(), apow.size() * 4);
                     h2d(ctx, zd->ptr, z.data(), z.size() * 4);
                     h2d(ctx, sd->ptr, sv.data(), sv.size() * 4);
                     h2d(ctx, od->ptr, ao.data(), ao.size() * 4);
                     launch_check(pcs_quotient_device(lde_view(L), g, bb_mont(bb_root(lN)), dptr<uint32_t>(ad),
                                                      dptr<uint32_t>(zd), dptr<uint32_t>(sd), dptr<uint32_t>(od), P,
-                                                     q->data<uint32_t>(), ctx->stream),
+                                                     dptr<uint32_t>(abd), q->data<uint32_t>(), ctx->stream),
                                  "pcs quotient");
                     hip_check(hipStreamSynchronize(ctx->stream), "sync");  // the small tables above are freed next
                 }
             }
+            ph.reset();
             // prove_fri over the quotient vectors, then its query phase
             p3h_fri* f = nullptr;
             std::vector<uint32_t> betas(4 * 32);
@@ -1858,6 +1860,7 @@ int p3h_pcs_open(p3h_ctx* ctx, const p3h_tree* tree, const uint64_t* shifts, con
             do_fri_commit(ctx, inputs[0], kb, log_final_poly_len, max_log_arity, nullptr, ch, commit_pow_bits,
                           roots_out, betas.data(), commit_witnesses_out, final_poly_out, n_rounds, &f, extra);
             std::unique_ptr<p3h_fri, int (*)(p3h_fri*)> fguard(f, p3h_fri_free);
+            Phase pq(ctx, "fri_queries");
             *query_witness_out = chal_grind(ch, query_pow_bits);
             const uint32_t log_max = log2_exact(inputs[0]->height);
             const uint32_t per = (log_max + 30) / 31;
