@@ -60,18 +60,37 @@ struct Goldilocks {
         uint64_t d = a - b;
         return a < b ? d + P : d;  // d + p wraps back into [0, p)
     }
-    // (hi, lo) = 128-bit value; returns it mod p, canonical.
-    __device__ __forceinline__ static uint64_t reduce128(uint64_t hi, uint64_t lo) {
+    // (hi, lo) = 128-bit value; returns a value below 2^64 congruent to it mod p
+    // (not necessarily canonical).  Neither correction can wrap again: after a
+    // borrow t0 >= 2^64 - 2^33, and t0 + t1 - 2^64 + EPS < 2^64 - 1.
+    __device__ __forceinline__ static uint64_t reduce128_lazy(uint64_t hi, uint64_t lo) {
         uint64_t hi_hi = hi >> 32, hi_lo = hi & EPS;
         uint64_t t0 = lo - hi_hi;
         if (lo < hi_hi) t0 -= EPS;                   // borrow: -2^64 == -(2^32 - 1)
         uint64_t t1 = (hi_lo << 32) - hi_lo;         // hi_lo * (2^32 - 1), no overflow
         uint64_t t2 = t0 + t1;
         if (t2 < t0) t2 += EPS;                      // carry: +2^64 == +(2^32 - 1)
-        return canon(t2);
+        return t2;
+    }
+    // (hi, lo) = 128-bit value; returns it mod p, canonical.
+    __device__ __forceinline__ static uint64_t reduce128(uint64_t hi, uint64_t lo) {
+        return canon(reduce128_lazy(hi, lo));
     }
     __device__ __forceinline__ static uint64_t mul(uint64_t a, uint64_t b) {
         return reduce128(__umul64hi(a, b), a * b);
+    }
+    // any a, b < 2^64 -> a*b mod p as a value below 2^64 (not necessarily canonical)
+    __device__ __forceinline__ static uint64_t mul_lazy(uint64_t a, uint64_t b) {
+        return reduce128_lazy(__umul64hi(a, b), a * b);
+    }
+    // x * 2^i mod p, canonical, for 1 <= i < 32: the product's high word is
+    // below 2^i, so only the hi * (2^32 - 1) term of the reduction remains.
+    __device__ __forceinline__ static uint64_t mul_pow2(uint64_t x, uint32_t i) {
+        const uint64_t lo = x << i, hi = x >> (64 - i);
+        const uint64_t t1 = (hi << 32) - hi;
+        uint64_t t2 = lo + t1;
+        if (t2 < lo) t2 += EPS;
+        return canon(t2);
     }
     __device__ __forceinline__ static uint64_t mul_c(uint64_t x, uint64_t c) { return mul(x, c); }
 };

@@ -209,9 +209,15 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_dense_f64_kernel(RowSe
 #pragma unroll
     for (int i = 0; i < 16; ++i) s[i] = 0.0;
     const uint32_t* rowp = seg.ptr + stored_row(seg, j) * seg.w;
+    // the next chunk's loads are issued before the current chunk's permutation
+    const uint4* v = reinterpret_cast<const uint4*>(rowp);
+    uint4 na = v[0], nb = v[1];
     for (uint32_t start = 0; start < seg.w; start += 8) {
-        const uint4* v = reinterpret_cast<const uint4*>(rowp + start);
-        uint4 a = v[0], b = v[1];
+        const uint4 a = na, b = nb;
+        if (start + 8 < seg.w) {
+            na = v[(start + 8) / 4];
+            nb = v[(start + 8) / 4 + 1];
+        }
         s[0] = double(a.x);
         s[1] = double(a.y);
         s[2] = double(a.z);

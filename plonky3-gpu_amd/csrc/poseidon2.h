@@ -35,6 +35,10 @@ struct BBDom {
     static constexpr int W = 16, RATE = 8, DIG = 8;
     __device__ __forceinline__ static T add(T a, T b) { return F::add(a, b); }
     __device__ __forceinline__ static T mul(T a, T b) { return F::mont(a, b); }
+    // a product that only feeds further products
+    __device__ __forceinline__ static T mul_inner(T a, T b) { return F::mont(a, b); }
+    // s * diag_i, diag_i = 2^i in the state domain
+    __device__ __forceinline__ static T mul_diag(T s, uint32_t, T diag_i) { return F::mont(s, diag_i); }
     __device__ __forceinline__ static T to_dom(T x) { return F::to_mont(x); }
     __device__ __forceinline__ static T from_dom(T x) { return F::from_mont(x); }
 };
@@ -45,6 +49,10 @@ struct GLDom {
     static constexpr int W = 8, RATE = 4, DIG = 4;
     __device__ __forceinline__ static T add(T a, T b) { return F::add(a, b); }
     __device__ __forceinline__ static T mul(T a, T b) { return F::mul(a, b); }
+    // a product that only feeds further products: any value below 2^64 will do
+    __device__ __forceinline__ static T mul_inner(T a, T b) { return F::mul_lazy(a, b); }
+    // s * 2^i by shifting (diag_i is 2^i)
+    __device__ __forceinline__ static T mul_diag(T s, uint32_t i, T) { return F::mul_pow2(s, i); }
     __device__ __forceinline__ static T to_dom(T x) { return x; }
     __device__ __forceinline__ static T from_dom(T x) { return x; }
 };
@@ -55,9 +63,9 @@ struct Poseidon2 {
     static constexpr int W = D::W;
 
     __device__ __forceinline__ static T sbox(T x) {
-        T x2 = D::mul(x, x);
-        T x3 = D::mul(x2, x);
-        T x4 = D::mul(x2, x2);
+        T x2 = D::mul_inner(x, x);
+        T x3 = D::mul_inner(x2, x);
+        T x4 = D::mul_inner(x2, x2);
         return D::mul(x3, x4);
     }
     // [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] x
@@ -100,7 +108,7 @@ struct Poseidon2 {
         for (int i = 1; i < W; ++i) sum = D::add(sum, s[i]);
         s[0] = D::add(s[0], sum);  // diag[0] = 1
 #pragma unroll
-        for (int i = 1; i < W; ++i) s[i] = D::add(D::mul(s[i], diag[i]), sum);
+        for (int i = 1; i < W; ++i) s[i] = D::add(D::mul_diag(s[i], uint32_t(i), diag[i]), sum);
     }
     __device__ __forceinline__ static void permute(T* s, const P2Params<T, W>& p) {
         mds_light(s);
