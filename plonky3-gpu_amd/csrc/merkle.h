@@ -53,6 +53,9 @@ int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, 
 // contracts as the uint32 versions above.
 int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t total_w, const RowSeg<uint32_t>& seg0,
                          uint64_t rows, uint32_t* out, const P2ParamsF64& p, hipStream_t st);
+// Whether hash_rows_f64_device reads the device table `segs` for these
+// arguments (otherwise it takes seg0 by value and segs may be null).
+bool hash_rows_f64_reads_table(uint32_t nsegs, const RowSeg<uint32_t>& seg0, uint64_t rows);
 // Levels in_count -> counts[0] -> ... -> counts[nlev-1] (each ceil(prev / 2),
 // zero digest for a missing right child) in one launch; counts[0] <= 64,
 // nlev <= 8.  Binary64 engine, no injection.
@@ -62,5 +65,22 @@ int compress_layer_f64_device(const uint32_t* prev, uint64_t prev_count, uint32_
                               const RowSeg<uint32_t>* inj, uint32_t ninj, uint32_t inj_w, const P2ParamsF64& p,
                               hipStream_t st);
 int permute_f64_device(uint32_t* states, uint64_t n, const P2ParamsF64& p, hipStream_t st);
+
+// Independent jobs sharing one launch (binary64 engine): the same level of
+// several flat trees, or the leaves of several matrices.  Every workgroup
+// belongs to one job.
+constexpr uint32_t MULTI_MAX = 32;  // jobs per launch; longer lists take several launches
+struct MultiJob {
+    const uint32_t* in;
+    uint32_t* out;
+    uint64_t in_count;   // compress: digests of the level below; leaves: unused
+    uint64_t out_count;  // digests written
+    uint32_t w;          // leaves: row width in elements, a multiple of 8
+};
+// out[j] = sponge(row j of `in`, a natural-order out_count x w matrix), 16-byte aligned rows.
+int hash_rows_multi_f64_device(const MultiJob* jobs, uint32_t n, const P2ParamsF64& p, hipStream_t st);
+// out[i] = compress(in[2i], in[2i+1] or the zero digest when 2i+1 >= in_count),
+// out_count = ceil(in_count / 2).
+int compress_multi_f64_device(const MultiJob* jobs, uint32_t n, const P2ParamsF64& p, hipStream_t st);
 
 }  // namespace p3h

@@ -351,7 +351,119 @@ __global__ __launch_bounds__(LANE_THREADS) void permute_lanes_f64_kernel(uint32_
     if (i < n) st[i * 16 + lane] = BBF64::canonical(s);
 }
 
+// Job table of a multi-job launch, passed by value.  Job t owns workgroups
+// [block0[t], block0[t+1]), so the job index is uniform in a workgroup.
+struct MultiTable {
+    const uint32_t* in[MULTI_MAX];
+    uint32_t* out[MULTI_MAX];
+    uint64_t in_count[MULTI_MAX];
+    uint64_t out_count[MULTI_MAX];
+    uint32_t w[MULTI_MAX];
+    uint32_t block0[MULTI_MAX + 1];
+    uint32_t n;
+};
+
+__device__ __forceinline__ uint32_t job_of(const MultiTable& m, uint32_t b) {
+    uint32_t t = 0;
+    for (uint32_t k = 1; k < m.n; ++k) t += b >= m.block0[k] ? 1u : 0u;
+    return t;
+}
+
+__global__ __launch_bounds__(HASH_THREADS) void hash_multi_f64_kernel(MultiTable m, P2ParamsF64 p) {
+    const uint32_t t = job_of(m, blockIdx.x);
+    const uint64_t j = uint64_t(blockIdx.x - m.block0[t]) * HASH_THREADS + threadIdx.x;
+    if (j >= m.out_count[t]) return;
+    const uint32_t w = m.w[t];
+    const uint4* v = reinterpret_cast<const uint4*>(m.in[t] + j * w);
+    double s[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = 0.0;
+    for (uint32_t start = 0; start < w; start += 8) {
+        const uint4 a = v[start / 4], b = v[start / 4 + 1];
+        s[0] = double(a.x);
+        s[1] = double(a.y);
+        s[2] = double(a.z);
+        s[3] = double(a.w);
+        s[4] = double(b.x);
+        s[5] = double(b.y);
+        s[6] = double(b.z);
+        s[7] = double(b.w);
+        Poseidon2F64::permute(s, p);
+    }
+    uint32_t* o = m.out[t] + j * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[q] = BBF64::canonical(s[q]);
+}
+
+__global__ __launch_bounds__(LANE_THREADS) void hash_multi_lanes_f64_kernel(MultiTable m, P2ParamsF64 p) {
+    const uint32_t t = job_of(m, blockIdx.x);
+    const uint32_t lane = threadIdx.x & 15;
+    const uint64_t j = uint64_t(blockIdx.x - m.block0[t]) * (LANE_THREADS / 16) + (threadIdx.x >> 4);
+    const bool live = j < m.out_count[t];
+    const uint32_t w = m.w[t];
+    const uint32_t* rowp = m.in[t] + (live ? j : 0) * w;
+    double s = 0.0;
+    for (uint32_t start = 0; start < w; start += 8) {  // w is uniform: every lane runs every permutation
+        if (live && lane < 8) s = double(rowp[start + lane]);
+        s = Poseidon2F64Lanes::permute(s, p, lane);
+    }
+    if (live && lane < 8) m.out[t][j * 8 + lane] = BBF64::canonical(s);
+}
+
+__global__ __launch_bounds__(HASH_THREADS) void compress_multi_f64_kernel(MultiTable m, P2ParamsF64 p) {
+    const uint32_t t = job_of(m, blockIdx.x);
+    const uint64_t i = uint64_t(blockIdx.x - m.block0[t]) * HASH_THREADS + threadIdx.x;
+    if (i >= m.out_count[t]) return;
+    const uint32_t* l = m.in[t] + 2 * i * 8;
+    double s[16];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] = double(l[q]);
+#pragma unroll
+    for (int q = 8; q < 16; ++q) s[q] = 0.0;
+    if (2 * i + 1 < m.in_count[t]) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[8 + q] = double(l[8 + q]);
+    }
+    Poseidon2F64::permute(s, p);
+    uint32_t* o = m.out[t] + i * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[q] = BBF64::canonical(s[q]);
+}
+
+__global__ __launch_bounds__(LANE_THREADS) void compress_multi_lanes_f64_kernel(MultiTable m, P2ParamsF64 p) {
+    const uint32_t t = job_of(m, blockIdx.x);
+    const uint32_t lane = threadIdx.x & 15;
+    const uint64_t node = uint64_t(blockIdx.x - m.block0[t]) * (LANE_THREADS / 16) + (threadIdx.x >> 4);
+    const bool live = node < m.out_count[t];
+    double s = 0.0;
+    if (live) {
+        const uint64_t child = 2 * node + (lane >> 3);  // lanes 0-7 left digest, 8-15 right digest
+        if (child < m.in_count[t]) s = double(m.in[t][child * 8 + (lane & 7)]);
+    }
+    s = Poseidon2F64Lanes::permute(s, p, lane);
+    if (live && lane < 8) m.out[t][node * 8 + lane] = BBF64::canonical(s);
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t threads) { return uint32_t((n + threads - 1) / threads); }
+
+// Table for jobs[0..n), n <= MULTI_MAX, with `per_block` states per
+// workgroup; returns the number of workgroups, or ~0 when that exceeds 2^31 - 1.
+uint64_t fill_table(MultiTable& m, const MultiJob* jobs, uint32_t n, uint32_t per_block) {
+    uint64_t b = 0;
+    m.n = n;
+    for (uint32_t k = 0; k < n; ++k) {
+        m.in[k] = jobs[k].in;
+        m.out[k] = jobs[k].out;
+        m.in_count[k] = jobs[k].in_count;
+        m.out_count[k] = jobs[k].out_count;
+        m.w[k] = jobs[k].w;
+        m.block0[k] = uint32_t(b);
+        b += (jobs[k].out_count + per_block - 1) / per_block;
+        if (b > 0x7FFFFFFFull) return ~0ull;
+    }
+    m.block0[n] = uint32_t(b);
+    return b;
+}
 
 }  // namespace
 
@@ -401,21 +513,33 @@ int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, 
 }
 
 
+static bool f64_lanes_form(uint32_t nsegs, const RowSeg<uint32_t>& seg0, uint64_t rows) {
+    return nsegs == 1 && seg0.ptr && seg0.w > 0 && rows <= LANE_MAX;
+}
+static bool f64_dense_form(uint32_t nsegs, const RowSeg<uint32_t>& seg0) {
+    return nsegs == 1 && seg0.ptr && seg0.w > 0 && seg0.w % 8 == 0 && (reinterpret_cast<uintptr_t>(seg0.ptr) & 15) == 0;
+}
+
+bool hash_rows_f64_reads_table(uint32_t nsegs, const RowSeg<uint32_t>& seg0, uint64_t rows) {
+    return !f64_lanes_form(nsegs, seg0, rows) && !f64_dense_form(nsegs, seg0);
+}
+
 int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t total_w, const RowSeg<uint32_t>& seg0,
                          uint64_t rows, uint32_t* out, const P2ParamsF64& p, hipStream_t st) {
     if (rows == 0) return 0;
-    if (nsegs == 1 && seg0.ptr && seg0.w > 0 && rows <= LANE_MAX) {
+    if (f64_lanes_form(nsegs, seg0, rows)) {
         hipLaunchKernelGGL(hash_rows_lanes_f64_kernel, dim3(grid_for(rows * 16, LANE_THREADS)), dim3(LANE_THREADS), 0,
                            st, seg0, rows, out, p);
         return int(hipGetLastError());
     }
-    bool dense = nsegs == 1 && seg0.ptr && seg0.w > 0 && seg0.w % 8 == 0 && (reinterpret_cast<uintptr_t>(seg0.ptr) & 15) == 0;
-    if (dense)
+    if (f64_dense_form(nsegs, seg0))
         hipLaunchKernelGGL(hash_rows_dense_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st,
                            seg0, rows, out, p);
-    else
+    else if (segs)
         hipLaunchKernelGGL(hash_rows_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st, segs,
                            total_w, rows, out, p);
+    else
+        return int(hipErrorInvalidValue);
     return int(hipGetLastError());
 }
 
@@ -463,7 +587,48 @@ int permute_f64_device(uint32_t* states, uint64_t n, const P2ParamsF64& p, hipSt
     return int(hipGetLastError());
 }
 
-#define P3H_INST(D)                                                                                             \
+// Launches a multi-job kernel over jobs in groups of MULTI_MAX.  With at most
+// LANE_MAX states in all, the lane-parallel kernel runs (16 states per
+// workgroup), otherwise the thread-per-state kernel.
+template <class Dense, class Lanes>
+static int launch_multi(const MultiJob* jobs, uint32_t n, Dense dense, Lanes lanes, const P2ParamsF64& p,
+                        hipStream_t st) {
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < n; ++k) total += jobs[k].out_count;
+    if (total == 0) return 0;
+    const bool lane_form = total <= LANE_MAX;
+    for (uint32_t a = 0; a < n; a += MULTI_MAX) {
+        const uint32_t c = n - a < MULTI_MAX ? n - a : MULTI_MAX;
+        MultiTable m{};
+        const uint64_t g = fill_table(m, jobs + a, c, lane_form ? LANE_THREADS / 16 : HASH_THREADS);
+        if (g == ~0ull) return int(hipErrorInvalidValue);
+        if (g == 0) continue;
+        if (lane_form)
+            hipLaunchKernelGGL(lanes, dim3(uint32_t(g)), dim3(LANE_THREADS), 0, st, m, p);
+        else
+            hipLaunchKernelGGL(dense, dim3(uint32_t(g)), dim3(HASH_THREADS), 0, st, m, p);
+        const int e = int(hipGetLastError());
+        if (e) return e;
+    }
+    return 0;
+}
+
+int hash_rows_multi_f64_device(const MultiJob* jobs, uint32_t n, const P2ParamsF64& p, hipStream_t st) {
+    for (uint32_t k = 0; k < n; ++k)
+        if (jobs[k].out_count && (!jobs[k].in || !jobs[k].out || jobs[k].w == 0 || jobs[k].w % 8 ||
+                                  (reinterpret_cast<uintptr_t>(jobs[k].in) & 15)))
+            return int(hipErrorInvalidValue);
+    return launch_multi(jobs, n, hash_multi_f64_kernel, hash_multi_lanes_f64_kernel, p, st);
+}
+
+int compress_multi_f64_device(const MultiJob* jobs, uint32_t n, const P2ParamsF64& p, hipStream_t st) {
+    for (uint32_t k = 0; k < n; ++k)
+        if (jobs[k].out_count && (!jobs[k].in || !jobs[k].out || jobs[k].out_count != (jobs[k].in_count + 1) / 2))
+            return int(hipErrorInvalidValue);
+    return launch_multi(jobs, n, compress_multi_f64_kernel, compress_multi_lanes_f64_kernel, p, st);
+}
+
+#define P3H_INST(D)                                                                                            \
     template int hash_rows_device<D>(const RowSeg<D::T>*, uint32_t, uint32_t, const RowSeg<D::T>&, uint64_t,    \
                                      D::T*, const P2Params<D::T, D::W>&, hipStream_t);                          \
     template int compress_layer_device<D>(const D::T*, uint64_t, D::T*, uint64_t, const RowSeg<D::T>*, uint32_t, \

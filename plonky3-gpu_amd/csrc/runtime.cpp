@@ -183,6 +183,9 @@ struct p3h_ctx {
     std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, NttTuning> tuned;
     bool profiling = false;
     bool capturing = false;
+    // pinned staging ring for small copies in stream order (h2d_async, d2h_gather)
+    uint8_t* stage = nullptr;
+    size_t stage_head = 0;
     std::map<std::string, PhaseStat> stats;
     struct Pending {
         std::string phase;
@@ -262,6 +265,63 @@ void d2h(p3h_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
     hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync D2H");
     hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+// Small copies through the context's pinned staging ring.  A region of the
+// ring is written again only after a stream synchronisation, so no copy still
+// queued on the stream reads bytes that have been overwritten.
+constexpr size_t STAGE_BYTES = size_t(4) << 20;
+constexpr size_t STAGE_MAX_ONE = size_t(64) << 10;
+
+uint8_t* stage_take(p3h_ctx* ctx, size_t bytes) {
+    if (!ctx->stage)
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&ctx->stage), STAGE_BYTES, hipHostMallocDefault),
+                  "hipHostMalloc");
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (ctx->stage_head + need > STAGE_BYTES) {
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        ctx->stage_head = 0;
+    }
+    uint8_t* p = ctx->stage + ctx->stage_head;
+    ctx->stage_head += need;
+    return p;
+}
+
+// Host-to-device copy that returns without waiting for the device.
+void h2d_async(p3h_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    if (bytes > STAGE_MAX_ONE || ctx->capturing) {
+        h2d(ctx, dst, src, bytes);
+        return;
+    }
+    uint8_t* p = stage_take(ctx, bytes);
+    std::memcpy(p, src, bytes);
+    hip_check(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync H2D");
+}
+
+// Several device-to-host copies with one synchronisation.
+struct Readback {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+void d2h_gather(p3h_ctx* ctx, const std::vector<Readback>& rb) {
+    size_t total = 0;
+    for (const auto& r : rb) total += (r.bytes + 255) & ~size_t(255);
+    if (total > STAGE_BYTES / 2) {
+        for (const auto& r : rb) d2h(ctx, r.dst, r.src, r.bytes);
+        return;
+    }
+    std::vector<uint8_t*> at;
+    for (const auto& r : rb) {
+        uint8_t* p = stage_take(ctx, r.bytes);
+        if (r.bytes)
+            hip_check(hipMemcpyAsync(p, r.src, r.bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync D2H");
+        at.push_back(p);
+    }
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < rb.size(); ++i)
+        if (rb[i].bytes) std::memcpy(rb[i].dst, at[i], rb[i].bytes);
 }
 
 // RAII phase timer: HIP events around the phase's launches on the context stream.
@@ -540,7 +600,7 @@ p3h_mat* do_dft(p3h_ctx* ctx, int kind, const p3h_mat* in, uint64_t shift64) {
 template <class H>
 DevPtr upload_segs(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs) {
     DevPtr d = dalloc(ctx, segs.size() * sizeof(RowSeg<typename H::T>));
-    h2d(ctx, d->ptr, segs.data(), segs.size() * sizeof(RowSeg<typename H::T>));
+    h2d_async(ctx, d->ptr, segs.data(), segs.size() * sizeof(RowSeg<typename H::T>));
     return d;
 }
 
@@ -677,9 +737,14 @@ p3h_tree* prepare_mmcs(p3h_ctx* ctx, const p3h_mat* const* mats, uint32_t n, uin
     return t;
 }
 
+// Consecutive launches of one phase share one pair of timing events.
 void run_launches(p3h_ctx* ctx, const LaunchList& ls) {
+    std::unique_ptr<Phase> ph;
     for (const auto& l : ls) {
-        Phase ph(ctx, l.first);
+        if (!ph || ph->name != l.first) {
+            ph.reset();
+            ph.reset(new Phase(ctx, l.first));
+        }
         l.second(ctx);
     }
 }
@@ -719,28 +784,41 @@ p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs
         t->widths.push_back(width);
         t->flat_segs_host.resize(segs.size() * sizeof(RowSeg<T>));
         std::memcpy(t->flat_segs_host.data(), segs.data(), t->flat_segs_host.size());
-        DevPtr dsegs = upload_segs<H>(ctx, segs);
-        t->seg_tables.push_back(dsegs);
+        bool f64 = false;
+        if constexpr (H::FIELD == P3H_BABYBEAR) f64 = ctx->hash_engine == 1;
+        // the device segment table is uploaded only for a kernel that reads it
+        // (openings upload their own from flat_segs_host)
+        DevPtr dsegs;
+        bool need_table = true;
+        if constexpr (H::FIELD == P3H_BABYBEAR)
+            if (f64) need_table = hash_rows_f64_reads_table(uint32_t(segs.size()), segs[0], height);
+        if (need_table) {
+            dsegs = upload_segs<H>(ctx, segs);
+            t->seg_tables.push_back(dsegs);
+        }
+        const RowSeg<T>* dsegs_ptr = need_table ? dptr<RowSeg<T>>(dsegs) : nullptr;
         DevPtr leaves = dalloc(ctx, height * D::DIG * sizeof(T));
         {
             Phase ph(ctx, "leaf_hash");
             bool done = false;
             if constexpr (H::FIELD == P3H_BABYBEAR) {
-                if (ctx->hash_engine == 1) {
-                    launch_check(hash_rows_f64_device(dptr<RowSeg<T>>(dsegs), uint32_t(segs.size()), width, segs[0],
-                                                      height, dptr<T>(leaves), ctx->bbp64, ctx->stream),
+                if (f64) {
+                    launch_check(hash_rows_f64_device(dsegs_ptr, uint32_t(segs.size()), width, segs[0], height,
+                                                      dptr<T>(leaves), ctx->bbp64, ctx->stream),
                                  "leaf hash");
                     done = true;
                 }
             }
             if (!done)
-                launch_check(hash_rows_device<D>(dptr<RowSeg<T>>(dsegs), uint32_t(segs.size()), width, segs[0], height,
+                launch_check(hash_rows_device<D>(dsegs_ptr, uint32_t(segs.size()), width, segs[0], height,
                                                  dptr<T>(leaves), pp, ctx->stream),
                              "leaf hash");
         }
         t->layers.push_back(leaves);
         t->counts.push_back(height);
         uint64_t cnt = height;
+        std::unique_ptr<Phase> cph;  // one timing phase for all levels
+        if (cnt > 1) cph.reset(new Phase(ctx, "compress"));
         while (cnt > 1) {
             if (H::FIELD == P3H_BABYBEAR && ctx->hash_engine == 1 && cnt <= 128) {
                 // the last levels in one launch
@@ -755,7 +833,6 @@ p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs
                     t->counts.push_back(c);
                     if (c == 1) break;
                 }
-                Phase ph(ctx, "compress");
                 launch_check(compress_tail_f64_device(prev, cnt, outs.data(), counts.data(), uint32_t(outs.size()),
                                                       ctx->bbp64, ctx->stream),
                              "compress");
@@ -763,7 +840,6 @@ p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs
             }
             uint64_t nxt = (cnt + 1) / 2;
             DevPtr nl = dalloc(ctx, nxt * D::DIG * sizeof(T));
-            Phase ph(ctx, "compress");
             bool done = false;
             if constexpr (H::FIELD == P3H_BABYBEAR) {
                 if (ctx->hash_engine == 1) {
@@ -786,6 +862,62 @@ p3h_tree* flat_tree(p3h_ctx* ctx, const std::vector<RowSeg<typename H::T>>& segs
         throw;
     }
     return t;
+}
+
+// Flat trees over several natural-order BabyBear matrices (widths multiples
+// of 8), built together with the binary64 engine: one leaf-hash launch for
+// all matrices, then one launch per level for every tree that still has that
+// level.  Each tree is the one flat_tree builds on its matrix alone.
+std::vector<p3h_tree*> flat_trees_multi(p3h_ctx* ctx, const std::vector<p3h_mat*>& mats) {
+    std::vector<p3h_tree*> out;
+    try {
+        for (p3h_mat* m : mats) {
+            if (m->field != P3H_BABYBEAR || m->k != 0 || m->height == 0 || m->width == 0 || m->width % 8)
+                throw Err{P3H_EINVAL, "batched commit needs natural-order BabyBear matrices of width 8k"};
+            auto* t = new p3h_tree{};
+            out.push_back(t);
+            t->ctx = ctx;
+            t->field = P3H_BABYBEAR;
+            t->flat = true;
+            t->total_width = m->width;
+            t->mats.push_back(m);
+            mat_ref(m);
+            t->heights.push_back(m->height);
+            t->widths.push_back(m->width);
+            const std::vector<RowSeg<uint32_t>> segs{seg_of<HBB>(m)};
+            t->flat_segs_host.resize(sizeof(RowSeg<uint32_t>));  // openings upload their table from this
+            std::memcpy(t->flat_segs_host.data(), segs.data(), sizeof(RowSeg<uint32_t>));
+            for (uint64_t c = m->height;; c = (c + 1) / 2) {
+                t->layers.push_back(dalloc(ctx, c * 8 * sizeof(uint32_t)));
+                t->counts.push_back(c);
+                if (c == 1) break;
+            }
+        }
+        std::vector<MultiJob> jobs;
+        for (size_t i = 0; i < mats.size(); ++i)
+            jobs.push_back(MultiJob{mats[i]->data<uint32_t>(), dptr<uint32_t>(out[i]->layers[0]), 0, mats[i]->height,
+                                    mats[i]->width});
+        {
+            Phase ph(ctx, "leaf_hash");
+            launch_check(hash_rows_multi_f64_device(jobs.data(), uint32_t(jobs.size()), ctx->bbp64, ctx->stream),
+                         "leaf hash");
+        }
+        Phase ph(ctx, "compress");
+        for (size_t lev = 1;; ++lev) {
+            jobs.clear();
+            for (p3h_tree* t : out)
+                if (lev < t->layers.size())
+                    jobs.push_back(MultiJob{dptr<uint32_t>(t->layers[lev - 1]), dptr<uint32_t>(t->layers[lev]),
+                                            t->counts[lev - 1], t->counts[lev], 0});
+            if (jobs.empty()) break;
+            launch_check(compress_multi_f64_device(jobs.data(), uint32_t(jobs.size()), ctx->bbp64, ctx->stream),
+                         "compress");
+        }
+    } catch (...) {
+        for (p3h_tree* t : out) p3h_tree_free(t);
+        throw;
+    }
+    return out;
 }
 
 template <class H>
@@ -1134,6 +1266,18 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
         f->layers.push_back(cur);
         uint32_t round = 0;
         size_t next_in = 0;
+        // With the challenges given, no layer depends on a root: every layer is
+        // folded first and all layer trees are then built together, level by
+        // level (binary64 engine).  With a transcript each root is observed
+        // before the next challenge, so each tree is built in its round.
+        const bool together = !ch && ctx->hash_engine == 1;
+        std::vector<p3h_mat*> views;
+        struct ViewRefs {
+            std::vector<p3h_mat*>& v;
+            ~ViewRefs() {
+                for (p3h_mat* m : v) mat_unref(m);
+            }
+        } view_refs{views};
         while (cur->height > (1ull << lfh)) {
             uint32_t lh = log2_exact(cur->height);
             const bool has_next = next_in < extra.size();
@@ -1142,19 +1286,23 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
             uint32_t la = std::min(lh - target, max_log_arity);
             uint32_t arity = 1u << la;
             p3h_mat* view = view_of(cur, cur->height >> la, 4 * arity);
-            p3h_tree* t = nullptr;
-            try {
-                t = flat_tree<HBB>(ctx, {seg_of<HBB>(view)}, view->height, view->width, {view});
-            } catch (...) {
-                mat_unref(view);
-                throw;
+            if (together) {
+                views.push_back(view);
+            } else {
+                p3h_tree* t = nullptr;
+                try {
+                    t = flat_tree<HBB>(ctx, {seg_of<HBB>(view)}, view->height, view->width, {view});
+                } catch (...) {
+                    mat_unref(view);
+                    throw;
+                }
+                mat_unref(view);  // the tree holds its own reference
+                f->trees.push_back(t);
             }
-            mat_unref(view);  // the tree holds its own reference
-            f->trees.push_back(t);
             const uint32_t* beta_dev = nullptr;
             if (ch) {
                 uint32_t* bd = dptr<uint32_t>(betas_dev) + 4 * round;
-                const uint32_t* root = dptr<uint32_t>(t->layers.back());
+                const uint32_t* root = dptr<uint32_t>(f->trees.back()->layers.back());
                 ChalProgram pr{};
                 pr.op[pr.n_ops++] = chal_op(CH_OBSERVE, 8, 0, root);  // observe_commitment
                 uint64_t w = 0;
@@ -1182,12 +1330,15 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
             arities.push_back(la);
             ++round;
         }
-        for (uint32_t r = 0; r < round; ++r) d2h(ctx, roots_out + 8 * r, f->trees[r]->layers.back()->ptr, 32);
-        if (ch) d2h(ctx, betas_out, dptr<uint32_t>(betas_dev), 4 * round * sizeof(uint32_t));
+        if (together) f->trees = flat_trees_multi(ctx, views);
+        std::vector<Readback> rb;
+        for (uint32_t r = 0; r < round; ++r) rb.push_back({roots_out + 8 * r, f->trees[r]->layers.back()->ptr, 32});
+        if (ch) rb.push_back({betas_out, dptr<uint32_t>(betas_dev), 4 * round * sizeof(uint32_t)});
         // final polynomial: truncate to 2^log_final_poly_len, bit-reverse, inverse DFT (host, tiny)
         uint32_t n = 1u << log_final_poly_len;
         std::vector<uint32_t> last(uint64_t(cur->height) * 4);
-        d2h(ctx, last.data(), cur->data<uint32_t>(), last.size() * 4);
+        rb.push_back({last.data(), cur->data<uint32_t>(), last.size() * 4});
+        d2h_gather(ctx, rb);
         std::vector<uint32_t> br(4 * n);
         for (uint32_t i = 0; i < n; ++i) {
             uint32_t j = 0;
@@ -1207,7 +1358,7 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
             std::vector<uint32_t> obs(final_poly_out, final_poly_out + 4 * n);
             obs.insert(obs.end(), arities.begin(), arities.end());
             DevPtr d = dalloc(ctx, obs.size() * sizeof(uint32_t));
-            h2d(ctx, d->ptr, obs.data(), obs.size() * sizeof(uint32_t));
+            h2d_async(ctx, d->ptr, obs.data(), obs.size() * sizeof(uint32_t));
             ChalProgram pr{};
             pr.op[0] = chal_op(CH_OBSERVE, uint32_t(obs.size()), 0, dptr<uint32_t>(d));
             pr.n_ops = 1;
@@ -1262,6 +1413,7 @@ int p3h_ctx_destroy(p3h_ctx* ctx) {
     ctx->row_scales.clear();
     ctx->fri_tinv.clear();
     trim(ctx);
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return P3H_OK;

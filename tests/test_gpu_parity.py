@@ -200,6 +200,39 @@ def test_fri_commit_vs_oracle(ctx, golden, orc):
     fri.free()
 
 
+def test_fri_commit_layers_built_together(ctx, golden, orc):
+    """With the challenges given, every layer is folded first and all layer
+    trees are built together (one leaf-hash launch, then one launch per level).
+    At 2^17 the leaf launch and the wide levels take the thread-per-state
+    kernels, the narrow levels the lane-parallel ones; arity 8 gives 32-wide
+    leaf rows.  Roots against the oracle, and every layer's openings against
+    the per-round trees of the integer engine."""
+    cw = rnd(21, ((1 << 17), 4))
+    betas = rnd(22, (20, 4))
+    idx = np.array([0, 1, 2, 3, 5, 8, 13, 21], dtype=np.uint64)
+    for mla in (1, 3):
+        roots, final, fri = ctx.fri_commit_phase(ctx.upload(cw), 1, 3, mla, betas, keep=True)
+        want_roots, want_final = orc.fri_commit(golden["consts_bb"], RF, 13, cw, 1, 3, mla, betas)
+        assert len(roots) == len(want_roots)
+        assert (roots == want_roots).all() and (final == want_final).all(), mla
+        opened = []
+        for r in range(len(roots)):
+            t = fri.layer_tree(r)
+            opened.append(t.open(idx % np.uint64(t.n_leaves)))
+        fri.free()
+        ctx.set_hash_engine(0)
+        try:
+            roots0, _, fri0 = ctx.fri_commit_phase(ctx.upload(cw), 1, 3, mla, betas, keep=True)
+            assert (roots0 == roots).all()
+            for r in range(len(roots)):
+                t = fri0.layer_tree(r)
+                vals, proof = t.open(idx % np.uint64(t.n_leaves))
+                assert (vals == opened[r][0]).all() and (proof == opened[r][1]).all(), (mla, r)
+            fri0.free()
+        finally:
+            ctx.set_hash_engine(1)
+
+
 # ------------------------------------------------------------------- PCS --
 def test_pcs_commit_golden(ctx, golden):
     for case in range(3):
