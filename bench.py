@@ -33,10 +33,11 @@ GL_P = 0xFFFFFFFF00000001
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # vector and matrix FP64 are the same figure on MI355X
 # The BabyBear leaf hash runs the exact binary64 Poseidon2 engine: one width-16
-# permutation (8 full + 13 partial rounds) is 2362 add + 564 mul + 2335 fma
-# binary64 operations = 7596 flops in 5261 FP64 VALU instructions (counted from
-# poseidon2_f64.h); each digest lane's final canonical reduction adds 5 flops.
-P2_F64_FLOPS_PER_PERM = 7596
+# permutation (8 full + 13 partial rounds, 132 of the 208 partial-round lane
+# reductions kept) is 2142 add + 564 mul + 2255 fma binary64 operations =
+# 7216 flops in 4961 FP64 VALU instructions (counted from poseidon2_f64.h);
+# each digest lane's final canonical reduction adds 5 flops.
+P2_F64_FLOPS_PER_PERM = 7216
 P2_F64_FLOPS_PER_DIGEST = 8 * 5
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 

@@ -22,8 +22,9 @@
 //   S-box output                             <  2^31
 //   MDS-light output (coefficient sum 35)    <  35 * 2^31 < 2^37.2
 //   S-box input (MDS output + constant)      <  2^37.3, square < 2^74.6 < 2^81
-//   internal layer: lanes enter <= 2^37.2, 2^15 * lane + sum < 2^52.3, then every
-//   lane is reduced (|r| < 2^31) before the next round
+//   internal layer: lanes enter <= 2^37.2, 2^15 * lane + sum < 2^52.3; which
+//   lanes are reduced after each partial round is fixed per engine below, with
+//   its own bound argument
 //   end of the permutation: every lane reduced, so a sponge's carried lanes
 //   start the next permutation below 2^31.
 #pragma once
@@ -109,12 +110,48 @@ struct Poseidon2F64 {
 #pragma unroll
         for (int i = 1; i < 16; ++i) s[i] = BBF64::reduce(__builtin_fma(s[i], double(1u << i), sum));
     }
+    // Partial rounds with deferred reductions, for the 13 partial rounds of the
+    // BabyBear width-16 permutation: after round r only the lanes whose bit is
+    // set in LAZY13[r] are reduced; the others carry exact unreduced integers.
+    // Worst-case magnitudes (log2), with S-box outputs < 2^31, reduced values
+    // <= p/2 + 1 < 2^30 and round constants < 2^31: lanes enter at < 35 * 2^31
+    // (MDS-light output); every round keeps lane 0 + constant < 2^40.9 (so the
+    // S-box square stays below 2^81.8, inside reduce's range) and every lane
+    // and the row sum below 2^53 (exact); after round 12 every lane is reduced,
+    // so the terminal full rounds see inputs < 2^31.  The masks are the fewest
+    // reductions (132 of 208) a search over these bounds found.
+    static constexpr uint16_t LAZY13[13] = {0xff80, 0xfc71, 0xff81, 0xffff, 0xfc00, 0xf3c0, 0xfc01,
+                                            0xfff1, 0xffff, 0xfc00, 0xf3c0, 0xfc01, 0xffff};
+    template <uint32_t MASK>
+    __device__ __forceinline__ static void internal_layer_lazy(double* s) {
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sum = sum + s[i];
+        s[0] = (MASK & 1u) ? BBF64::reduce(s[0] + sum) : s[0] + sum;
+#pragma unroll
+        for (int i = 1; i < 16; ++i) {
+            const double v = __builtin_fma(s[i], double(1u << i), sum);
+            s[i] = ((MASK >> i) & 1u) ? BBF64::reduce(v) : v;
+        }
+    }
+    template <int R>
+    __device__ __forceinline__ static void partial_rounds13(double* s, const P2ParamsF64& p) {
+        if constexpr (R < 13) {
+            s[0] = sbox(s[0] + p.internal[R]);
+            internal_layer_lazy<LAZY13[R]>(s);
+            partial_rounds13<R + 1>(s, p);
+        }
+    }
     __device__ __forceinline__ static void permute(double* s, const P2ParamsF64& p) {
         mds_light(s);
         for (uint32_t r = 0; r < p.half_full; ++r) full_round(s, p.init[r]);
-        for (uint32_t r = 0; r < p.partial; ++r) {
-            s[0] = sbox(s[0] + p.internal[r]);
-            internal_layer(s);
+        if (p.partial == 13) {
+            partial_rounds13<0>(s, p);
+        } else {
+            for (uint32_t r = 0; r < p.partial; ++r) {
+                s[0] = sbox(s[0] + p.internal[r]);
+                internal_layer(s);
+            }
         }
         for (uint32_t r = 0; r < p.half_full; ++r) full_round(s, p.term[r]);
 #pragma unroll
@@ -127,9 +164,10 @@ struct Poseidon2F64 {
 // about 900 dependent instructions instead of 5261, which is what the narrow
 // levels of a Merkle tree are bound by.  Cross-lane reads use DPP inside a
 // row: quad_perm for the 4x4 circulant, row_ror for sums across the row.  The
-// arithmetic is the exact integer arithmetic of Poseidon2F64 (same values,
-// same bounds; only the order of exact additions differs), so the canonical
-// output is identical.  Every lane of the row must execute it.
+// arithmetic is exact integer arithmetic on values congruent to those of
+// Poseidon2F64 (the order of exact additions and which lanes are reduced when
+// differ), so the canonical output is identical.  Every lane of the row must
+// execute it.
 struct Poseidon2F64Lanes {
     template <int CTRL>
     __device__ __forceinline__ static double dpp(double v) {
