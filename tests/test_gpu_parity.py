@@ -92,6 +92,51 @@ def test_lde_vs_oracle(ctx, orc, f, lh, w, k):
     assert (got == orc.lde(f, m, k, shift)).all()
 
 
+@pytest.fixture
+def shape_ctx(ctx):
+    yield ctx
+    ctx.set_ntt_tuning(0, 0, 0)
+
+
+@pytest.mark.parametrize("max_stages", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+def test_bb_lde_every_pass_width(shape_ctx, orc, max_stages):
+    """Every pass width the planner can choose (1..10 stages), one and two
+    outer pass levels, extreme values: an all p-1 and an all-zero column
+    beside random ones."""
+    ctx = shape_ctx
+    ctx.set_ntt_tuning(0, max_stages, 0)
+    for L, w, k in ((12, 6, 1), (14, 3, 2)):
+        m = rnd(L * 31 + max_stages, (1 << L, w))
+        m[:, 0] = BB_P - 1
+        m[:, 1] = 0
+        assert (ctx.coset_lde(ctx.upload(m), k, 31).download() == orc.lde("bb", m, k, 31)).all(), (L, w, k)
+
+
+@pytest.mark.parametrize("log_cols", [1, 3, 5, 7])
+def test_bb_lde_column_tiles(shape_ctx, orc, log_cols):
+    """Column tiles narrower and wider than the matrix, a ragged last tile."""
+    ctx = shape_ctx
+    ctx.set_ntt_tuning(log_cols, 0, 0)
+    m = rnd(log_cols, (1 << 13, 45))
+    assert (ctx.coset_lde(ctx.upload(m), 1, 31).download() == orc.lde("bb", m, 1, 31)).all()
+
+
+def test_bb_lde_pass_shapes_agree_at_scale(shape_ctx):
+    """2^20 x 64 with 2 added bits under three pass shapes (5-, 3- and
+    2-pass plans): identical Merkle roots of the 2^22 x 64 LDEs."""
+    ctx = shape_ctx
+    m = ctx.upload(rnd(4242, (1 << 20, 64)))
+    roots = []
+    for tune in ((0, 0, 0), (4, 7, 0), (4, 10, 0)):
+        ctx.set_ntt_tuning(*tune)
+        lde = ctx.coset_lde(m, 2, 31)
+        t = ctx.mmcs_commit([lde], 0)
+        roots.append(t.root())
+        t.free()
+        lde.free()
+    assert (roots[0] == roots[1]).all() and (roots[0] == roots[2]).all()
+
+
 # ------------------------------------------------------------------ MMCS --
 def test_flat_commit_golden(ctx, golden):
     mmcs = p3hip.FlatMerkleTreeMmcsHip(ctx)
