@@ -76,12 +76,28 @@ struct Goldilocks {
     __device__ __forceinline__ static uint64_t reduce128(uint64_t hi, uint64_t lo) {
         return canon(reduce128_lazy(hi, lo));
     }
+    // the 128-bit product from four 32 x 32 -> 64 partial products (each one
+    // v_mad_u64_u32 with the running carry as its addend):
+    // a b = a1 b1 2^64 + (a0 b1 + a1 b0) 2^32 + a0 b0; m0 and m1 stay below
+    // (2^32 - 1)^2 + 2^32 - 1 < 2^64.
+    __device__ __forceinline__ static void mul128(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
+        const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+        const uint64_t p00 = uint64_t(a0) * b0;
+        const uint64_t m0 = uint64_t(a0) * b1 + (p00 >> 32);
+        const uint64_t m1 = uint64_t(a1) * b0 + uint32_t(m0);
+        hi = uint64_t(a1) * b1 + (m0 >> 32) + (m1 >> 32);
+        lo = (m1 << 32) | uint32_t(p00);
+    }
     __device__ __forceinline__ static uint64_t mul(uint64_t a, uint64_t b) {
-        return reduce128(__umul64hi(a, b), a * b);
+        uint64_t hi, lo;
+        mul128(a, b, hi, lo);
+        return reduce128(hi, lo);
     }
     // any a, b < 2^64 -> a*b mod p as a value below 2^64 (not necessarily canonical)
     __device__ __forceinline__ static uint64_t mul_lazy(uint64_t a, uint64_t b) {
-        return reduce128_lazy(__umul64hi(a, b), a * b);
+        uint64_t hi, lo;
+        mul128(a, b, hi, lo);
+        return reduce128_lazy(hi, lo);
     }
     // x * 2^i mod p, canonical, for 1 <= i < 32: the product's high word is
     // below 2^i, so only the hi * (2^32 - 1) term of the reduction remains.

@@ -11,6 +11,8 @@
 // internal layer s_i <- 2^i * s_i + sum(s), rf/2 full rounds; S-box x^7.
 #pragma once
 
+#include <type_traits>
+
 #include "field.h"
 
 namespace p3h {
@@ -55,6 +57,126 @@ struct GLDom {
     __device__ __forceinline__ static T mul_diag(T s, uint32_t i, T) { return F::mul_pow2(s, i); }
     __device__ __forceinline__ static T to_dom(T x) { return x; }
     __device__ __forceinline__ static T from_dom(T x) { return x; }
+};
+
+// Goldilocks width-8 permutation with exact-integer linear layers.
+//
+// State words are any 64-bit values congruent to the field elements; only the
+// permutation's output is canonical.
+//   S-box: four Goldilocks::mul_lazy products, which accept any 64-bit inputs.
+//   Round constant: one 64-bit add; a carry adds 2^64 mod p = 2^32 - 1, and a
+//     second carry cannot occur because the constant is canonical (at most
+//     2^64 - 2^32), so the wrapped sum is at most 2^64 - 2^32 - 1.
+//   Linear layers (MDS-light, internal layer): every output is a combination
+//     of the inputs with small non-negative integer coefficients.  The low and
+//     high 32-bit halves are combined separately in 64-bit registers, where
+//     each term is one 64-bit add or shift-add with no carry to handle
+//     (MDS-light: coefficient sum 21, so below 21 * 2^32 < 2^37; internal
+//     layer: below 2^7 * 2^32 + 8 * 2^32 < 2^40), and each output is folded
+//     once: H * 2^32 + L (mod p).
+// This replaces a canonical modular add (compare, select, canonical
+// subtraction) per term of the linear layers.
+struct Poseidon2GL {
+    static constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
+
+    // H * 2^32 + L (mod p) as a value below 2^64, for L, H < 2^56:
+    // H 2^32 + L = h_hi 2^64 + (h_lo + l_hi) 2^32 + l_lo = k 2^64 + T with the
+    // carry of h_lo + l_hi moved into k < 2^24 + 1; 2^64 == 2^32 - 1, so the
+    // value is T + k (2^32 - 1), whose wrap-around (at most once: the wrapped
+    // sum is below k 2^32 < 2^57) adds 2^32 - 1 again.
+    __device__ __forceinline__ static uint64_t fold(uint64_t L, uint64_t H) {
+        const uint32_t l_lo = uint32_t(L), l_hi = uint32_t(L >> 32);
+        const uint32_t h_lo = uint32_t(H), h_hi = uint32_t(H >> 32);
+        const uint32_t t_hi = h_lo + l_hi;
+        const uint32_t k = h_hi + (t_hi < h_lo ? 1u : 0u);
+        const uint64_t T = (uint64_t(t_hi) << 32) | l_lo;
+        const uint64_t m = (uint64_t(k) << 32) - k;
+        const uint64_t r = T + m;
+        return r < T ? r + EPS : r;
+    }
+    __device__ __forceinline__ static uint64_t add_rc(uint64_t x, uint64_t rc) {
+        const uint64_t t = x + rc;
+        return t < x ? t + EPS : t;
+    }
+    __device__ __forceinline__ static uint64_t sbox(uint64_t x) {
+        const uint64_t x2 = Goldilocks::mul_lazy(x, x);
+        const uint64_t x3 = Goldilocks::mul_lazy(x2, x);
+        const uint64_t x4 = Goldilocks::mul_lazy(x2, x2);
+        return Goldilocks::mul_lazy(x3, x4);
+    }
+    // [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] x over the integers
+    __device__ __forceinline__ static void mat4(uint64_t* x) {
+        const uint64_t t01 = x[0] + x[1];
+        const uint64_t t23 = x[2] + x[3];
+        const uint64_t t0123 = t01 + t23;
+        const uint64_t t01123 = t0123 + x[1];
+        const uint64_t t01233 = t0123 + x[3];
+        const uint64_t y3 = t01233 + (x[0] << 1);
+        const uint64_t y1 = t01123 + (x[2] << 1);
+        const uint64_t y0 = t01123 + t01;
+        const uint64_t y2 = t01233 + t23;
+        x[0] = y0;
+        x[1] = y1;
+        x[2] = y2;
+        x[3] = y3;
+    }
+    // mat4 on both halves of the state, then s_i + (s_i + s_{i+-4})
+    __device__ __forceinline__ static void mds_light(uint64_t* s) {
+        uint64_t lo[8], hi[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            lo[i] = s[i] & 0xFFFFFFFFull;
+            hi[i] = s[i] >> 32;
+        }
+        mat4(lo);
+        mat4(lo + 4);
+        mat4(hi);
+        mat4(hi + 4);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t sl = lo[k] + lo[k + 4], sh = hi[k] + hi[k + 4];
+            s[k] = fold(lo[k] + sl, hi[k] + sh);
+            s[k + 4] = fold(lo[k + 4] + sl, hi[k + 4] + sh);
+        }
+    }
+    // s_0 <- s_0 + sum(s), s_i <- 2^i s_i + sum(s)
+    __device__ __forceinline__ static void internal_layer(uint64_t* s) {
+        uint64_t lo[8], hi[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            lo[i] = s[i] & 0xFFFFFFFFull;
+            hi[i] = s[i] >> 32;
+        }
+        uint64_t sl = lo[0], sh = hi[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i) {
+            sl += lo[i];
+            sh += hi[i];
+        }
+        s[0] = fold(lo[0] + sl, hi[0] + sh);
+#pragma unroll
+        for (int i = 1; i < 8; ++i) s[i] = fold((lo[i] << i) + sl, (hi[i] << i) + sh);
+    }
+    template <class P>
+    __device__ __forceinline__ static void permute(uint64_t* s, const P& p) {
+        mds_light(s);
+        for (uint32_t r = 0; r < p.half_full; ++r) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s[i] = sbox(add_rc(s[i], p.init[r][i]));
+            mds_light(s);
+        }
+        for (uint32_t r = 0; r < p.partial; ++r) {
+            s[0] = sbox(add_rc(s[0], p.internal[r]));
+            internal_layer(s);
+        }
+        for (uint32_t r = 0; r < p.half_full; ++r) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s[i] = sbox(add_rc(s[i], p.term[r][i]));
+            mds_light(s);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s[i] = Goldilocks::canon(s[i]);
+    }
 };
 
 template <class D>
@@ -111,6 +233,10 @@ struct Poseidon2 {
         for (int i = 1; i < W; ++i) s[i] = D::add(D::mul_diag(s[i], uint32_t(i), diag[i]), sum);
     }
     __device__ __forceinline__ static void permute(T* s, const P2Params<T, W>& p) {
+        if constexpr (std::is_same<D, GLDom>::value) {
+            Poseidon2GL::permute(s, p);
+            return;
+        }
         mds_light(s);
         for (uint32_t r = 0; r < p.half_full; ++r) full_round(s, p.init[r]);
         for (uint32_t r = 0; r < p.partial; ++r) {
