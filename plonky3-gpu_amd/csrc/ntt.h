@@ -60,6 +60,7 @@ struct LdePassGroup {
     int mode;
     uint32_t s;
     uint32_t blocks;
+    uint32_t lds_bytes;  // dynamic LDS of the launch: the largest of its passes
     std::vector<NttWork<T>> work;
 };
 

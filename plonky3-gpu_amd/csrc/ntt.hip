@@ -37,20 +37,13 @@ namespace p3h {
 namespace {
 
 constexpr int NTT_THREADS = 512;
-constexpr int TILE_BYTES = 65536;
+constexpr int TILE_BYTES = 65536;  // largest exchange tile (the LDS is sized per pass)
 constexpr uint32_t MAX_STAGES = 10;
 
 template <class T>
 constexpr uint32_t tile_log() {
     return 31 - __builtin_clz(uint32_t(TILE_BYTES / sizeof(T)));
 }
-// passes span at most tile_log - 4 bits (>= 16 columns per tile), so a pass
-// needs at most 2^(tile_log - 4) - 1 twiddles
-template <class T>
-constexpr uint32_t tw_lds() {
-    return 1u << (tile_log<T>() - 4);
-}
-
 enum Mode : int {
     OUTER_DIF_INV = 0,     // DIF stages, bits descending
     OUTER_DIT_FWD = 1,     // DIT stages, bits ascending, per region
@@ -224,24 +217,27 @@ __device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC
     }
 }
 
+// Dynamic LDS of a pass (pass_lds_bytes on the host): the column-major
+// exchange tile with one padding word per column, (2^S + 1) << logC words,
+// when the pass has two rounds; then the gathered twiddles, 2^S - 1 words
+// (twice for the fused inner pass).  Sized to the tile actually used, so
+// narrow tiles leave room for more resident workgroups per CU.
 template <class F, int MODE, int S>
-__device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, uint32_t block) {
+__device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, uint32_t block, typename F::T* smem) {
     using T = typename F::T;
     constexpr uint32_t STRIDE = Tile<S>::STRIDE;
-    constexpr uint32_t TILE = TILE_BYTES / sizeof(T);
-    constexpr uint32_t TW_LDS = tw_lds<T>();
     // the host never plans a pass wider than tile_log - 4 bits for this field
     if constexpr (uint32_t(S) > tile_log<T>() - 4) {
         return;
     } else {
-    // column-major exchange tile with one padding word per column; C <= TILE / 16
-    __shared__ T lds[TILE + TILE / 16];
-    __shared__ T tws[(MODE == INNER_LDE ? 2 : 1) * TW_LDS];
     // rounds: low = bits [0, R0), high = bits [R0, S)
     constexpr int R0 = S <= 5 ? S : (S + 1) / 2;
     constexpr int R1 = S - R0;
 
     const uint32_t a = A.a, logC = A.logC;
+    T* lds = smem;
+    T* tws = smem + (R1 > 0 ? (STRIDE << logC) : 0u);
+    T* twf = tws + ((1u << S) - 1);  // forward twiddles of the fused inner pass
 
     // block -> (column tile, low index, high index, region)
     uint32_t bid = block;
@@ -263,7 +259,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
 
     if constexpr (S > 0) {
         build_twiddles(tws, A.tw, A.L, a, S, lo0);
-        if (MODE == INNER_LDE) build_twiddles(tws + TW_LDS, A.tw + (n >> 1), A.L, a, S, lo0);
+        if (MODE == INNER_LDE) build_twiddles(twf, A.tw + (n >> 1), A.L, a, S, lo0);
         __syncthreads();
     }
 
@@ -293,7 +289,6 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
             else gio.template load<R0, 0>(g.c, g.tb, x);
             butterflies<F, R0, 0, false>(x, tws);
         }
-        const T* twf = tws + TW_LDS;
         for (int r = int(A.n_regions) - 1; r >= 0; --r) {
             const T* twist = A.scale + uint64_t(r) * n + row0 + g.tb;
             T* dst_r = A.dst + uint64_t(r) * A.dst_region_rows * ld + tile_off;
@@ -385,10 +380,11 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t g, uint32_t grid) {
 // grid holds (a dispatch counts at most 2^32 - 1 work-items).
 template <class F, int MODE, int S>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A, uint32_t total) {
+    extern __shared__ __align__(16) unsigned char ntt_smem[];
     const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x);
     for (uint32_t b = b0; b < total; b += gridDim.x) {
         if (b != b0) __syncthreads();  // the previous block is done with the LDS tile
-        ntt_pass_body<F, MODE, S>(A, b);
+        ntt_pass_body<F, MODE, S>(A, b, reinterpret_cast<typename F::T*>(ntt_smem));
     }
 }
 
@@ -398,6 +394,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A
 template <class F, int MODE, int S>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count,
                                                              uint32_t total) {
+    extern __shared__ __align__(16) unsigned char ntt_smem[];
     const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x);
     for (uint32_t b = b0; b < total; b += gridDim.x) {
         if (b != b0) __syncthreads();
@@ -409,7 +406,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_list(const NttWork<typen
             else hi = mid;
         }
         const NttArgs<typename F::T> A = work[lo].args;
-        ntt_pass_body<F, MODE, S>(A, b - work[lo].block_start);
+        ntt_pass_body<F, MODE, S>(A, b - work[lo].block_start, reinterpret_cast<typename F::T*>(ntt_smem));
     }
 }
 
@@ -519,54 +516,75 @@ uint64_t finalize_args(NttArgs<T>& A, int mode, uint32_t s, const NttTuning& tun
     return blocks;
 }
 
+// Dynamic LDS bytes of a pass of width s with 2^logC-column tiles (the layout
+// of ntt_pass_body).  At most 2^(tile_log) + 2^(tile_log - 4) tile words plus
+// 2 (2^10 - 1) twiddles: below 76 KiB for both fields.
+template <class T>
+uint32_t pass_lds_bytes(int mode, uint32_t s, uint32_t logC) {
+    const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;
+    const uint64_t tile = r0 < s ? (((1ull << s) + 1) << logC) : 0;
+    const uint64_t tws = s ? ((1ull << s) - 1) * (mode == INNER_LDE ? 2 : 1) : 0;
+    return uint32_t((tile + tws) * sizeof(T));
+}
+
 // grid cap: 2^22 workgroups x 512 work-items = 2^31 work-items per dispatch
 constexpr uint64_t MAX_GRID = 1ull << 22;
+constexpr int MAX_DYN_LDS = 160 * 1024;  // LDS of one gfx950 workgroup
 
 template <class F, int MODE, int S>
-void launch_s(const NttArgs<typename F::T>& A, uint64_t blocks, const NttWork<typename F::T>* work, uint32_t count,
-              hipStream_t st) {
+void launch_s(const NttArgs<typename F::T>& A, uint64_t blocks, uint32_t lds, const NttWork<typename F::T>* work,
+              uint32_t count, hipStream_t st) {
+    // allow dynamic LDS above the 64 KiB default, once per kernel
+    static const bool attr = [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&ntt_pass<F, MODE, S>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, MAX_DYN_LDS);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&ntt_pass_list<F, MODE, S>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, MAX_DYN_LDS);
+        return true;
+    }();
+    (void)attr;
     const uint32_t grid = uint32_t(blocks < MAX_GRID ? blocks : MAX_GRID);
     if (work)
-        hipLaunchKernelGGL((ntt_pass_list<F, MODE, S>), dim3(grid), dim3(NTT_THREADS), 0, st, work, count,
+        hipLaunchKernelGGL((ntt_pass_list<F, MODE, S>), dim3(grid), dim3(NTT_THREADS), lds, st, work, count,
                            uint32_t(blocks));
     else
-        hipLaunchKernelGGL((ntt_pass<F, MODE, S>), dim3(grid), dim3(NTT_THREADS), 0, st, A, uint32_t(blocks));
+        hipLaunchKernelGGL((ntt_pass<F, MODE, S>), dim3(grid), dim3(NTT_THREADS), lds, st, A, uint32_t(blocks));
 }
 
 template <class F, int MODE>
-void launch_mode(uint32_t s, const NttArgs<typename F::T>& A, uint64_t blocks, const NttWork<typename F::T>* work,
-                 uint32_t count, hipStream_t st) {
+void launch_mode(uint32_t s, const NttArgs<typename F::T>& A, uint64_t blocks, uint32_t lds,
+                 const NttWork<typename F::T>* work, uint32_t count, hipStream_t st) {
     switch (s) {
-        case 0: launch_s<F, MODE, 0>(A, blocks, work, count, st); break;
-        case 1: launch_s<F, MODE, 1>(A, blocks, work, count, st); break;
-        case 2: launch_s<F, MODE, 2>(A, blocks, work, count, st); break;
-        case 3: launch_s<F, MODE, 3>(A, blocks, work, count, st); break;
-        case 4: launch_s<F, MODE, 4>(A, blocks, work, count, st); break;
-        case 5: launch_s<F, MODE, 5>(A, blocks, work, count, st); break;
-        case 6: launch_s<F, MODE, 6>(A, blocks, work, count, st); break;
-        case 7: launch_s<F, MODE, 7>(A, blocks, work, count, st); break;
-        case 8: launch_s<F, MODE, 8>(A, blocks, work, count, st); break;
-        case 9: launch_s<F, MODE, 9>(A, blocks, work, count, st); break;
-        default: launch_s<F, MODE, 10>(A, blocks, work, count, st); break;
+        case 0: launch_s<F, MODE, 0>(A, blocks, lds, work, count, st); break;
+        case 1: launch_s<F, MODE, 1>(A, blocks, lds, work, count, st); break;
+        case 2: launch_s<F, MODE, 2>(A, blocks, lds, work, count, st); break;
+        case 3: launch_s<F, MODE, 3>(A, blocks, lds, work, count, st); break;
+        case 4: launch_s<F, MODE, 4>(A, blocks, lds, work, count, st); break;
+        case 5: launch_s<F, MODE, 5>(A, blocks, lds, work, count, st); break;
+        case 6: launch_s<F, MODE, 6>(A, blocks, lds, work, count, st); break;
+        case 7: launch_s<F, MODE, 7>(A, blocks, lds, work, count, st); break;
+        case 8: launch_s<F, MODE, 8>(A, blocks, lds, work, count, st); break;
+        case 9: launch_s<F, MODE, 9>(A, blocks, lds, work, count, st); break;
+        default: launch_s<F, MODE, 10>(A, blocks, lds, work, count, st); break;
     }
 }
 
 template <class F>
-void launch_any(int mode, uint32_t s, const NttArgs<typename F::T>& A, uint64_t blocks,
+void launch_any(int mode, uint32_t s, const NttArgs<typename F::T>& A, uint64_t blocks, uint32_t lds,
                 const NttWork<typename F::T>* work, uint32_t count, hipStream_t st) {
     switch (mode) {
-        case OUTER_DIF_INV: launch_mode<F, OUTER_DIF_INV>(s, A, blocks, work, count, st); break;
-        case OUTER_DIT_FWD: launch_mode<F, OUTER_DIT_FWD>(s, A, blocks, work, count, st); break;
-        case INNER_LDE: launch_mode<F, INNER_LDE>(s, A, blocks, work, count, st); break;
-        case INNER_DIT_GATHER: launch_mode<F, INNER_DIT_GATHER>(s, A, blocks, work, count, st); break;
-        default: launch_mode<F, INNER_DIF_SCATTER>(s, A, blocks, work, count, st); break;
+        case OUTER_DIF_INV: launch_mode<F, OUTER_DIF_INV>(s, A, blocks, lds, work, count, st); break;
+        case OUTER_DIT_FWD: launch_mode<F, OUTER_DIT_FWD>(s, A, blocks, lds, work, count, st); break;
+        case INNER_LDE: launch_mode<F, INNER_LDE>(s, A, blocks, lds, work, count, st); break;
+        case INNER_DIT_GATHER: launch_mode<F, INNER_DIT_GATHER>(s, A, blocks, lds, work, count, st); break;
+        default: launch_mode<F, INNER_DIF_SCATTER>(s, A, blocks, lds, work, count, st); break;
     }
 }
 
 template <class F, int MODE>
 void launch_pass(NttArgs<typename F::T> A, uint32_t s, const NttTuning& tune, hipStream_t st) {
     uint64_t blocks = finalize_args(A, MODE, s, tune);
-    launch_mode<F, MODE>(s, A, blocks, nullptr, 0, st);
+    launch_mode<F, MODE>(s, A, blocks, pass_lds_bytes<typename F::T>(MODE, s, A.logC), nullptr, 0, st);
 }
 
 // All passes of one LDE over columns [0, w) of row-stride-ld matrices.
@@ -771,11 +789,14 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
             g.mode = slot < 0 ? OUTER_DIF_INV : (slot == 0 ? INNER_LDE : OUTER_DIT_FWD);
             g.s = s;
             g.blocks = 0;
+            g.lds_bytes = 0;
             for (const auto& ps : passes) {
                 if (ps.slot != slot || ps.s != s) continue;
                 NttWork<T> wk{};
                 wk.args = ps.A;
                 uint64_t b = finalize_args(wk.args, ps.mode, ps.s, ps.tune);
+                const uint32_t lds = pass_lds_bytes<T>(ps.mode, ps.s, wk.args.logC);
+                g.lds_bytes = lds > g.lds_bytes ? lds : g.lds_bytes;
                 wk.block_start = g.blocks;
                 g.blocks += uint32_t(b);
                 g.work.push_back(wk);
@@ -788,7 +809,7 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
 template <class F>
 int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typename F::T>* d_work, hipStream_t st) {
     if (g.blocks == 0) return 0;
-    launch_any<F>(g.mode, g.s, g.work[0].args, g.blocks, d_work, uint32_t(g.work.size()), st);
+    launch_any<F>(g.mode, g.s, g.work[0].args, g.blocks, g.lds_bytes, d_work, uint32_t(g.work.size()), st);
     return int(hipGetLastError());
 }
 
