@@ -37,7 +37,7 @@ namespace p3h {
 namespace {
 
 constexpr int NTT_THREADS = 512;
-constexpr int TILE_BYTES = 65536;  // largest exchange tile (the LDS is sized per pass)
+constexpr int TILE_BYTES = 131072;  // largest exchange tile (the LDS is sized per pass)
 constexpr uint32_t MAX_STAGES = 10;
 
 template <class T>
@@ -504,6 +504,14 @@ uint32_t cols_log(uint32_t s, uint32_t w, const NttTuning& tune) {
 template <class T>
 uint64_t finalize_args(NttArgs<T>& A, int mode, uint32_t s, const NttTuning& tune) {
     A.logC = cols_log<T>(s, A.w, tune);
+    if (s <= 5) {
+        // a single-round pass keeps no LDS tile and gives each thread one
+        // column's group: 2^logC threads work, so take up to one column per
+        // thread (the matrix width rounded up to a power of two, at most 512)
+        uint32_t need = 0;
+        while ((1u << need) < A.w) ++need;
+        A.logC = need < 9 ? need : 9;  // log2(NTT_THREADS) = 9
+    }
     if (mode == INNER_LDE) {
         // one low-round group per thread: (2^s >> R0) * C <= NTT_THREADS
         const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;
@@ -518,7 +526,8 @@ uint64_t finalize_args(NttArgs<T>& A, int mode, uint32_t s, const NttTuning& tun
 
 // Dynamic LDS bytes of a pass of width s with 2^logC-column tiles (the layout
 // of ntt_pass_body).  At most 2^(tile_log) + 2^(tile_log - 4) tile words plus
-// 2 (2^10 - 1) twiddles: below 76 KiB for both fields.
+// 2 (2^10 - 1) twiddles: below 152 KiB for both fields, inside the 160 KiB of
+// one workgroup; tiles above 80 KiB leave one workgroup per CU.
 template <class T>
 uint32_t pass_lds_bytes(int mode, uint32_t s, uint32_t logC) {
     const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;

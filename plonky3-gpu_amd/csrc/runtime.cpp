@@ -2194,8 +2194,9 @@ int p3h_autotune_lde(p3h_ctx* ctx, int field, uint32_t log_n, uint32_t width, ui
         p3h_mat* in = new_mat(ctx, field, h, width);
         std::unique_ptr<p3h_mat, void (*)(p3h_mat*)> guard(in, mat_unref);
         hip_check(hipMemsetAsync(in->buf->ptr, 0, h * width * elem_size(field), ctx->stream), "memset");
-        const uint32_t tile_log = field == P3H_BABYBEAR ? 14 : 13;
-        const uint32_t smax = tile_log - 4;
+        // passes of at most 10 stages (the tile holds 2^15 BabyBear or 2^14
+        // Goldilocks words, at least 16 columns at 10 stages)
+        const uint32_t smax = 10;
         std::vector<NttTuning> cands;
         for (uint32_t sl : {0u, 4u, 5u, 6u}) {
             if (sl && (1u << sl) >= width) continue;
