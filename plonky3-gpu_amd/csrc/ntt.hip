@@ -83,7 +83,10 @@ struct Tile {
 
 // Butterflies of one radix-2^R round on a register group x[0..2^R) holding
 // tile positions tb + m * 2^Q0; twb = tws + (tb mod 2^Q0).
-template <class F, int R, int Q0, bool DIT>
+// A0: the pass starts at index bit 0 (a = 0, so lo0 = 0) and this is its low
+// round (Q0 = 0).  Then tws[0] = omega^0 and tws[1] = omega^0: the stage of
+// bit 0 and the even-m butterflies of bit 1 multiply by 1, and skip it.
+template <class F, int R, int Q0, bool DIT, bool A0 = false>
 __device__ __forceinline__ void butterflies(typename F::T* x, const typename F::T* twb) {
     using T = typename F::T;
     constexpr uint32_t M = 1u << R;
@@ -93,8 +96,10 @@ __device__ __forceinline__ void butterflies(typename F::T* x, const typename F::
 #pragma unroll
             for (uint32_t m = 0; m < M; ++m) {
                 if (m & (1u << j)) continue;
+                constexpr bool a0 = A0 && Q0 == 0;
+                const bool one = a0 && (j == 0 || (j == 1 && (m & 1u) == 0));
                 const T w = twb[(1u << (Q0 + j)) - 1 + ((m & ((1u << j) - 1)) << Q0)];
-                const T yw = F::mul_c(x[m | (1u << j)], w);
+                const T yw = one ? x[m | (1u << j)] : F::mul_c(x[m | (1u << j)], w);
                 const T xv = x[m];
                 x[m] = F::add(xv, yw);
                 x[m | (1u << j)] = F::sub(xv, yw);
@@ -107,10 +112,12 @@ __device__ __forceinline__ void butterflies(typename F::T* x, const typename F::
 #pragma unroll
             for (uint32_t m = 0; m < M; ++m) {
                 if (m & (1u << j)) continue;
+                constexpr bool a0 = A0 && Q0 == 0;
+                const bool one = a0 && (j == 0 || (j == 1 && (m & 1u) == 0));
                 const T w = twb[(1u << (Q0 + j)) - 1 + ((m & ((1u << j) - 1)) << Q0)];
                 const T xv = x[m], yv = x[m | (1u << j)];
                 x[m] = F::add(xv, yv);
-                x[m | (1u << j)] = F::mul_c(F::sub(xv, yv), w);
+                x[m | (1u << j)] = one ? F::sub(xv, yv) : F::mul_c(F::sub(xv, yv), w);
             }
         }
     }
@@ -203,7 +210,7 @@ __device__ __forceinline__ ElemIO<T, Ld, St> elem_io(Ld ld, St st) {
 }
 
 // A full round: every group is loaded from `in`, transformed, stored to `out`.
-template <class F, int S, int R, int Q0, bool DIT, class In, class Out>
+template <class F, int S, int R, int Q0, bool DIT, bool A0 = false, class In, class Out>
 __device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC, const In& in, const Out& out) {
     using T = typename F::T;
     constexpr uint32_t M = 1u << R;
@@ -212,7 +219,7 @@ __device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC
         Group<R, Q0> g(gi, logC);
         T x[M];
         in.template load<R, Q0>(g.c, g.tb, x);
-        butterflies<F, R, Q0, DIT>(x, tws + (g.b & ((1u << Q0) - 1)));
+        butterflies<F, R, Q0, DIT, A0>(x, tws + (g.b & ((1u << Q0) - 1)));
         out.template store<R, Q0>(g.c, g.tb, x);
     }
 }
@@ -287,7 +294,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         if (active) {
             if constexpr (R1 > 0) lio.template load<R0, 0>(g.c, g.tb, x);
             else gio.template load<R0, 0>(g.c, g.tb, x);
-            butterflies<F, R0, 0, false>(x, tws);
+            butterflies<F, R0, 0, false, true>(x, tws);
         }
         for (int r = int(A.n_regions) - 1; r >= 0; --r) {
             const T* twist = A.scale + uint64_t(r) * n + row0 + g.tb;
@@ -298,7 +305,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
                 T y[M];
 #pragma unroll
                 for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_c(x[m], twist[m]);
-                butterflies<F, R0, 0, true>(y, twf);
+                butterflies<F, R0, 0, true, true>(y, twf);
                 if constexpr (R1 > 0) lio.template store<R0, 0>(g.c, g.tb, y);
                 else gio_r.template store<R0, 0>(g.c, g.tb, y);
             }
@@ -320,11 +327,11 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         auto stf = [&](uint32_t, uint32_t, T) {};
         const auto gather = elem_io<T>(ldf, stf);
         if constexpr (R1 > 0) {
-            round_io<F, S, R0, 0, true>(tws, logC, gather, lio);
+            round_io<F, S, R0, 0, true, true>(tws, logC, gather, lio);
             __syncthreads();
             round_io<F, S, R1, R0, true>(tws, logC, lio, gio);
         } else {
-            round_io<F, S, R0, 0, true>(tws, logC, gather, gio);
+            round_io<F, S, R0, 0, true, true>(tws, logC, gather, gio);
         }
     } else if constexpr (MODE == INNER_DIF_SCATTER) {
         // store position p at row bitrev(p), times scale[row]
@@ -339,9 +346,9 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         if constexpr (R1 > 0) {
             round_io<F, S, R1, R0, false>(tws, logC, gio, lio);
             __syncthreads();
-            round_io<F, S, R0, 0, false>(tws, logC, lio, scatter);
+            round_io<F, S, R0, 0, false, true>(tws, logC, lio, scatter);
         } else {
-            round_io<F, S, R0, 0, false>(tws, logC, gio, scatter);
+            round_io<F, S, R0, 0, false, true>(tws, logC, gio, scatter);
         }
     } else if constexpr (MODE == OUTER_DIT_FWD) {
         if constexpr (R1 > 0) {
