@@ -121,6 +121,23 @@ def test_bb_lde_column_tiles(shape_ctx, orc, log_cols):
     assert (ctx.coset_lde(ctx.upload(m), 1, 31).download() == orc.lde("bb", m, 1, 31)).all()
 
 
+@pytest.mark.parametrize("f,L,w,k", [("bb", 22, 64, 1), ("gl", 22, 8, 2)])
+def test_lde_round_trip_at_full_height(ctx, f, L, w, k):
+    """Full-height property check (no oracle at this size): LDE row i * 2^k is
+    P(shift * omega_n^i), so the coset inverse DFT of those rows gives P's
+    coefficients and the plain DFT of the coefficients gives the trace back,
+    bit for bit.  2^22 rows: two outer passes plus the fused inner pass for
+    the LDE, and the inverse and forward DFT kinds at their full depth."""
+    shift = 31 if f == "bb" else 7
+    trace = rnd(9000 + L + k, (1 << L, w), f)
+    lde = ctx.coset_lde(ctx.upload(trace), k, shift)
+    coset0 = lde.download()[:: 1 << k].copy()
+    lde.free()
+    coefs = ctx.dft(3, ctx.upload(coset0), shift)
+    back = ctx.dft(0, coefs, 1).download()
+    assert (back == trace).all()
+
+
 def test_bb_lde_pass_shapes_agree_at_scale(shape_ctx):
     """2^20 x 64 with 2 added bits under three pass shapes (5-, 3- and
     2-pass plans): identical Merkle roots of the 2^22 x 64 LDEs."""
