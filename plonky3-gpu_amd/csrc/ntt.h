@@ -68,6 +68,10 @@ struct LdePassGroup {
 template <class F>
 std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob<typename F::T>>& jobs);
 
+// Once per device, before the first pass launch (the context does it at
+// creation): lets every pass kernel take up to 160 KiB of dynamic LDS.
+int ntt_prepare_device();
+
 // Launch one group; d_work holds g.work in device memory.
 template <class F>
 int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typename F::T>* d_work, hipStream_t st);

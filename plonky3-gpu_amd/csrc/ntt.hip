@@ -29,6 +29,8 @@
 // (leaf hashing, openings, downloads) map LDE row j to stored row
 // (j mod 2^k) * n + j / 2^k.  Zero padding is never materialised.
 
+#include <utility>
+
 #include "field.h"
 #include "ntt.h"
 
@@ -547,18 +549,38 @@ uint32_t pass_lds_bytes(int mode, uint32_t s, uint32_t logC) {
 constexpr uint64_t MAX_GRID = 1ull << 22;
 constexpr int MAX_DYN_LDS = 160 * 1024;  // LDS of one gfx950 workgroup
 
+// Allow every pass kernel the full dynamic LDS of a workgroup (above the
+// 64 KiB default).  Done once per device before any launch, so that no first
+// launch happens inside a stream capture.
+template <class F, int MODE, int S>
+hipError_t allow_lds_s() {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ntt_pass<F, MODE, S>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, MAX_DYN_LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ntt_pass_list<F, MODE, S>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, MAX_DYN_LDS);
+    return e;
+}
+template <class F, int MODE, int... S>
+hipError_t allow_lds_mode(std::integer_sequence<int, S...>) {
+    hipError_t e = hipSuccess;
+    ((e = e == hipSuccess ? allow_lds_s<F, MODE, S>() : e), ...);
+    return e;
+}
+template <class F>
+hipError_t allow_lds_field() {
+    using Widths = std::make_integer_sequence<int, int(MAX_STAGES) + 1>;
+    hipError_t e = allow_lds_mode<F, OUTER_DIF_INV>(Widths{});
+    if (e == hipSuccess) e = allow_lds_mode<F, OUTER_DIT_FWD>(Widths{});
+    if (e == hipSuccess) e = allow_lds_mode<F, INNER_LDE>(Widths{});
+    if (e == hipSuccess) e = allow_lds_mode<F, INNER_DIT_GATHER>(Widths{});
+    if (e == hipSuccess) e = allow_lds_mode<F, INNER_DIF_SCATTER>(Widths{});
+    return e;
+}
+
 template <class F, int MODE, int S>
 void launch_s(const NttArgs<typename F::T>& A, uint64_t blocks, uint32_t lds, const NttWork<typename F::T>* work,
               uint32_t count, hipStream_t st) {
-    // allow dynamic LDS above the 64 KiB default, once per kernel
-    static const bool attr = [] {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&ntt_pass<F, MODE, S>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, MAX_DYN_LDS);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&ntt_pass_list<F, MODE, S>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, MAX_DYN_LDS);
-        return true;
-    }();
-    (void)attr;
     const uint32_t grid = uint32_t(blocks < MAX_GRID ? blocks : MAX_GRID);
     if (work)
         hipLaunchKernelGGL((ntt_pass_list<F, MODE, S>), dim3(grid), dim3(NTT_THREADS), lds, st, work, count,
@@ -714,6 +736,12 @@ int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F
         launch_pass<F, INNER_DIF_SCATTER>(A, p.s_inner, tune, st);
     }
     return int(hipGetLastError());
+}
+
+int ntt_prepare_device() {
+    hipError_t e = allow_lds_field<BabyBear>();
+    if (e == hipSuccess) e = allow_lds_field<Goldilocks>();
+    return int(e);
 }
 
 template <class F>

@@ -1390,6 +1390,7 @@ int p3h_ctx_create(int device, uint64_t budget, p3h_ctx** out) {
         hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
         if (device < 0 || device >= n) throw Err{P3H_EINVAL, "no such device"};
         hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipError_t(ntt_prepare_device()), "NTT kernel LDS limits");
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         size_t fr = 0, tot = 0;
         hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");

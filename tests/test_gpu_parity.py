@@ -377,6 +377,38 @@ def test_batch_plan_replays_match_oracle(ctx, golden, orc, use_graph):
     plan.free()
 
 
+def test_batch_graph_as_first_gpu_work(tmp_path):
+    """A fresh process whose first GPU work is a batch plan captured as a HIP
+    graph (so every pass kernel's first launch is inside the capture) gives
+    the cap of the eager batched commit."""
+    import os
+    import subprocess
+    import sys
+
+    pkg = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "plonky3-gpu_amd")
+    script = tmp_path / "first_work.py"
+    script.write_text(f"""
+import sys
+import numpy as np
+sys.path.insert(0, {pkg!r})
+import p3hip
+g = np.load({os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.npz")!r})
+rng = np.random.default_rng(5)
+traces = [rng.integers(0, {BB_P}, size=(1 << h, w), dtype=np.uint32) for h, w in ((12, 20), (10, 33), (12, 7))]
+with p3hip.Context(0) as ctx:
+    ctx.set_poseidon2(p3hip.BABYBEAR, g["consts_bb"], 8, 13)
+    mats = [ctx.upload(t) for t in traces]
+    plan = ctx.batch_plan(mats, [1, 2, 1], 31, cap_height=0, use_graph=True)
+    cap = plan.run()
+    want = ctx.batch_commit(mats, [1, 2, 1], 31, 0).cap()
+    assert (cap == want).all(), (cap, want)
+    plan.free()
+print("ok")
+""")
+    r = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
+
+
 def test_autotuned_lde_matches_oracle(ctx, orc):
     best = ctx.autotune_lde(p3hip.BABYBEAR, 12, 24, 1, iters=1)
     assert best["ms"] > 0
