@@ -278,7 +278,8 @@ def batch_traces(seed: int = 7, count: int = 64):
 
 
 def bench_batch(p3hip, args):
-    """Config 4: one multi-height MMCS over 64 LDEs, planned and replayed as a
+    """64 BabyBear traces (heights 2^12..2^20, widths 8..128, blowup 2 or 4): one
+    multi-height MMCS over 64 LDEs, planned and replayed as a
     HIP graph, against the plain per-trace loop (LDE + single-matrix commit +
     root per trace) on the same GPU and inputs."""
     ctx = p3hip.Context(0)
