@@ -149,17 +149,25 @@ __device__ __forceinline__ uint32_t reduce_u64(uint64_t x) {
 
 // Quotient, pass 2: 16 lanes per row read the row with 16-byte loads (16
 // lanes x 4 columns per step), multiply by alpha^c from an LDS table and sum
-// across the 16 lanes; lane 0 of the row adds A_j - R_j B_j at bitrev(j).
+// across the 16 lanes; lane 0 of the row stores A_j - R_j B_j over the row's
+// A_j (words 0-3 of its 8 in ab), in stored-row order.  Pass 3 adds it at
+// bitrev(j): scattered reads and contiguous writes, instead of one scattered
+// read-modify-write per row at the end of pass 2.
+// The LDS table is coefficient-major (at[q * W + c] = coefficient q of
+// alpha^c, W = the width rounded up to a multiple of 4), so a lane's 4 columns
+// are one 16-byte read per coefficient and the 16 lanes of a row read 256
+// consecutive bytes: no bank conflicts.  It is dynamic LDS of 16 W bytes, so
+// narrow matrices leave the CU room for many workgroups.
 constexpr uint32_t QR_THREADS = 256;
 constexpr uint32_t QR_LDS_COLS = 4096;
 __global__ __launch_bounds__(QR_THREADS) void quotient_reduce_kernel(LdeView v, const uint32_t* apow_m,
-                                                                     const uint32_t* ab, uint32_t* qv) {
-    __shared__ uint32_t at[QR_LDS_COLS * 4];
+                                                                     uint32_t* ab) {
+    extern __shared__ __align__(16) uint32_t at[];
     const bool in_lds = v.w <= QR_LDS_COLS;
+    const uint32_t W = (v.w + 3) & ~3u;
     if (in_lds)
-        for (uint32_t t = threadIdx.x; t < v.w * 4; t += QR_THREADS) at[t] = apow_m[t];
+        for (uint32_t t = threadIdx.x; t < v.w * 4; t += QR_THREADS) at[(t & 3) * W + (t >> 2)] = apow_m[t];
     __syncthreads();
-    const uint32_t* ap = in_lds ? at : apow_m;
     const uint32_t lN = v.L + v.k;
     const uint64_t N = 1ull << lN;
     const uint32_t sub = threadIdx.x & 15;
@@ -168,7 +176,27 @@ __global__ __launch_bounds__(QR_THREADS) void quotient_reduce_kernel(LdeView v, 
          s += (uint64_t(gridDim.x) * QR_THREADS) >> 4) {
         const uint32_t* row = v.data + s * v.w;
         uint64_t acc[4] = {0, 0, 0, 0};
-        if (vec) {
+        if (vec && in_lds) {
+            for (uint32_t c = sub * 4; c < v.w; c += 64) {
+                const uint4 x = *reinterpret_cast<const uint4*>(row + c);
+                const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint4 a = *reinterpret_cast<const uint4*>(at + q * W + c);
+                    acc[q] += BB::mont_lazy(xs[0], a.x);
+                    acc[q] += BB::mont_lazy(xs[1], a.y);
+                    acc[q] += BB::mont_lazy(xs[2], a.z);
+                    acc[q] += BB::mont_lazy(xs[3], a.w);
+                }
+            }
+        } else if (in_lds) {
+            for (uint32_t c = sub; c < v.w; c += 16) {
+                const uint32_t x = row[c];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] += BB::mont_lazy(x, at[q * W + c]);
+            }
+        } else if (vec) {
+            const uint32_t* ap = apow_m;
             for (uint32_t c = sub * 4; c < v.w; c += 64) {
                 const uint4 x = *reinterpret_cast<const uint4*>(row + c);
                 const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
@@ -178,6 +206,7 @@ __global__ __launch_bounds__(QR_THREADS) void quotient_reduce_kernel(LdeView v, 
                     for (int q = 0; q < 4; ++q) acc[q] += BB::mont_lazy(xs[e], ap[4 * (c + e) + q]);
             }
         } else {
+            const uint32_t* ap = apow_m;
             for (uint32_t c = sub; c < v.w; c += 16) {
                 const uint32_t x = row[c];
 #pragma unroll
@@ -192,13 +221,25 @@ __global__ __launch_bounds__(QR_THREADS) void quotient_reduce_kernel(LdeView v, 
             const uint32_t* a = ab + s * 8;
             const E4 A = {{a[0], a[1], a[2], a[3]}}, Bm = {{a[4], a[5], a[6], a[7]}};
             const E4 t = e4_sub(A, e4_mul_m(R, Bm));
-            const uint32_t j = lde_row_of(s, v.L, v.k);
-            const uint64_t pos = lN == 0 ? 0 : (__brev(j) >> (32 - lN));
-            uint32_t* o = qv + pos * 4;
+            uint32_t* o = ab + s * 8;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) o[q] = BB::add(o[q], t.c[q]);
+            for (int q = 0; q < 4; ++q) o[q] = t.c[q];
         }
     }
+}
+
+// Quotient, pass 3 (one thread per output position): q[pos] += term of LDE
+// row j = bitrev(pos), read from that row's stored position in ab.
+__global__ void quotient_scatter_kernel(LdeView v, const uint32_t* ab, uint32_t* qv) {
+    const uint32_t lN = v.L + v.k;
+    const uint64_t pos = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pos >= (1ull << lN)) return;
+    const uint32_t j = lN == 0 ? 0u : (__brev(uint32_t(pos)) >> (32 - lN));
+    const uint64_t s = (uint64_t(j & ((1u << v.k) - 1)) << v.L) | (j >> v.k);  // stored row of LDE row j
+    const uint32_t* t = ab + s * 8;
+    uint32_t* o = qv + pos * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = BB::add(o[q], t[q]);
 }
 
 __global__ void fri_mix_kernel(uint32_t* cur, const uint32_t* next, uint64_t n, E4 beta, const uint32_t* beta_dev,
@@ -255,7 +296,9 @@ int pcs_quotient_device(const LdeView& v, uint32_t shift, uint32_t omega_m, cons
     hipLaunchKernelGGL(quotient_weights_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, v, shift, omega_m, z, s,
                        aoff_m, P, ab);
     const uint64_t grid = std::min<uint64_t>(blocks_for(N * 16, QR_THREADS), 1u << 16);
-    hipLaunchKernelGGL(quotient_reduce_kernel, dim3(uint32_t(grid)), dim3(QR_THREADS), 0, st, v, apow_m, ab, q);
+    const uint32_t lds = v.w <= QR_LDS_COLS ? ((v.w + 3) & ~3u) * 16 : 0;
+    hipLaunchKernelGGL(quotient_reduce_kernel, dim3(uint32_t(grid)), dim3(QR_THREADS), lds, st, v, apow_m, ab);
+    hipLaunchKernelGGL(quotient_scatter_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, st, v, ab, q);
     return int(hipGetLastError());
 }
 
