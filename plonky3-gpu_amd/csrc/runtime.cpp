@@ -1434,6 +1434,24 @@ int p3h_ctx_memory(const p3h_ctx* ctx, uint64_t* in_use, uint64_t* budget, uint6
     return P3H_OK;
 }
 
+int p3h_host_alloc(uint64_t bytes, void** out) {
+    if (!out) return P3H_EINVAL;
+    *out = nullptr;
+    if (bytes == 0) return P3H_EINVAL;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();  // clear the sticky error of the failed allocation
+        return P3H_ENOMEM;
+    }
+    *out = p;
+    return P3H_OK;
+}
+
+int p3h_host_free(void* p) {
+    if (!p) return P3H_OK;
+    return hipHostFree(p) == hipSuccess ? P3H_OK : P3H_EDEVICE;
+}
+
 int p3h_ctx_trim(p3h_ctx* ctx) {
     return api(ctx, [&] { trim(ctx); });
 }

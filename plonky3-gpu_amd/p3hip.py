@@ -62,6 +62,8 @@ _SIGS = {
     "p3h_phase_stats": (_i32, [_vp, C.c_char_p, C.POINTER(_u64), C.POINTER(C.c_double)]),
     "p3h_ctx_set_ntt_tuning": (_i32, [_vp, _u32, _u32, _u32]),
     "p3h_ctx_set_hash_engine": (_i32, [_vp, _i32]),
+    "p3h_host_alloc": (_i32, [_u64, C.POINTER(_vp)]),
+    "p3h_host_free": (_i32, [_vp]),
     "p3h_poseidon2_set": (_i32, [_vp, _i32, _u32, _u32, _u32, _vp]),
     "p3h_poseidon2_permute": (_i32, [_vp, _i32, _vp, _u64]),
     "p3h_mat_upload": (_i32, [_vp, _i32, _vp, _u64, _u32, C.POINTER(_vp)]),
@@ -139,6 +141,37 @@ def load_library(path: str = LIB_PATH):
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+class PinnedArray(np.ndarray):
+    """A numpy array whose storage is page-locked host memory (p3h_host_alloc);
+    the memory is freed when the last view of it goes away."""
+
+
+class _PinnedBlock:
+    def __init__(self, lib, addr):
+        self.lib, self.addr = lib, addr
+
+    def __del__(self):
+        if self.addr:
+            self.lib.p3h_host_free(self.addr)
+            self.addr = None
+
+
+def pinned_empty(shape, dtype) -> PinnedArray:
+    """An uninitialised array in page-locked host memory: uploads from it, and
+    serving submissions of it, run at the full PCIe DMA rate."""
+    lib = load_library()
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape, dtype=np.int64)) * dtype.itemsize
+    p = _vp()
+    rc = lib.p3h_host_alloc(max(nbytes, 1), C.byref(p))
+    if rc != _OK:
+        raise MemoryError(f"p3h_host_alloc({nbytes}) failed with status {rc}")
+    buf = (C.c_char * max(nbytes, 1)).from_address(p.value)
+    # every numpy view of the memory keeps `buf` alive, and `buf` the block
+    buf._p3h_owner = _PinnedBlock(lib, p.value)
+    return np.frombuffer(buf, dtype=dtype, count=nbytes // dtype.itemsize).reshape(shape).view(PinnedArray)
 
 
 def _ptr(a: np.ndarray):

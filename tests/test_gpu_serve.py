@@ -60,3 +60,27 @@ def test_admission_under_a_small_budget(golden):
     with p3hip.Server(golden["consts_bb"], 8, 13, hbm_budget=1 << 16, n_slots=1) as sv:
         with pytest.raises(p3hip.BudgetError):
             sv.submit(reqs[0], 1, 31)
+
+
+def test_pinned_host_buffers(ctx, golden, orc):
+    """Uploads and serving submissions from page-locked host memory give the
+    oracle's commitment, and the memory reads back what was written."""
+    rng = np.random.default_rng(21)
+    t = rng.integers(0, BB_P, size=(1 << 9, 13), dtype=np.uint32)
+    pt = p3hip.pinned_empty(t.shape, np.uint32)
+    pt[...] = t
+    assert isinstance(pt, p3hip.PinnedArray) and (pt == t).all()
+    want = orc.flat_commit("bb", golden["consts_bb"], 8, 13, orc.lde("bb", t, 1, 31))
+    m = ctx.upload(pt)
+    assert (m.download() == t).all()
+    lde = ctx.coset_lde(m, 1, 31)
+    assert (ctx.mmcs_commit([lde], 0).root() == want).all()
+    with p3hip.Server(golden["consts_bb"], 8, 13, n_slots=2, max_batch=2) as sv:
+        ids = [sv.submit(pt, 1, 31), sv.submit(pt[: 1 << 8], 1, 31)]
+        assert (sv.wait(ids[0])[0] == want).all()
+        want_half = orc.flat_commit("bb", golden["consts_bb"], 8, 13, orc.lde("bb", t[: 1 << 8], 1, 31))
+        assert (sv.wait(ids[1])[0] == want_half).all()
+    # a view outlives the array it was cut from
+    tail = pt[-4:]
+    del pt
+    assert (tail == t[-4:]).all()
