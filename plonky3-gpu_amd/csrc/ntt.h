@@ -17,12 +17,24 @@ struct NttTuning {
     uint32_t log_slab;    // LDE only: run all passes on slabs of 2^log_slab columns in turn (0 = whole width)
 };
 
+// Stage twiddles: omega_{2^(q+1)}^(+-j) at (2^q - 1) + j for q < 10, j < 2^q,
+// multiply-constant form; the same for every transform size.  The twiddle
+// buffer of a 2^L transform is [omega^-e][omega^e] (e < n/2), then the
+// inverse and the forward stage table.
+constexpr uint32_t TWC_LEN = 1023;
+template <class T>
+inline const T* twc_of(const T* tw_inv_fwd, uint32_t L, bool forward) {
+    return tw_inv_fwd + (L ? (uint64_t(1) << L) : 0) + (forward ? TWC_LEN : 0);
+}
+inline uint64_t twiddle_buffer_len(uint32_t L) { return (L ? (uint64_t(1) << L) : 0) + 2 * TWC_LEN; }
+
 // Arguments of one NTT pass over a row-major matrix (see ntt.hip).
 template <class T>
 struct NttArgs {
     const T* src;
     T* dst;
     const T* tw;     // twiddles omega^(+-e), e < n/2, in multiply-constant form
+    const T* twc;    // stage twiddles of the same direction (TWC_LEN entries, see twc_of)
     const T* scale;  // mode-dependent table, or nullptr
     uint64_t src_region_rows;  // row offset between regions in src
     uint64_t dst_region_rows;  // row offset between regions in dst

@@ -51,7 +51,9 @@ enum Mode : int {
     OUTER_DIT_FWD = 1,     // DIT stages, bits ascending, per region
     INNER_LDE = 2,         // DIF stages + per-coset twist + DIT stages
     INNER_DIT_GATHER = 3,  // load rows bitrev(p) (times scale[row]), DIT stages
-    INNER_DIF_SCATTER = 4  // DIF stages, store position p at row bitrev(p) times scale[row]
+    INNER_DIF_SCATTER = 4,  // DIF stages, store position p at row bitrev(p) times scale[row]
+    OUTER_DIF_INV_SPLIT = 5,  // OUTER_DIF_INV on a full tile, exchanged half by half
+    OUTER_DIT_FWD_SPLIT = 6   // OUTER_DIT_FWD on a full tile, exchanged half by half
 };
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits) {
@@ -60,19 +62,24 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t x, uint32_t bits) {
 
 // Twiddles of one pass.  Stage q (bit a+q) at tile position t, low index lo0
 // (the whole tile shares one low index), uses
-//   omega_{2^(a+q+1)}^(((t mod 2^q) << a) + lo0),
-// stored at tws[(2^q - 1) + (t mod 2^q)], gathered once per workgroup from the
-// global table (omega_n^e, e < n/2).
-template <class T>
-__device__ __forceinline__ void build_twiddles(T* tws, const T* tw, uint32_t L, uint32_t a, uint32_t S,
+//   omega_{2^(a+q+1)}^(((t mod 2^q) << a) + lo0)
+//     = omega_{2^(q+1)}^(t mod 2^q) * omega_{2^(a+q+1)}^lo0,
+// stored at tws[(2^q - 1) + (t mod 2^q)].  The first factor comes from the
+// contiguous stage table twc (the same 4 or 8 KiB for every pass and size, so
+// it stays in cache), the second, one value per stage, from the size-n table
+// tw (omega_n^e, e < n/2); inner passes (a = 0) have lo0 = 0 and copy twc.
+template <class F>
+__device__ __forceinline__ void build_twiddles(typename F::T* tws, const typename F::T* twc,
+                                               const typename F::T* tw, uint32_t L, uint32_t a, uint32_t S,
                                                uint32_t lo0) {
     const uint32_t count = (1u << S) - 1;
     for (uint32_t i = threadIdx.x; i < count; i += NTT_THREADS) {
-        uint32_t jj = i + 1;  // 2^q + j
-        uint32_t q = 31 - __clz(jj);
-        uint32_t j = jj - (1u << q);
-        uint64_t e = uint64_t((j << a) + lo0) << (L - a - q - 1);
-        tws[i] = tw[e];
+        typename F::T v = twc[i];
+        if (lo0) {
+            const uint32_t q = 31 - __clz(i + 1);
+            v = F::mul_c(v, tw[uint64_t(lo0) << (L - a - q - 1)]);
+        }
+        tws[i] = v;
     }
 }
 
@@ -189,6 +196,57 @@ struct GlobalIO {
     }
 };
 
+// The same rows through buffer instructions: a group's 2^R elements share one
+// 32-bit lane offset (tb * tstride + c) and element m adds the wave-uniform
+// m * 2^Q0 * tstride as the scalar offset, so a group needs one address
+// register instead of 2^R 64-bit addresses.  Offsets are bytes from the tile's
+// origin; the host uses this only when every offset of the pass stays below
+// 2^32.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+    // gfx9 raw buffer: stride 0, the largest range, 32-bit data format word
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(0xFFFFFFFFu), 0x00020000);
+}
+__device__ __forceinline__ void buf_ld(uint32_t& v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    v = __builtin_amdgcn_raw_buffer_load_b32(r, int(vo), int(so), 0);
+}
+__device__ __forceinline__ void buf_ld(uint64_t& v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    v = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, int(vo), int(so), 0));
+}
+__device__ __forceinline__ void buf_st(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, int(vo), int(so), 0);
+}
+__device__ __forceinline__ void buf_st(uint64_t v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, int(vo), int(so), 0);
+}
+
+template <class T>
+struct BufIO {
+    __amdgpu_buffer_rsrc_t src, dst;
+    uint32_t tstride_b;  // bytes between tile positions
+    uint32_t cols;
+    template <int R, int Q0>
+    __device__ __forceinline__ void load(uint32_t c, uint32_t tb, T* x) const {
+        if (c >= cols) {
+#pragma unroll
+            for (uint32_t m = 0; m < (1u << R); ++m) x[m] = T(0);
+            return;
+        }
+        const uint32_t vo = tb * tstride_b + c * uint32_t(sizeof(T));
+        const uint32_t step = tstride_b << Q0;
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) buf_ld(x[m], src, vo, m * step);
+    }
+    template <int R, int Q0>
+    __device__ __forceinline__ void store(uint32_t c, uint32_t tb, const T* x) const {
+        if (c >= cols) return;
+        const uint32_t vo = tb * tstride_b + c * uint32_t(sizeof(T));
+        const uint32_t step = tstride_b << Q0;
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << R); ++m) buf_st(x[m], dst, vo, m * step);
+    }
+};
+
 // Element-wise access through ld(c, t) / st(c, t, v), for the bit-reversed
 // gather and scatter of the plain transforms.
 template <class T, class Ld, class St>
@@ -226,10 +284,73 @@ __device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC
     }
 }
 
+// Outer pass on a full tile: 2^S positions x 2^LOGC columns = 2^tile_log
+// words, LOGC = tile_log - S.  The tile sits in registers, 2^(tile_log - 9)
+// words per thread, and goes through an LDS tile of half the columns, one
+// half after the other, between the two rounds.  A thread keeps the column
+// c = tid mod 2^LOGC in both rounds (its groups are tid + g * NTT_THREADS and
+// NTT_THREADS is a multiple of 2^LOGC), so it only ever exchanges its own
+// column's data, and 2^LOGC consecutive lanes read and write whole row
+// segments of 2^tile_log / 2^S words (128 bytes at S = 10) with the LDS
+// footprint of a tile half as wide.  Lanes of the other half idle while one
+// half exchanges.
+template <class F, bool DIT, int S, class IO>
+__device__ __forceinline__ void outer_split(const typename F::T* tws, const IO& gio, typename F::T* lds) {
+    using T = typename F::T;
+    constexpr int TL = int(tile_log<T>());
+    constexpr int LOGC = TL - S;
+    static_assert(S > 5 && LOGC >= 1 && LOGC <= 9, "split pass shape");
+    constexpr int R0 = (S + 1) / 2, R1 = S - R0;
+    constexpr int RI = DIT ? R0 : R1, QI = DIT ? 0 : R0;  // the round that reads HBM
+    constexpr int RO = DIT ? R1 : R0, QO = DIT ? R0 : 0;  // the round that writes HBM
+    constexpr uint32_t NW = 1u << (TL - 9);
+    constexpr uint32_t GI = NW >> RI, GO = NW >> RO, MI = 1u << RI, MO = 1u << RO;
+    const LdsIO<T, Tile<S>::STRIDE> half{lds};
+    const uint32_t c = threadIdx.x & ((1u << LOGC) - 1);
+    const uint32_t phase = c >> (LOGC - 1);
+    const uint32_t cl = c & ((1u << (LOGC - 1)) - 1);
+    T v[NW];
+#pragma unroll
+    for (uint32_t g = 0; g < GI; ++g) {
+        const Group<RI, QI> gr(threadIdx.x + g * NTT_THREADS, LOGC);
+        gio.template load<RI, QI>(gr.c, gr.tb, v + g * MI);
+    }
+#pragma unroll
+    for (uint32_t g = 0; g < GI; ++g) {
+        const Group<RI, QI> gr(threadIdx.x + g * NTT_THREADS, LOGC);
+        butterflies<F, RI, QI, DIT>(v + g * MI, tws + (gr.b & ((1u << QI) - 1)));
+    }
+#pragma unroll
+    for (uint32_t p = 0; p < 2; ++p) {
+        if (p) __syncthreads();  // the first half has read its exchanged data
+        if (phase == p) {
+#pragma unroll
+            for (uint32_t g = 0; g < GI; ++g) {
+                const Group<RI, QI> gr(threadIdx.x + g * NTT_THREADS, LOGC);
+                half.template store<RI, QI>(cl, gr.tb, v + g * MI);
+            }
+        }
+        __syncthreads();
+        if (phase == p) {
+#pragma unroll
+            for (uint32_t g = 0; g < GO; ++g) {
+                const Group<RO, QO> gr(threadIdx.x + g * NTT_THREADS, LOGC);
+                half.template load<RO, QO>(cl, gr.tb, v + g * MO);
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t g = 0; g < GO; ++g) {
+        const Group<RO, QO> gr(threadIdx.x + g * NTT_THREADS, LOGC);
+        butterflies<F, RO, QO, DIT>(v + g * MO, tws + (gr.b & ((1u << QO) - 1)));
+        gio.template store<RO, QO>(gr.c, gr.tb, v + g * MO);
+    }
+}
+
 // Dynamic LDS of a pass (pass_lds_bytes on the host): the column-major
 // exchange tile with one padding word per column, (2^S + 1) << logC words,
-// when the pass has two rounds; then the gathered twiddles, 2^S - 1 words
-// (twice for the fused inner pass).  Sized to the tile actually used, so
+// when the pass has two rounds (half the columns for the split passes); then
+// the gathered twiddles, 2^S - 1 words (twice for the fused inner pass).  Sized to the tile actually used, so
 // narrow tiles leave room for more resident workgroups per CU.
 template <class F, int MODE, int S>
 __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, uint32_t block, typename F::T* smem) {
@@ -243,9 +364,10 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     constexpr int R0 = S <= 5 ? S : (S + 1) / 2;
     constexpr int R1 = S - R0;
 
+    constexpr bool SPLIT = MODE == OUTER_DIF_INV_SPLIT || MODE == OUTER_DIT_FWD_SPLIT;
     const uint32_t a = A.a, logC = A.logC;
     T* lds = smem;
-    T* tws = smem + (R1 > 0 ? (STRIDE << logC) : 0u);
+    T* tws = smem + (R1 > 0 ? (STRIDE << (SPLIT ? logC - 1 : logC)) : 0u);
     T* twf = tws + ((1u << S) - 1);  // forward twiddles of the fused inner pass
 
     // block -> (column tile, low index, high index, region)
@@ -267,8 +389,9 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     auto pos = [&](uint32_t t) -> uint64_t { return row0 | (uint64_t(t) << a); };
 
     if constexpr (S > 0) {
-        build_twiddles(tws, A.tw, A.L, a, S, lo0);
-        if (MODE == INNER_LDE) build_twiddles(twf, A.tw + (n >> 1), A.L, a, S, lo0);
+        build_twiddles<F>(tws, A.twc, A.tw, A.L, a, S, lo0);
+        // the fused inner pass (a = 0, lo0 = 0): forward stage table after the inverse one
+        if (MODE == INNER_LDE) build_twiddles<F>(twf, A.twc + TWC_LEN, A.tw + (n >> 1), A.L, a, S, lo0);
         __syncthreads();
     }
 
@@ -352,6 +475,14 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         } else {
             round_io<F, S, R0, 0, false, true>(tws, logC, gio, scatter);
         }
+    } else if constexpr (SPLIT) {
+        // the host plans the split form only for two-round passes with
+        // logC = tile_log - S and a region below 4 GiB
+        if constexpr (R1 > 0) {
+            const BufIO<T> bio{buf_rsrc(src + tile_off), buf_rsrc(dst + tile_off), uint32_t((ld << a) * sizeof(T)),
+                               w - c0};
+            outer_split<F, MODE == OUTER_DIT_FWD_SPLIT, S>(tws, bio, lds);
+        }
     } else if constexpr (MODE == OUTER_DIT_FWD) {
         if constexpr (R1 > 0) {
             round_io<F, S, R0, 0, true>(tws, logC, gio, lio);
@@ -385,10 +516,20 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t g, uint32_t grid) {
     return x < full ? x * per + i : full * per + (x - full) * (per - 1) + i;
 }
 
+// Waves per SIMD the register allocation must leave room for: the BabyBear
+// split passes hold 64 words per lane and would otherwise settle at 3, one
+// 8-wave workgroup per CU; 4 lets two workgroups (and their two LDS half
+// tiles) share a CU.  1 = no bound.
+template <class F, int MODE>
+constexpr int pass_min_waves() {
+    return (MODE == OUTER_DIF_INV_SPLIT || MODE == OUTER_DIT_FWD_SPLIT) && sizeof(typename F::T) == 4 ? 4 : 1;
+}
+
 // Workgroups stride over the pass's blocks when there are more blocks than the
 // grid holds (a dispatch counts at most 2^32 - 1 work-items).
 template <class F, int MODE, int S>
-__global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A, uint32_t total) {
+__global__ __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(pass_min_waves<F, MODE>())))
+void ntt_pass(NttArgs<typename F::T> A, uint32_t total) {
     extern __shared__ __align__(16) unsigned char ntt_smem[];
     const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x);
     for (uint32_t b = b0; b < total; b += gridDim.x) {
@@ -401,7 +542,8 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass(NttArgs<typename F::T> A
 // run in one grid; block_start[i] is the first block of descriptor i
 // (ascending, block_start[count] = total).
 template <class F, int MODE, int S>
-__global__ __launch_bounds__(NTT_THREADS) void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count,
+__global__ __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(pass_min_waves<F, MODE>())))
+void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count,
                                                              uint32_t total) {
     extern __shared__ __align__(16) unsigned char ntt_smem[];
     const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x);
@@ -540,9 +682,24 @@ uint64_t finalize_args(NttArgs<T>& A, int mode, uint32_t s, const NttTuning& tun
 template <class T>
 uint32_t pass_lds_bytes(int mode, uint32_t s, uint32_t logC) {
     const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;
-    const uint64_t tile = r0 < s ? (((1ull << s) + 1) << logC) : 0;
+    const bool split = mode == OUTER_DIF_INV_SPLIT || mode == OUTER_DIT_FWD_SPLIT;
+    const uint64_t tile = r0 < s ? (((1ull << s) + 1) << (split ? logC - 1 : logC)) : 0;
     const uint64_t tws = s ? ((1ull << s) - 1) * (mode == INNER_LDE ? 2 : 1) : 0;
     return uint32_t((tile + tws) * sizeof(T));
+}
+
+// The kernel variant of a planned pass: an outer two-round pass whose tile is
+// the full 2^tile_log words (one workgroup per CU if held in LDS whole) runs
+// in the split form, with half the tile in LDS at a time, when every byte
+// offset from a tile's origin fits 32 bits (an n x ld region below 4 GiB).
+template <class T>
+int pass_mode(int mode, uint32_t s, const NttArgs<T>& A) {
+    const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;
+    const uint64_t region_bytes = (uint64_t(A.ld) << A.L) * sizeof(T);
+    if ((mode == OUTER_DIF_INV || mode == OUTER_DIT_FWD) && r0 < s && s + A.logC == tile_log<T>() &&
+        region_bytes <= 0xFFFFFFFFull)
+        return mode == OUTER_DIF_INV ? OUTER_DIF_INV_SPLIT : OUTER_DIT_FWD_SPLIT;
+    return mode;
 }
 
 // grid cap: 2^22 workgroups x 512 work-items = 2^31 work-items per dispatch
@@ -575,6 +732,8 @@ hipError_t allow_lds_field() {
     if (e == hipSuccess) e = allow_lds_mode<F, INNER_LDE>(Widths{});
     if (e == hipSuccess) e = allow_lds_mode<F, INNER_DIT_GATHER>(Widths{});
     if (e == hipSuccess) e = allow_lds_mode<F, INNER_DIF_SCATTER>(Widths{});
+    if (e == hipSuccess) e = allow_lds_mode<F, OUTER_DIF_INV_SPLIT>(Widths{});
+    if (e == hipSuccess) e = allow_lds_mode<F, OUTER_DIT_FWD_SPLIT>(Widths{});
     return e;
 }
 
@@ -615,14 +774,18 @@ void launch_any(int mode, uint32_t s, const NttArgs<typename F::T>& A, uint64_t 
         case OUTER_DIT_FWD: launch_mode<F, OUTER_DIT_FWD>(s, A, blocks, lds, work, count, st); break;
         case INNER_LDE: launch_mode<F, INNER_LDE>(s, A, blocks, lds, work, count, st); break;
         case INNER_DIT_GATHER: launch_mode<F, INNER_DIT_GATHER>(s, A, blocks, lds, work, count, st); break;
+        case OUTER_DIF_INV_SPLIT: launch_mode<F, OUTER_DIF_INV_SPLIT>(s, A, blocks, lds, work, count, st); break;
+        case OUTER_DIT_FWD_SPLIT: launch_mode<F, OUTER_DIT_FWD_SPLIT>(s, A, blocks, lds, work, count, st); break;
         default: launch_mode<F, INNER_DIF_SCATTER>(s, A, blocks, lds, work, count, st); break;
     }
 }
 
 template <class F, int MODE>
 void launch_pass(NttArgs<typename F::T> A, uint32_t s, const NttTuning& tune, hipStream_t st) {
-    uint64_t blocks = finalize_args(A, MODE, s, tune);
-    launch_mode<F, MODE>(s, A, blocks, pass_lds_bytes<typename F::T>(MODE, s, A.logC), nullptr, 0, st);
+    using T = typename F::T;
+    const uint64_t blocks = finalize_args(A, MODE, s, tune);
+    const int mode = pass_mode<T>(MODE, s, A);
+    launch_any<F>(mode, s, A, blocks, pass_lds_bytes<T>(mode, s, A.logC), nullptr, 0, st);
 }
 
 // All passes of one LDE over columns [0, w) of row-stride-ld matrices.
@@ -637,6 +800,7 @@ void lde_slab(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t 
     A.ld = ld;
     A.n_regions = 1;
     A.tw = tw_inv_fwd;  // inverse half first
+    A.twc = twc_of(tw_inv_fwd, L, false);
     // 1. outer DIF passes over the high bits, top down: input -> region 0 of out
     uint32_t top = L;
     const typename F::T* cur = in;
@@ -660,6 +824,7 @@ void lde_slab(const typename F::T* in, typename F::T* out, uint32_t L, uint32_t 
     // 3. outer DIT passes per coset, bottom up, in place
     A.scale = nullptr;
     A.tw = tw_inv_fwd + (n >> 1);
+    A.twc = twc_of(tw_inv_fwd, L, true);
     A.src = out;
     A.dst = out;
     A.src_region_rows = A.dst_region_rows = n;
@@ -702,6 +867,7 @@ int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F
         // forward (optionally coset): gather rows in bit-reversed order with
         // the row scale, DIT inner, DIT outer in place.
         A.tw = tw_inv_fwd + (n >> 1);
+        A.twc = twc_of(tw_inv_fwd, L, true);
         A.a = 0;
         A.src = in;
         A.dst = out;
@@ -719,6 +885,7 @@ int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F
         // inverse (optionally coset): DIF outer (in -> tmp), DIF inner with
         // bit-reversed scatter and the row scale (tmp -> out).
         A.tw = tw_inv_fwd;
+        A.twc = twc_of(tw_inv_fwd, L, false);
         uint32_t top = L;
         const typename F::T* cur = in;
         for (uint32_t i = 0; i < p.n_outer; ++i) {
@@ -792,6 +959,7 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
         A.ld = j.w;
         A.n_regions = 1;
         A.tw = j.tw;
+        A.twc = twc_of(j.tw, j.L, false);
         uint32_t top = j.L;
         const T* cur = j.in;
         for (uint32_t i = 0; i < p.n_outer; ++i) {
@@ -811,6 +979,7 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
         passes.push_back({INNER_LDE, p.s_inner, 0, A, j.tune});
         A.scale = nullptr;
         A.tw = j.tw + (n >> 1);
+        A.twc = twc_of(j.tw, j.L, true);
         A.src = j.out;
         A.dst = j.out;
         A.src_region_rows = A.dst_region_rows = n;
@@ -827,26 +996,40 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
         lo = ps.slot < lo ? ps.slot : lo;
         hi = ps.slot > hi ? ps.slot : hi;
     }
+    // tile geometry and kernel variant of every pass
+    std::vector<NttWork<T>> geo(passes.size());
+    std::vector<uint64_t> nblk(passes.size());
+    std::vector<int> kmode(passes.size());
+    for (size_t i = 0; i < passes.size(); ++i) {
+        geo[i] = NttWork<T>{};
+        geo[i].args = passes[i].A;
+        nblk[i] = finalize_args(geo[i].args, passes[i].mode, passes[i].s, passes[i].tune);
+        kmode[i] = pass_mode<T>(passes[i].mode, passes[i].s, geo[i].args);
+    }
     for (int slot = lo; slot <= hi; ++slot)
-        for (uint32_t s = 0; s <= MAX_STAGES; ++s) {
-            LdePassGroup<T> g;
-            g.mode = slot < 0 ? OUTER_DIF_INV : (slot == 0 ? INNER_LDE : OUTER_DIT_FWD);
-            g.s = s;
-            g.blocks = 0;
-            g.lds_bytes = 0;
-            for (const auto& ps : passes) {
-                if (ps.slot != slot || ps.s != s) continue;
-                NttWork<T> wk{};
-                wk.args = ps.A;
-                uint64_t b = finalize_args(wk.args, ps.mode, ps.s, ps.tune);
-                const uint32_t lds = pass_lds_bytes<T>(ps.mode, ps.s, wk.args.logC);
-                g.lds_bytes = lds > g.lds_bytes ? lds : g.lds_bytes;
-                wk.block_start = g.blocks;
-                g.blocks += uint32_t(b);
-                g.work.push_back(wk);
+        for (uint32_t s = 0; s <= MAX_STAGES; ++s)
+            for (int split = 0; split < 2; ++split) {
+                const int base = slot < 0 ? OUTER_DIF_INV : (slot == 0 ? INNER_LDE : OUTER_DIT_FWD);
+                const int mode = !split ? base
+                                        : (base == OUTER_DIF_INV ? OUTER_DIF_INV_SPLIT
+                                                                 : (base == OUTER_DIT_FWD ? OUTER_DIT_FWD_SPLIT : -1));
+                if (mode < 0) continue;
+                LdePassGroup<T> g;
+                g.mode = mode;
+                g.s = s;
+                g.blocks = 0;
+                g.lds_bytes = 0;
+                for (size_t i = 0; i < passes.size(); ++i) {
+                    if (passes[i].slot != slot || passes[i].s != s || kmode[i] != mode) continue;
+                    NttWork<T> wk = geo[i];
+                    const uint32_t lds = pass_lds_bytes<T>(mode, s, wk.args.logC);
+                    g.lds_bytes = lds > g.lds_bytes ? lds : g.lds_bytes;
+                    wk.block_start = g.blocks;
+                    g.blocks += uint32_t(nblk[i]);
+                    g.work.push_back(wk);
+                }
+                if (!g.work.empty()) groups.push_back(std::move(g));
             }
-            if (!g.work.empty()) groups.push_back(std::move(g));
-        }
     return groups;
 }
 

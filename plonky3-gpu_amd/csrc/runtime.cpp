@@ -455,7 +455,8 @@ const P2Params<uint64_t, 8>& p2<HGL>(p3h_ctx* ctx) {
     return ctx->glp;
 }
 
-// [omega^-e][omega^e], e < n/2, multiply-constant form
+// [omega^-e][omega^e], e < n/2, then the inverse and forward stage tables
+// (omega_{2^(q+1)}^(-+j) at (2^q - 1) + j, q < 10), multiply-constant form
 template <class H>
 const typename H::T* twiddles(p3h_ctx* ctx, uint32_t L) {
     using T = typename H::T;
@@ -463,7 +464,19 @@ const typename H::T* twiddles(p3h_ctx* ctx, uint32_t L) {
     auto it = ctx->twiddles.find(key);
     if (it != ctx->twiddles.end()) return dptr<T>(it->second);
     uint64_t half = L ? (1ull << (L - 1)) : 0;
-    DevPtr buf = dalloc(ctx, std::max<uint64_t>(1, 2 * half) * sizeof(T));
+    DevPtr buf = dalloc(ctx, twiddle_buffer_len(L) * sizeof(T));
+    std::vector<T> stages(2 * TWC_LEN);
+    for (uint32_t q = 0; (1u << (q + 1)) - 1 <= TWC_LEN; ++q) {
+        const T wf = H::root(q + 1), wi = H::inv(wf);
+        T xf = 1, xi = 1;
+        for (uint32_t j = 0; j < (1u << q); ++j) {
+            stages[(1u << q) - 1 + j] = H::c(xi);
+            stages[TWC_LEN + (1u << q) - 1 + j] = H::c(xf);
+            xf = H::mul(xf, wf);
+            xi = H::mul(xi, wi);
+        }
+    }
+    h2d(ctx, const_cast<T*>(twc_of(dptr<T>(buf), L, false)), stages.data(), stages.size() * sizeof(T));
     if (half) {
         T w = H::root(L);
         launch_check(powers_device<typename H::F>(dptr<T>(buf), half, H::c(H::inv(w)), H::c(1), H::c(1), ctx->stream),

@@ -121,6 +121,45 @@ def test_bb_lde_column_tiles(shape_ctx, orc, log_cols):
     assert (ctx.coset_lde(ctx.upload(m), 1, 31).download() == orc.lde("bb", m, 1, 31)).all()
 
 
+@pytest.mark.parametrize("f,L,ms,w,k", [("bb", 12, 6, 520, 2), ("bb", 14, 7, 300, 1), ("bb", 16, 8, 130, 1),
+                                         ("gl", 14, 7, 70, 2), ("gl", 16, 8, 40, 1)])
+def test_lde_split_exchange_passes(shape_ctx, orc, f, L, ms, w, k):
+    """Outer passes whose tile is the full 128 KiB (2^S positions x 2^(15-S)
+    BabyBear or 2^(14-S) Goldilocks columns) keep the tile in registers and
+    exchange it through LDS one column half at a time.  Every such pass width
+    from 6 to 8 stages, both outer directions (the LDE's inverse and forward
+    halves, and all four DFT kinds), ragged last column tiles, and tiles whose
+    second half lies partly or wholly past the matrix width."""
+    ctx = shape_ctx
+    ctx.set_ntt_tuning(0, ms, 0)
+    shift = 31 if f == "bb" else 7
+    m = rnd(L * 13 + w, (1 << L, w), f)
+    m[:, 0] = (BB_P if f == "bb" else GL_P) - 1
+    assert (ctx.coset_lde(ctx.upload(m), k, shift).download() == orc.lde(f, m, k, shift)).all()
+    for kind in range(4):
+        assert (ctx.dft(kind, ctx.upload(m), shift).download() == orc.dft(f, kind, m, shift)).all(), kind
+
+
+@pytest.mark.parametrize("f,w,tunes", [("bb", 45, ((0, 0, 0), (4, 10, 0), (3, 7, 0))),
+                                       ("gl", 12, ((0, 0, 0), (3, 10, 0), (2, 7, 0)))])
+def test_lde_full_tile_passes_agree_at_scale(shape_ctx, f, w, tunes):
+    """2^20 rows, one added bit: the default plan runs both outer passes at
+    10 stages on full tiles (32 BabyBear / 16 Goldilocks columns, exchanged
+    half by half; the BabyBear width leaves a 13-column last tile whose second
+    half is empty); the other two plans use narrower tiles held in LDS whole.
+    All three LDEs must be identical."""
+    ctx = shape_ctx
+    shift = 31 if f == "bb" else 7
+    m = ctx.upload(rnd(4545 + w, (1 << 20, w), f))
+    outs = []
+    for tune in tunes:
+        ctx.set_ntt_tuning(*tune)
+        lde = ctx.coset_lde(m, 1, shift)
+        outs.append(lde.download())
+        lde.free()
+    assert (outs[0] == outs[1]).all() and (outs[0] == outs[2]).all()
+
+
 @pytest.mark.parametrize("f,L,w,k", [("bb", 22, 64, 1), ("gl", 22, 8, 2)])
 def test_lde_round_trip_at_full_height(ctx, f, L, w, k):
     """Full-height property check (no oracle at this size): LDE row i * 2^k is
