@@ -583,6 +583,19 @@ def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(p3hip.BABYBEAR, consts, 8, 13)
 
+    def h2d_rate(a):  # best of 3 uploads, bytes/s
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            m = ctx.upload(a)
+            best = min(best, time.perf_counter() - t0)
+            m.free()
+        return a.nbytes / best
+
+    big = max(range(pool), key=lambda i: pageable[i].nbytes)
+    h2d = {"bytes": int(pageable[big].nbytes), "pinned_GBps": h2d_rate(pinned[big]) / 1e9,
+           "pageable_GBps": h2d_rate(pageable[big]) / 1e9}
+
     def plain_run(arrivals):
         lat, start = [], time.perf_counter()
         for k, (a, i) in enumerate(zip(arrivals, order)):
@@ -642,7 +655,8 @@ def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 
     _, burst_pageable = burst()
     traces = pinned
     span_p, burst_pinned = burst()
-    res = {"budget_gib": budget_gib, "committed_elems": elems, "host_buffers": "pinned",
+    res = {"budget_gib": budget_gib, "committed_elems": elems, "host_buffers": "pinned", "h2d": h2d,
+           "uploaded_bytes": int(sum(pageable[i].nbytes for i in order)),
            "burst": burst_pinned, "burst_pageable": burst_pageable, "rates": []}
     svc = span_p / n_req  # plain per-request service time
     for factor in (0.5, 1.0, 1.6):

@@ -160,6 +160,25 @@ def test_lde_full_tile_passes_agree_at_scale(shape_ctx, f, w, tunes):
     assert (outs[0] == outs[1]).all() and (outs[0] == outs[2]).all()
 
 
+def test_lde_region_of_4_gib(shape_ctx):
+    """2^20 x 1024 BabyBear: each 4 GiB coset region is past the 32-bit byte
+    offsets of the split outer passes, so the default plan runs its 10-stage
+    outer passes with the full 32-column tile in LDS; the first 512 columns
+    alone (2 GiB regions) run them split.  An LDE is column-wise, so the two
+    must agree on those columns (rows sampled at the start, middle and end)."""
+    ctx = shape_ctx
+    ctx.set_ntt_tuning(0, 0, 0)
+    rng = np.random.default_rng(4096)
+    t = rng.integers(0, BB_P, size=(1 << 20, 1024), dtype=np.uint32)
+    full = ctx.coset_lde(ctx.upload(t), 1, 31)
+    half = ctx.coset_lde(ctx.upload(np.ascontiguousarray(t[:, :512])), 1, 31)
+    del t
+    for r0 in (0, (1 << 20) - 1024, (1 << 21) - 1024):
+        assert (full.download_rows(r0, 1024)[:, :512] == half.download_rows(r0, 1024)).all(), r0
+    full.free()
+    half.free()
+
+
 @pytest.mark.parametrize("f,L,w,k", [("bb", 22, 64, 1), ("gl", 22, 8, 2)])
 def test_lde_round_trip_at_full_height(ctx, f, L, w, k):
     """Full-height property check (no oracle at this size): LDE row i * 2^k is
