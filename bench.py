@@ -99,6 +99,34 @@ def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
                       f"{dt:.2f} s on 1 host thread"}
 
 
+def cpu_baseline_threads(field: str, width: int, log_blowup: int, rp: int, threads: int = 0):
+    """The oracle restatement on several host threads (columns split for the
+    LDE, rows and tree nodes for the commit; the root equals the single-thread
+    one), on a sample 4x the single-thread one.  Threads: the 16 host cores a
+    one-GPU share of the box has, or fewer if the host has fewer."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import native  # test infrastructure; used here only as the CPU baseline
+
+    lib = native.oracle()
+    threads = threads or min(16, os.cpu_count() or 1)
+    log_n = 17 if field == "bb" else 15
+    n = 1 << log_n
+    rng = np.random.default_rng(5)
+    if field == "bb":
+        trace = rng.integers(0, BB_P, size=(n, width), dtype=np.uint32)
+    else:
+        trace = rng.integers(0, GL_P, size=(n, width), dtype=np.uint64)
+    consts = p2_constants(field, rp)
+    t0 = time.perf_counter()
+    lib.lde_commit_mt(field, consts, 8, rp, trace, log_blowup, 31 if field == "bb" else 7, threads)
+    dt = time.perf_counter() - t0
+    elems = (n << log_blowup) * width
+    name = "BabyBear" if field == "bb" else "Goldilocks"
+    return {"value": elems / dt, "unit": "committed field elems/s", "cores": threads, "kind": "port",
+            "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
+                      f"{dt:.2f} s on {threads} host threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +269,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field, width, lb, rp)
+        out["cpu_baseline_threads"] = cpu_baseline_threads(field, width, lb, rp)
     print(json.dumps(out))
     ctx.close()
 

@@ -17,6 +17,7 @@
 #include <numeric>
 #include <random>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -689,6 +690,57 @@ std::vector<E4> final_poly(std::vector<E4> cur, size_t lfpl) {
     return res;
 }
 
+// LDE + flat commit on `threads` host threads: the CPU baseline's multi-core
+// figure.  Every column's transform is independent, so the LDE runs on column
+// slabs (one per thread, lde() above on each); leaf hashes split the rows and
+// each wide tree level splits its nodes.  The arithmetic is the single-threaded
+// code's, so the root is the same (tests/test_oracle.py checks it).
+template <class Body>
+void parallel_for(size_t n, int threads, Body body) {
+    const size_t t = std::max<size_t>(1, std::min<size_t>(size_t(threads), n));
+    if (t == 1) {
+        body(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t i = 0; i < t; ++i) pool.emplace_back(body, n * i / t, n * (i + 1) / t);
+    for (auto& th : pool) th.join();
+}
+
+template <class F>
+Digest<F> lde_commit_mt(const typename F::Raw* c, int rf, int rp, const typename F::Raw* in, size_t log_h,
+                        size_t w, size_t added, typename F::Raw shift, int threads) {
+    using R = typename F::Raw;
+    if (w == 0) throw std::invalid_argument("empty matrix");
+    const size_t h = size_t(1) << log_h, N = h << added;
+    std::vector<R> lde_m(N * w);
+    parallel_for(w, threads, [&](size_t c0, size_t c1) {
+        if (c0 == c1) return;
+        const size_t wc = c1 - c0;
+        std::vector<R> slab(h * wc);
+        for (size_t i = 0; i < h; ++i) std::copy(in + i * w + c0, in + i * w + c1, slab.begin() + i * wc);
+        Mat<F> m = lde<F>(slab.data(), log_h, wc, added, shift);
+        for (size_t i = 0; i < N; ++i) std::copy(m.row(i), m.row(i) + wc, lde_m.begin() + i * w + c0);
+    });
+    const Perm<F, Hcfg<F>::W> perm(c, rf, rp);
+    std::vector<Digest<F>> layer(N);
+    parallel_for(N, threads, [&](size_t r0, size_t r1) {
+        for (size_t i = r0; i < r1; ++i) layer[i] = sponge<F>(perm, lde_m.data() + i * w, w);
+    });
+    while (layer.size() > 1) {
+        std::vector<Digest<F>> nxt((layer.size() + 1) / 2);
+        parallel_for(nxt.size(), nxt.size() >= 4096 ? threads : 1, [&](size_t j0, size_t j1) {
+            for (size_t j = j0; j < j1; ++j) {
+                Digest<F> r{};
+                if (2 * j + 1 < layer.size()) r = layer[2 * j + 1];
+                nxt[j] = compress<F>(perm, layer[2 * j], r);
+            }
+        });
+        layer = std::move(nxt);
+    }
+    return layer[0];
+}
+
 template <class Fn>
 int guarded(Fn&& f) {
     try {
@@ -860,6 +912,23 @@ int orc_gl_flat_commit(const uint64_t* c, int rf, int rp, const uint64_t* d, siz
         return 0;
     });
 }
+int orc_bb_lde_commit_mt(const uint32_t* c, int rf, int rp, const uint32_t* in, size_t log_h, size_t w,
+                         size_t added, uint32_t shift, int threads, uint32_t* root) {
+    return guarded([&] {
+        auto d = lde_commit_mt<BB>(c, rf, rp, in, log_h, w, added, shift % BB::P, threads);
+        std::copy(d.begin(), d.end(), root);
+        return 0;
+    });
+}
+int orc_gl_lde_commit_mt(const uint64_t* c, int rf, int rp, const uint64_t* in, size_t log_h, size_t w,
+                         size_t added, uint64_t shift, int threads, uint64_t* root) {
+    return guarded([&] {
+        auto d = lde_commit_mt<GL>(c, rf, rp, in, log_h, w, added, shift % GL::P, threads);
+        std::copy(d.begin(), d.end(), root);
+        return 0;
+    });
+}
+
 // MerkleTreeMmcs::open_row (fri_merkle_tree_mmcs.hpp:129-141): siblings, with a
 // zero digest where the sibling falls past an odd layer's end.
 int orc_bb_flat_open(const uint32_t* c, int rf, int rp, const uint32_t* d, size_t h, size_t w,

@@ -65,6 +65,13 @@ int orc_bb_flat_commit(const uint32_t* consts, int rf, int rp, const uint32_t* d
                        size_t width, uint32_t* root8, uint32_t* tree_out);
 int orc_gl_flat_commit(const uint64_t* consts, int rf, int rp, const uint64_t* data, size_t height,
                        size_t width, uint64_t* root4, uint64_t* tree_out);
+/* LDE + flat commit on `threads` host threads (column slabs for the LDE, row
+ * and node ranges for the tree): the same root as orc_*_lde then
+ * orc_*_flat_commit; the CPU baseline's multi-core figure */
+int orc_bb_lde_commit_mt(const uint32_t* consts, int rf, int rp, const uint32_t* in, size_t log_h, size_t w,
+                         size_t added, uint32_t shift, int threads, uint32_t* root8);
+int orc_gl_lde_commit_mt(const uint64_t* consts, int rf, int rp, const uint64_t* in, size_t log_h, size_t w,
+                         size_t added, uint64_t shift, int threads, uint64_t* root4);
 /* single-matrix opening: row values and log2(height) sibling digests */
 int orc_bb_flat_open(const uint32_t* consts, int rf, int rp, const uint32_t* data, size_t height,
                      size_t width, size_t index, uint32_t* row_out, uint32_t* proof_out);

@@ -126,6 +126,18 @@ class Native:
             cnt = (cnt + 1) // 2
         return root, layers
 
+    def lde_commit_mt(self, f, consts, rf, rp, mat, added, shift, threads):
+        """Root of the flat commit of the LDE, computed on `threads` host threads."""
+        m = np.ascontiguousarray(mat, dtype=DT[f])
+        h, w = m.shape
+        c = np.ascontiguousarray(consts, dtype=DT[f])
+        root = np.zeros(DIG[f], dtype=DT[f])
+        sh = C.c_uint32(shift) if f == "bb" else C.c_uint64(shift)
+        self._rc(self.fn(f"{f}_lde_commit_mt")(_p(c), C.c_int(rf), C.c_int(rp), _p(m), C.c_size_t(h.bit_length() - 1),
+                                               C.c_size_t(w), C.c_size_t(added), sh, C.c_int(threads), _p(root)),
+                 "lde_commit_mt")
+        return root
+
     def mmcs(self, f, consts, rf, rp, mats, cap_height=0, open_index=-1):
         ms = [np.ascontiguousarray(m, dtype=DT[f]) for m in mats]
         n = len(ms)

@@ -12,6 +12,7 @@ What pins each oracle function (a plausible mistake in it fails at least one):
 |                           | brute-force DFT; idft(dft(x)) == x                                         |
 | coset LDE (BB and GL)     | golden vectors; O(n*N) polynomial evaluation (independent of the FFT)      |
 | flat MMCS commit / open   | golden roots (odd heights exercise the zero-digest padding)               |
+| threaded LDE + commit     | equals the single-threaded LDE and commit (baseline use only)             |
 | multi-height MMCS         | golden caps, openings and proofs (cap 0/1, height ties, injection)         |
 | verify_batch              | accepts golden openings, rejects single-element tampering                 |
 | FRI fold                  | golden folds (arity 2/4/8); arity-2 closed form in pure Python integers    |
@@ -161,6 +162,20 @@ def test_flat_commit_golden(orc, golden):
     for i in range(3):
         assert (orc.flat_commit("gl", golden["consts_gl"], RF, 22, golden[f"flat_gl_{i}_in"])
                 == golden[f"flat_gl_{i}_root"]).all(), i
+
+
+@pytest.mark.parametrize("f,lh,w,added,threads", [("bb", 8, 20, 1, 3), ("bb", 13, 37, 2, 16), ("bb", 3, 1, 1, 8),
+                                                   ("gl", 7, 9, 2, 4), ("gl", 12, 5, 1, 16)])
+def test_threaded_lde_commit_matches_single_thread(orc, golden, f, lh, w, added, threads):
+    """The multi-core CPU baseline computes the same root as the single-threaded
+    LDE and flat commit (more threads than columns, a single column, a tree
+    wider than the per-level threading cut-off)."""
+    consts, rp = (golden["consts_bb"], 13) if f == "bb" else (golden["consts_gl"], 22)
+    rng = np.random.default_rng(lh * 10 + w)
+    m = rng.integers(0, BB_P if f == "bb" else GL_P, size=(1 << lh, w), dtype=U32 if f == "bb" else U64)
+    shift = 31 if f == "bb" else 7
+    want = orc.flat_commit(f, consts, RF, rp, orc.lde(f, m, added, shift))
+    assert (orc.lde_commit_mt(f, consts, RF, rp, m, added, shift, threads) == want).all()
 
 
 def test_flat_open_consistent_with_tree(orc, golden):
