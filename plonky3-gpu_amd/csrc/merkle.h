@@ -43,11 +43,23 @@ template <class D>
 int permute_device(typename D::T* states, uint64_t n, const P2Params<typename D::T, D::W>& p,
                    hipStream_t st);
 
-// Gather opening data: for each of nq query indices, write the sibling digest of
-// each layer (layers given by device pointer table + counts).
+// out[q * total_w + c] = column c of the concatenation over segs of row rows[q]
+// (each segment through its row map; a null segment reads as zero).
 template <class T>
 int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows,
                        uint32_t nq, T* out, hipStream_t st);
+
+// A batch of Merkle openings in one launch.  Rows: vals[q * total_w + c] =
+// column c of the concatenated segment rows, segment s read at row
+// rows[q * nsegs + s] through its row map (a null segment reads as zero).
+// Proofs: proof[(q * steps + l) * dig + d] = word d of digest sib[q * steps + l]
+// of layers[l], or zero where that entry is ~0 (past an odd layer's end).
+// segs, rows, layers (a device table of layer pointers) and sib are in device
+// memory; total_w or steps may be zero.
+template <class T>
+int open_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows,
+                     const T* const* layers, uint32_t steps, uint32_t dig, const uint64_t* sib, uint32_t nq, T* vals,
+                     T* proof, hipStream_t st);
 
 // BabyBear kernels with the binary64 permutation (poseidon2_f64.h); same
 // contracts as the uint32 versions above.

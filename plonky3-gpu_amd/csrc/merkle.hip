@@ -162,6 +162,31 @@ __global__ void gather_rows_kernel(const RowSeg<T>* segs, uint32_t nsegs, uint32
     out[idx] = s.ptr ? s.ptr[stored_row(s, rows[q]) * s.w + c] : T(0);
 }
 
+// One thread per output word of a batch of openings (open_rows_device).
+template <class T>
+__global__ void open_rows_kernel(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows,
+                                 const T* const* layers, uint32_t steps, uint32_t dig, const uint64_t* sib,
+                                 uint32_t nq, T* vals, T* proof) {
+    const uint32_t per_q = total_w + steps * dig;
+    const uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= uint64_t(nq) * per_q) return;
+    const uint32_t q = uint32_t(idx / per_q), k = uint32_t(idx % per_q);
+    if (k < total_w) {
+        uint32_t seg = 0, c = k;
+        while (seg + 1 < nsegs && c >= segs[seg].w) {
+            c -= segs[seg].w;
+            ++seg;
+        }
+        const RowSeg<T>& s = segs[seg];
+        vals[uint64_t(q) * total_w + k] =
+            s.ptr ? s.ptr[stored_row(s, rows[uint64_t(q) * nsegs + seg]) * s.w + c] : T(0);
+    } else {
+        const uint32_t r = k - total_w, l = r / dig, d = r % dig;
+        const uint64_t si = sib[uint64_t(q) * steps + l];
+        proof[(uint64_t(q) * steps + l) * dig + d] = si == ~0ull ? T(0) : layers[l][si * dig + d];
+    }
+}
+
 
 // ---------------------------------------------------------------- FP64 ----
 // The same sponge / compression with the BabyBear permutation evaluated in
@@ -512,6 +537,17 @@ int gather_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, 
     return int(hipGetLastError());
 }
 
+template <class T>
+int open_rows_device(const RowSeg<T>* segs, uint32_t nsegs, uint32_t total_w, const uint64_t* rows,
+                     const T* const* layers, uint32_t steps, uint32_t dig, const uint64_t* sib, uint32_t nq, T* vals,
+                     T* proof, hipStream_t st) {
+    const uint64_t total = uint64_t(nq) * (total_w + steps * dig);
+    if (total == 0) return 0;
+    hipLaunchKernelGGL((open_rows_kernel<T>), dim3(grid_for(total, 256)), dim3(256), 0, st, segs, nsegs, total_w,
+                       rows, layers, steps, dig, sib, nq, vals, proof);
+    return int(hipGetLastError());
+}
+
 
 static bool f64_lanes_form(uint32_t nsegs, const RowSeg<uint32_t>& seg0, uint64_t rows) {
     return nsegs == 1 && seg0.ptr && seg0.w > 0 && rows <= LANE_MAX;
@@ -635,7 +671,10 @@ int compress_multi_f64_device(const MultiJob* jobs, uint32_t n, const P2ParamsF6
                                           uint32_t, const P2Params<D::T, D::W>&, hipStream_t);                  \
     template int permute_device<D>(D::T*, uint64_t, const P2Params<D::T, D::W>&, hipStream_t);                  \
     template int gather_rows_device<D::T>(const RowSeg<D::T>*, uint32_t, uint32_t, const uint64_t*, uint32_t,   \
-                                          D::T*, hipStream_t);
+                                          D::T*, hipStream_t);                                                     \
+    template int open_rows_device<D::T>(const RowSeg<D::T>*, uint32_t, uint32_t, const uint64_t*,               \
+                                        const D::T* const*, uint32_t, uint32_t, const uint64_t*, uint32_t, D::T*, \
+                                        D::T*, hipStream_t);
 P3H_INST(BBDom)
 P3H_INST(GLDom)
 #undef P3H_INST

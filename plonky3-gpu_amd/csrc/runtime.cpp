@@ -933,6 +933,11 @@ std::vector<p3h_tree*> flat_trees_multi(p3h_ctx* ctx, const std::vector<p3h_mat*
     return out;
 }
 
+// MerkleTreeMmcs::open_batch (merkle_tree_mmcs.hpp:120-160) and the flat
+// open_row (fri_merkle_tree_mmcs.hpp:129-141) for a batch of indices: the rows
+// of every committed matrix at the index (scaled to each matrix's height) and
+// the sibling digest of every layer below the top, zero where the sibling falls
+// past an odd layer's end.  One table upload, one launch, one copy back.
 template <class H>
 void tree_open(p3h_ctx* ctx, const p3h_tree* t, const uint64_t* idx, uint32_t nq, typename H::T* values,
                typename H::T* proof) {
@@ -942,73 +947,67 @@ void tree_open(p3h_ctx* ctx, const p3h_tree* t, const uint64_t* idx, uint32_t nq
     for (uint32_t q = 0; q < nq; ++q)
         if (idx[q] >= maxh) throw Err{P3H_EINVAL, "opening index out of range"};
     if (nq == 0) return;
-    DevPtr didx = dalloc(ctx, nq * sizeof(uint64_t));
-    std::vector<uint64_t> rows(nq);
-    // values
+    // rows: one segment per committed matrix (or the flat tree's segments) and
+    // the row each query reads in it
+    std::vector<RowSeg<T>> segs;
+    std::vector<uint64_t> rows;
+    uint32_t total_w = 0;
     if (values) {
-        std::vector<std::vector<T>> parts;
-        std::vector<RowSeg<T>> segs;
         if (t->flat) {
             segs.resize(t->flat_segs_host.size() / sizeof(RowSeg<T>));
             std::memcpy(segs.data(), t->flat_segs_host.data(), t->flat_segs_host.size());
-            for (uint32_t q = 0; q < nq; ++q) rows[q] = idx[q];
-            h2d(ctx, didx->ptr, rows.data(), nq * sizeof(uint64_t));
-            DevPtr ds = upload_segs<H>(ctx, segs);
-            DevPtr out = dalloc(ctx, uint64_t(nq) * t->total_width * sizeof(T));
-            launch_check(gather_rows_device<T>(dptr<RowSeg<T>>(ds), uint32_t(segs.size()), t->total_width,
-                                               dptr<uint64_t>(didx), nq, dptr<T>(out), ctx->stream),
-                         "gather rows");
-            d2h(ctx, values, out->ptr, uint64_t(nq) * t->total_width * sizeof(T));
+            total_w = t->total_width;
         } else {
-            std::vector<T> host(uint64_t(nq) * t->total_width);
-            uint32_t col = 0;
             for (size_t m = 0; m < t->mats.size(); ++m) {
-                uint32_t w = t->widths[m];
-                uint64_t stride = maxh / t->heights[m];
-                for (uint32_t q = 0; q < nq; ++q) rows[q] = idx[q] / stride;
-                h2d(ctx, didx->ptr, rows.data(), nq * sizeof(uint64_t));
-                std::vector<RowSeg<T>> one{seg_of<H>(t->mats[m])};
-                DevPtr ds = upload_segs<H>(ctx, one);
-                DevPtr out = dalloc(ctx, std::max<uint64_t>(1, uint64_t(nq) * w) * sizeof(T));
-                launch_check(gather_rows_device<T>(dptr<RowSeg<T>>(ds), 1, w, dptr<uint64_t>(didx), nq, dptr<T>(out),
-                                                   ctx->stream),
-                             "gather rows");
-                std::vector<T> part(uint64_t(nq) * w);
-                d2h(ctx, part.data(), out->ptr, part.size() * sizeof(T));
-                for (uint32_t q = 0; q < nq; ++q)
-                    std::memcpy(&host[uint64_t(q) * t->total_width + col], &part[uint64_t(q) * w], w * sizeof(T));
-                col += w;
+                segs.push_back(seg_of<H>(t->mats[m]));
+                total_w += t->widths[m];
             }
-            std::memcpy(values, host.data(), host.size() * sizeof(T));
+        }
+        rows.resize(uint64_t(nq) * segs.size());
+        for (uint32_t q = 0; q < nq; ++q)
+            for (size_t m = 0; m < segs.size(); ++m)
+                rows[q * segs.size() + m] = t->flat ? idx[q] : idx[q] / (maxh / t->heights[m]);
+    }
+    const uint32_t steps = proof ? uint32_t(t->layers.size() - 1) : 0;
+    std::vector<const T*> lptr(steps);
+    std::vector<uint64_t> sib(uint64_t(nq) * steps);
+    for (uint32_t l = 0; l < steps; ++l) {
+        lptr[l] = dptr<T>(t->layers[l]);
+        for (uint32_t q = 0; q < nq; ++q) {
+            const uint64_t s = (idx[q] >> l) ^ 1;
+            sib[uint64_t(q) * steps + l] = s < t->counts[l] ? s : ~0ull;
         }
     }
-    // proofs: sibling of each layer below the top
-    if (proof) {
-        size_t steps = t->layers.size() - 1;
-        std::vector<T> host(uint64_t(nq) * steps * DIG, T(0));
-        for (size_t s = 0; s < steps; ++s) {
-            std::vector<uint64_t> sib(nq);
-            std::vector<bool> valid(nq);
-            for (uint32_t q = 0; q < nq; ++q) {
-                uint64_t sidx = (idx[q] >> s) ^ 1;
-                valid[q] = sidx < t->counts[s];
-                sib[q] = valid[q] ? sidx : 0;
-            }
-            h2d(ctx, didx->ptr, sib.data(), nq * sizeof(uint64_t));
-            std::vector<RowSeg<T>> one{RowSeg<T>{dptr<T>(t->layers[s]), ~0ull, DIG, 0, 0, 0}};
-            DevPtr ds = upload_segs<H>(ctx, one);
-            DevPtr out = dalloc(ctx, uint64_t(nq) * DIG * sizeof(T));
-            launch_check(gather_rows_device<T>(dptr<RowSeg<T>>(ds), 1, DIG, dptr<uint64_t>(didx), nq, dptr<T>(out),
-                                               ctx->stream),
-                         "gather proof");
-            std::vector<T> part(uint64_t(nq) * DIG);
-            d2h(ctx, part.data(), out->ptr, part.size() * sizeof(T));
-            for (uint32_t q = 0; q < nq; ++q)
-                if (valid[q])
-                    std::memcpy(&host[(uint64_t(q) * steps + s) * DIG], &part[uint64_t(q) * DIG], DIG * sizeof(T));
-        }
-        std::memcpy(proof, host.data(), host.size() * sizeof(T));
-    }
+    if (total_w == 0 && steps == 0) return;
+    // one table: segments | rows | layer pointers | siblings (16-byte aligned parts)
+    auto up16 = [](size_t b) { return (b + 15) & ~size_t(15); };
+    const size_t o_rows = up16(segs.size() * sizeof(RowSeg<T>));
+    const size_t o_lay = o_rows + up16(rows.size() * sizeof(uint64_t));
+    const size_t o_sib = o_lay + up16(lptr.size() * sizeof(const T*));
+    const size_t tbytes = o_sib + sib.size() * sizeof(uint64_t);
+    std::vector<uint8_t> tab(tbytes, 0);
+    std::memcpy(tab.data(), segs.data(), segs.size() * sizeof(RowSeg<T>));
+    std::memcpy(tab.data() + o_rows, rows.data(), rows.size() * sizeof(uint64_t));
+    std::memcpy(tab.data() + o_lay, lptr.data(), lptr.size() * sizeof(const T*));
+    std::memcpy(tab.data() + o_sib, sib.data(), sib.size() * sizeof(uint64_t));
+    DevPtr dtab = dalloc(ctx, tbytes);
+    h2d(ctx, dtab->ptr, tab.data(), tbytes);
+    const uint64_t nvals = uint64_t(nq) * total_w, nproof = uint64_t(nq) * steps * DIG;
+    DevPtr out = dalloc(ctx, (nvals + nproof) * sizeof(T));
+    uint8_t* base = static_cast<uint8_t*>(dtab->ptr);
+    launch_check(open_rows_device<T>(reinterpret_cast<const RowSeg<T>*>(base), uint32_t(segs.size()), total_w,
+                                     reinterpret_cast<const uint64_t*>(base + o_rows),
+                                     reinterpret_cast<const T* const*>(base + o_lay), steps, DIG,
+                                     reinterpret_cast<const uint64_t*>(base + o_sib), nq, dptr<T>(out),
+                                     dptr<T>(out) + nvals, ctx->stream),
+                 "open rows");
+    if (nvals)
+        hip_check(hipMemcpyAsync(values, dptr<T>(out), nvals * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
+                  "hipMemcpyAsync D2H");
+    if (nproof)
+        hip_check(hipMemcpyAsync(proof, dptr<T>(out) + nvals, nproof * sizeof(T), hipMemcpyDeviceToHost, ctx->stream),
+                  "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
 // ------------------------------------------------------------------- FRI ---
