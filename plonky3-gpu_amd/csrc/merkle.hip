@@ -257,6 +257,40 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_dense_f64_kernel(RowSe
     for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);
 }
 
+// One matrix of any width: the dense kernel's schedule (the next chunk's loads
+// issued before the current chunk's permutation) with 4-byte loads, since rows
+// of a width that is not a multiple of 4 are not 16-byte aligned.  A last
+// partial chunk overwrites only its own rate slots (PaddingFreeSponge).
+__global__ __launch_bounds__(HASH_THREADS) void hash_rows_ragged_f64_kernel(RowSeg<uint32_t> seg, uint64_t rows,
+                                                                            uint32_t* out, P2ParamsF64 p) {
+    uint64_t j = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
+    if (j >= rows) return;
+    double s[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = 0.0;
+    const uint32_t w = seg.w;
+    const uint32_t* rowp = seg.ptr + stored_row(seg, j) * w;
+    uint32_t nx[8];
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) nx[q] = q < w ? rowp[q] : 0u;
+    for (uint32_t start = 0; start < w; start += 8) {
+        uint32_t cur[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) cur[q] = nx[q];
+        const uint32_t n = w - start;  // words left, this chunk takes min(n, 8)
+        if (n > 8) {
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) nx[q] = start + 8 + q < w ? rowp[start + 8 + q] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q)
+            if (q < n) s[q] = double(cur[q]);
+        Poseidon2F64::permute(s, p);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);
+}
+
 __global__ __launch_bounds__(HASH_THREADS) void compress_f64_kernel(const uint32_t* prev, uint64_t prev_count,
                                                                     uint32_t* next, uint64_t next_count,
                                                                     const RowSeg<uint32_t>* inj, uint32_t inj_w,
@@ -556,8 +590,12 @@ static bool f64_dense_form(uint32_t nsegs, const RowSeg<uint32_t>& seg0) {
     return nsegs == 1 && seg0.ptr && seg0.w > 0 && seg0.w % 8 == 0 && (reinterpret_cast<uintptr_t>(seg0.ptr) & 15) == 0;
 }
 
+static bool f64_ragged_form(uint32_t nsegs, const RowSeg<uint32_t>& seg0) {
+    return nsegs == 1 && seg0.ptr && seg0.w > 0;
+}
+
 bool hash_rows_f64_reads_table(uint32_t nsegs, const RowSeg<uint32_t>& seg0, uint64_t rows) {
-    return !f64_lanes_form(nsegs, seg0, rows) && !f64_dense_form(nsegs, seg0);
+    return !f64_lanes_form(nsegs, seg0, rows) && !f64_dense_form(nsegs, seg0) && !f64_ragged_form(nsegs, seg0);
 }
 
 int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t total_w, const RowSeg<uint32_t>& seg0,
@@ -570,6 +608,9 @@ int hash_rows_f64_device(const RowSeg<uint32_t>* segs, uint32_t nsegs, uint32_t 
     }
     if (f64_dense_form(nsegs, seg0))
         hipLaunchKernelGGL(hash_rows_dense_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st,
+                           seg0, rows, out, p);
+    else if (f64_ragged_form(nsegs, seg0))
+        hipLaunchKernelGGL(hash_rows_ragged_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st,
                            seg0, rows, out, p);
     else if (segs)
         hipLaunchKernelGGL(hash_rows_f64_kernel, dim3(grid_for(rows, HASH_THREADS)), dim3(HASH_THREADS), 0, st, segs,

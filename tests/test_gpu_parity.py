@@ -240,6 +240,21 @@ def test_flat_commit_and_open_vs_oracle(ctx, golden, orc, h, w):
         assert (proof[q] == pr).all()
 
 
+@pytest.mark.parametrize("w", [5, 12, 13, 61])
+def test_single_matrix_leaves_of_any_width(ctx, golden, orc, w):
+    """Leaves of one matrix taller than the lane-parallel cut-off (2^15 rows)
+    at widths that are not a multiple of 8 (12 rows are 16-byte aligned, the
+    others are not), straight and through an LDE's coset row map."""
+    d = rnd(700 + w, (1 << 16, w))
+    want = orc.flat_commit("bb", golden["consts_bb"], RF, 13, d)
+    assert (ctx.flat_commit(ctx.upload(d)).root() == want).all()
+    assert (ctx.mmcs_commit([ctx.upload(d)], 0).root() == want).all()
+    t = rnd(800 + w, (1 << 15, w))
+    lde = ctx.coset_lde(ctx.upload(t), 1, 31)
+    want = orc.flat_commit("bb", golden["consts_bb"], RF, 13, orc.lde("bb", t, 1, 31))
+    assert (ctx.mmcs_commit([lde], 0).root() == want).all()
+
+
 def test_mmcs_golden(ctx, golden):
     for s in range(4):
         mats = mats_of(golden, f"mm_bb_{s}")
