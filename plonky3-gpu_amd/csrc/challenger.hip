@@ -24,6 +24,7 @@ constexpr uint32_t BB_PRIME = 0x78000001u;
 __global__ __launch_bounds__(64) void chal_run_kernel(DevChal* st, ChalProgram prog, P2ParamsF64 p) {
     __shared__ uint32_t inb[8], outb[8];
     const uint32_t lane = threadIdx.x;
+    const auto pc = Poseidon2F64Lanes::consts(p, lane & 15);
     double s = lane < 16 ? double(st->sponge[lane]) : 0.0;
     if (lane < 8) {
         inb[lane] = st->input[lane];
@@ -34,7 +35,7 @@ __global__ __launch_bounds__(64) void chal_run_kernel(DevChal* st, ChalProgram p
 
     auto duplex = [&]() {
         if (lane < in_len) s = double(inb[lane]);
-        s = Poseidon2F64Lanes::permute(s, p, lane & 15);
+        s = Poseidon2F64Lanes::permute(s, pc, lane & 15);
         const uint32_t v = BBF64::canonical(s);
         s = double(v);
         if (lane < 8) outb[lane] = v;

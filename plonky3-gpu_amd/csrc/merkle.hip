@@ -340,26 +340,28 @@ __global__ __launch_bounds__(LANE_THREADS) void compress_lanes_f64_kernel(const 
                                                                           uint32_t* next, uint64_t next_count,
                                                                           P2ParamsF64 p) {
     const uint32_t lane = threadIdx.x & 15;
+    const auto pc = Poseidon2F64Lanes::consts(p, lane);
     const uint64_t node = (uint64_t(blockIdx.x) * LANE_THREADS + threadIdx.x) >> 4;
     double s = 0.0;
     if (node < next_count) {
         const uint64_t child = 2 * node + (lane >> 3);  // lanes 0-7 left digest, 8-15 right digest
         if (child < prev_count) s = double(prev[child * 8 + (lane & 7)]);
     }
-    s = Poseidon2F64Lanes::permute(s, p, lane);
+    s = Poseidon2F64Lanes::permute(s, pc, lane);
     if (node < next_count && lane < 8) next[node * 8 + lane] = BBF64::canonical(s);
 }
 
 __global__ __launch_bounds__(LANE_THREADS) void hash_rows_lanes_f64_kernel(RowSeg<uint32_t> seg, uint64_t rows,
                                                                            uint32_t* out, P2ParamsF64 p) {
     const uint32_t lane = threadIdx.x & 15;
+    const auto pc = Poseidon2F64Lanes::consts(p, lane);
     const uint64_t j = (uint64_t(blockIdx.x) * LANE_THREADS + threadIdx.x) >> 4;
     const bool live = j < rows;
     const uint32_t* rowp = live ? seg.ptr + stored_row(seg, j) * seg.w : nullptr;
     double s = 0.0;
     for (uint32_t start = 0; start < seg.w; start += 8) {
         if (live && lane < 8 && start + lane < seg.w) s = double(rowp[start + lane]);  // overwrite mode
-        s = Poseidon2F64Lanes::permute(s, p, lane);
+        s = Poseidon2F64Lanes::permute(s, pc, lane);
     }
     if (live && lane < 8) out[j * 8 + lane] = BBF64::canonical(s);
 }
@@ -381,6 +383,7 @@ struct TailLevels {
 __global__ __launch_bounds__(TAIL_THREADS) void compress_tail_lanes_f64_kernel(TailLevels t, P2ParamsF64 p) {
     __shared__ uint32_t buf[2][64 * 8];
     const uint32_t lane = threadIdx.x & 15;
+    const auto pc = Poseidon2F64Lanes::consts(p, lane);
     const uint32_t node = threadIdx.x >> 4;  // < 64
     uint32_t prev_count = uint32_t(t.in_count);
     for (uint32_t l = 0; l < t.nlev; ++l) {
@@ -391,7 +394,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void compress_tail_lanes_f64_kernel(T
             if (child < prev_count)
                 s = double(l == 0 ? t.in[uint64_t(child) * 8 + (lane & 7)] : buf[(l - 1) & 1][child * 8 + (lane & 7)]);
         }
-        s = Poseidon2F64Lanes::permute(s, p, lane);
+        s = Poseidon2F64Lanes::permute(s, pc, lane);
         if (node < cnt && lane < 8) {
             const uint32_t v = BBF64::canonical(s);
             buf[l & 1][node * 8 + lane] = v;
@@ -404,9 +407,10 @@ __global__ __launch_bounds__(TAIL_THREADS) void compress_tail_lanes_f64_kernel(T
 
 __global__ __launch_bounds__(LANE_THREADS) void permute_lanes_f64_kernel(uint32_t* st, uint64_t n, P2ParamsF64 p) {
     const uint32_t lane = threadIdx.x & 15;
+    const auto pc = Poseidon2F64Lanes::consts(p, lane);
     const uint64_t i = (uint64_t(blockIdx.x) * LANE_THREADS + threadIdx.x) >> 4;
     double s = i < n ? double(st[i * 16 + lane]) : 0.0;
-    s = Poseidon2F64Lanes::permute(s, p, lane);
+    s = Poseidon2F64Lanes::permute(s, pc, lane);
     if (i < n) st[i * 16 + lane] = BBF64::canonical(s);
 }
 
@@ -457,6 +461,7 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_multi_f64_kernel(MultiTable
 __global__ __launch_bounds__(LANE_THREADS) void hash_multi_lanes_f64_kernel(MultiTable m, P2ParamsF64 p) {
     const uint32_t t = job_of(m, blockIdx.x);
     const uint32_t lane = threadIdx.x & 15;
+    const auto pc = Poseidon2F64Lanes::consts(p, lane);
     const uint64_t j = uint64_t(blockIdx.x - m.block0[t]) * (LANE_THREADS / 16) + (threadIdx.x >> 4);
     const bool live = j < m.out_count[t];
     const uint32_t w = m.w[t];
@@ -464,7 +469,7 @@ __global__ __launch_bounds__(LANE_THREADS) void hash_multi_lanes_f64_kernel(Mult
     double s = 0.0;
     for (uint32_t start = 0; start < w; start += 8) {  // w is uniform: every lane runs every permutation
         if (live && lane < 8) s = double(rowp[start + lane]);
-        s = Poseidon2F64Lanes::permute(s, p, lane);
+        s = Poseidon2F64Lanes::permute(s, pc, lane);
     }
     if (live && lane < 8) m.out[t][j * 8 + lane] = BBF64::canonical(s);
 }
@@ -492,6 +497,7 @@ __global__ __launch_bounds__(HASH_THREADS) void compress_multi_f64_kernel(MultiT
 __global__ __launch_bounds__(LANE_THREADS) void compress_multi_lanes_f64_kernel(MultiTable m, P2ParamsF64 p) {
     const uint32_t t = job_of(m, blockIdx.x);
     const uint32_t lane = threadIdx.x & 15;
+    const auto pc = Poseidon2F64Lanes::consts(p, lane);
     const uint64_t node = uint64_t(blockIdx.x - m.block0[t]) * (LANE_THREADS / 16) + (threadIdx.x >> 4);
     const bool live = node < m.out_count[t];
     double s = 0.0;
@@ -499,7 +505,7 @@ __global__ __launch_bounds__(LANE_THREADS) void compress_multi_lanes_f64_kernel(
         const uint64_t child = 2 * node + (lane >> 3);  // lanes 0-7 left digest, 8-15 right digest
         if (child < m.in_count[t]) s = double(m.in[t][child * 8 + (lane & 7)]);
     }
-    s = Poseidon2F64Lanes::permute(s, p, lane);
+    s = Poseidon2F64Lanes::permute(s, pc, lane);
     if (live && lane < 8) m.out[t][node * 8 + lane] = BBF64::canonical(s);
 }
 

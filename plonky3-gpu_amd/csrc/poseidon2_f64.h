@@ -200,15 +200,45 @@ struct Poseidon2F64Lanes {
         t = t + dpp<ROR + 1>(t);
         return BBF64::reduce(__builtin_fma(s, diag, t));
     }
-    __device__ __forceinline__ static double permute(double s, const P2ParamsF64& p, uint32_t lane) {
-        const double diag = double(1u << lane);
-        s = mds_light(s);
-        for (uint32_t r = 0; r < p.half_full; ++r) s = mds_light(Poseidon2F64::sbox(s + p.init[r][lane]));
-        for (uint32_t r = 0; r < p.partial; ++r) {
-            const double t = Poseidon2F64::sbox(s + p.internal[r]);
-            s = internal_layer(lane == 0 ? t : s, diag);
+    // One lane's round constants, loaded once per kernel: a permutation here is
+    // a chain of dependent instructions, and a constant load inside a round
+    // would put its memory latency on that chain every round.  Full-round
+    // constants are per lane (one state word each), partial-round ones shared.
+    struct Consts {
+        double init[4], term[4], internal[32];
+        uint32_t half_full, partial;
+        double diag;
+    };
+    __device__ __forceinline__ static Consts consts(const P2ParamsF64& p, uint32_t lane) {
+        Consts c;
+        c.half_full = p.half_full;
+        c.partial = p.partial;
+        c.diag = double(1u << lane);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            c.init[r] = p.init[r][lane];
+            c.term[r] = p.term[r][lane];
         }
-        for (uint32_t r = 0; r < p.half_full; ++r) s = mds_light(Poseidon2F64::sbox(s + p.term[r][lane]));
+#pragma unroll
+        for (int r = 0; r < 32; ++r) c.internal[r] = p.internal[r];
+        return c;
+    }
+    // half_full <= 4 and partial <= 32 (the sizes of P2ParamsF64's tables)
+    __device__ __forceinline__ static double permute(double s, const Consts& c, uint32_t lane) {
+        s = mds_light(s);
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r)
+            if (r < c.half_full) s = mds_light(Poseidon2F64::sbox(s + c.init[r]));
+#pragma unroll
+        for (uint32_t r = 0; r < 32; ++r) {
+            if (r < c.partial) {
+                const double t = Poseidon2F64::sbox(s + c.internal[r]);
+                s = internal_layer(lane == 0 ? t : s, c.diag);
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r)
+            if (r < c.half_full) s = mds_light(Poseidon2F64::sbox(s + c.term[r]));
         return BBF64::reduce(s);
     }
 };
