@@ -394,7 +394,10 @@ __global__ __launch_bounds__(TAIL_THREADS) void compress_tail_lanes_f64_kernel(T
             if (child < prev_count)
                 s = double(l == 0 ? t.in[uint64_t(child) * 8 + (lane & 7)] : buf[(l - 1) & 1][child * 8 + (lane & 7)]);
         }
-        s = Poseidon2F64Lanes::permute(s, pc, lane);
+        // a wave holds nodes 4w..4w+3; waves past the level's last node skip
+        // the permutation, so the live waves do not share their SIMDs' issue
+        // slots with idle ones (DPP stays within a 16-lane row, one node)
+        if (((threadIdx.x >> 6) << 2) < cnt) s = Poseidon2F64Lanes::permute(s, pc, lane);
         if (node < cnt && lane < 8) {
             const uint32_t v = BBF64::canonical(s);
             buf[l & 1][node * 8 + lane] = v;
