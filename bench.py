@@ -67,6 +67,17 @@ def p2_constants(field: str, rp: int, seed: int = 1) -> np.ndarray:
     return rng.integers(0, GL_P, size=8 * 8 + rp, dtype=np.uint64)
 
 
+def host_cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
     """The reference's own CPU path (oracle/_ref, compiled from the reference
     sources in the build container) on a bounded sample of the same workload:
@@ -96,7 +107,7 @@ def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
     name = "BabyBear" if field == "bb" else "Goldilocks"
     return {"value": elems / dt, "unit": "committed field elems/s", "cores": 1, "kind": kind,
             "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
-                      f"{dt:.2f} s on 1 host thread"}
+                      f"{dt:.2f} s on 1 host thread", "cpu_model": host_cpu_model()}
 
 
 def cpu_baseline_threads(field: str, width: int, log_blowup: int, rp: int, threads: int = 0):
@@ -124,7 +135,7 @@ def cpu_baseline_threads(field: str, width: int, log_blowup: int, rp: int, threa
     name = "BabyBear" if field == "bb" else "Goldilocks"
     return {"value": elems / dt, "unit": "committed field elems/s", "cores": threads, "kind": "port",
             "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
-                      f"{dt:.2f} s on {threads} host threads"}
+                      f"{dt:.2f} s on {threads} host threads", "cpu_model": host_cpu_model()}
 
 
 def main():
