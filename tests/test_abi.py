@@ -30,3 +30,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_binding_covers_the_header():
     assert sorted(p3hip.exported_symbols()) == declared()
+
+
+def test_host_alloc_rejects_bad_arguments_before_any_device_call():
+    """Argument checks of the pinned-buffer calls return status codes before any
+    HIP call, so they run without a device."""
+    lib = p3hip.load_library()
+    out = ctypes.c_void_p(12345)
+    assert lib.p3h_host_alloc(0, ctypes.byref(out)) == 1  # P3H_EINVAL
+    assert out.value is None  # cleared on failure
+    assert lib.p3h_host_alloc(64, None) == 1
+    assert lib.p3h_host_free(None) == 0  # freeing nothing is a no-op
