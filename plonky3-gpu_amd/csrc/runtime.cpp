@@ -1945,8 +1945,8 @@ int p3h_pcs_open(p3h_ctx* ctx, const p3h_tree* tree, const uint64_t* shifts, con
                         }
                     }
                     DevPtr zd = dalloc(ctx, z.size() * 4), vd = dalloc(ctx, van.size() * 4);
-                    h2d(ctx, zd->ptr, z.data(), z.size() * 4);
-                    h2d(ctx, vd->ptr, van.data(), van.size() * 4);
+                    h2d_async(ctx, zd->ptr, z.data(), z.size() * 4);
+                    h2d_async(ctx, vd->ptr, van.data(), van.size() * 4);
                     DevPtr u = dalloc(ctx, N * P * 16);
                     DevPtr part = dalloc(ctx, uint64_t(n_blocks) * P * w * 16);
                     const LdeView v = lde_view(L);
@@ -2024,15 +2024,14 @@ int p3h_pcs_open(p3h_ctx* ctx, const p3h_tree* tree, const uint64_t* shifts, con
                     DevPtr ad = dalloc(ctx, apow.size() * 4), zd = dalloc(ctx, z.size() * 4),
                            sd = dalloc(ctx, sv.size() * 4), od = dalloc(ctx, ao.size() * 4),
                            abd = dalloc(ctx, L->height * 32);
-                    h2d(ctx, ad->ptr, apow.data(), apow.size() * 4);
-                    h2d(ctx, zd->ptr, z.data(), z.size() * 4);
-                    h2d(ctx, sd->ptr, sv.data(), sv.size() * 4);
-                    h2d(ctx, od->ptr, ao.data(), ao.size() * 4);
+                    h2d_async(ctx, ad->ptr, apow.data(), apow.size() * 4);
+                    h2d_async(ctx, zd->ptr, z.data(), z.size() * 4);
+                    h2d_async(ctx, sd->ptr, sv.data(), sv.size() * 4);
+                    h2d_async(ctx, od->ptr, ao.data(), ao.size() * 4);
                     launch_check(pcs_quotient_device(lde_view(L), g, bb_mont(bb_root(lN)), dptr<uint32_t>(ad),
                                                      dptr<uint32_t>(zd), dptr<uint32_t>(sd), dptr<uint32_t>(od), P,
                                                      dptr<uint32_t>(abd), q->data<uint32_t>(), ctx->stream),
                                  "pcs quotient");
-                    hip_check(hipStreamSynchronize(ctx->stream), "sync");  // the small tables above are freed next
                 }
             }
             ph.reset();
