@@ -572,3 +572,32 @@ def test_u32_engine_trees(ctx64, golden, orc):
     c, _, _ = _oracle_batch(orc, golden, traces, ks, 1)
     assert (plan.run() == c).all() and (plan.run() == c).all()
     plan.free()
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_lde_commit_open_sweep(shape_ctx, golden, orc, seed):
+    """Seeded random shapes through the whole path against the oracle: field,
+    height 2^0..2^12, width 1..70, added bits 0..3, pass-width cap and column
+    tile from the tuner's ranges; LDE, single-matrix commit root and (BabyBear)
+    openings of random rows."""
+    ctx = shape_ctx
+    rng = np.random.default_rng(31337 + seed)
+    f = "bb" if seed % 3 else "gl"
+    lh, w, k = int(rng.integers(0, 13)), int(rng.integers(1, 71)), int(rng.integers(0, 4))
+    if f == "gl":
+        lh = min(lh, 10)
+    ctx.set_ntt_tuning(int(rng.choice([0, 2, 3, 4, 5, 6, 7])), int(rng.choice([0, 3, 5, 7, 9, 10])), 0)
+    m = rnd(seed * 7919 + lh, (1 << lh, w), f)
+    shift = 31 if f == "bb" else 7
+    want = orc.lde(f, m, k, shift)
+    lde = ctx.coset_lde(ctx.upload(m), k, shift)
+    assert (lde.download() == want).all(), (f, lh, w, k)
+    consts, rp = (golden["consts_bb"], 13) if f == "bb" else (golden["consts_gl"], 22)
+    t = ctx.mmcs_commit([lde], 0)
+    assert (t.root() == orc.flat_commit(f, consts, RF, rp, want)).all(), (f, lh, w, k)
+    if f == "bb":
+        idx = rng.integers(0, want.shape[0], size=3)
+        vals, proof = t.open(idx)
+        for q, i in enumerate(idx):
+            row, pr = orc.flat_open(consts, RF, 13, want, int(i))
+            assert (vals[q] == row).all() and (proof[q] == pr).all(), (lh, w, k, int(i))
