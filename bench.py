@@ -113,8 +113,9 @@ def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
 def cpu_baseline_threads(field: str, width: int, log_blowup: int, rp: int, threads: int = 0):
     """The oracle restatement on several host threads (columns split for the
     LDE, rows and tree nodes for the commit; the root equals the single-thread
-    one), on a sample 4x the single-thread one.  Threads: the 16 host cores a
-    one-GPU share of the box has, or fewer if the host has fewer."""
+    one), on a sample 4x the single-thread one.  Threads: 16, the host cores
+    that go with one GPU of an eight-GPU MI355X node, or fewer if the host has
+    fewer."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import native  # test infrastructure; used here only as the CPU baseline
 

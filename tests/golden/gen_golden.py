@@ -1,10 +1,10 @@
 """Write tests/golden/golden.npz from the reference build (oracle/_ref/libp3ref.so).
 
 The reference (dloghin/plonky3-gpu) is header-only C++; `make -C oracle ref`
-compiles a driver over its headers in this container.  This script runs that
-build on small seeded inputs and stores inputs and outputs as plain arrays, so
-the oracle and the GPU path can be checked against the reference where it is
-absent (on the GPU box).  Nothing from the reference is copied: only data.
+compiles a driver over its headers where they are present.  This script runs
+that build on small seeded inputs and stores inputs and outputs as plain
+arrays, so the oracle and the GPU path can be checked against the reference on
+machines that do not have it.  Nothing from the reference is copied: only data.
 
     make -C oracle ref && python tests/golden/gen_golden.py
 """
