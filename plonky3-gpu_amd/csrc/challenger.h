@@ -44,10 +44,15 @@ struct ChalProgram {
 // Run a program of transcript operations on *st (one wavefront).
 int chal_run_device(DevChal* st, const ChalProgram& prog, const P2ParamsF64& p, hipStream_t s);
 
-// For every witness w in [base, base + count): the sample that follows
+// For witnesses w in [base, base + count): the sample that follows
 // observe_witness(w) on a copy of *st; the least w whose sample has `bits`
-// zero low bits is folded into *found with an atomic minimum.
-int grind_search_device(const DevChal* st, uint64_t base, uint32_t count, uint32_t bits,
-                        unsigned long long* found, const P2ParamsF64& p, hipStream_t s);
+// zero low bits is folded into *found with an atomic minimum.  Workgroups take
+// consecutive chunks in increasing order from the counter *next (zero before
+// the launch) and stop taking them once *found lies below the next chunk, so
+// the search ends soon after the least witness instead of covering the whole
+// range; *found (all ones before the launch) is final when the grid drains.
+int grind_search_device(const DevChal* st, uint64_t base, uint64_t count, uint32_t bits,
+                        unsigned long long* found, unsigned long long* next, const P2ParamsF64& p,
+                        hipStream_t s);
 
 }  // namespace p3h

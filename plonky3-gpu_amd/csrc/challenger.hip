@@ -86,11 +86,9 @@ __global__ __launch_bounds__(64) void chal_run_kernel(DevChal* st, ChalProgram p
 // the second word fills the buffer and the sample is taken from that
 // duplexing's outputs), and for l = 7 two: [inputs, lo, state[8..]] and then
 // [hi, state[1..]].  The sample is word 7 of the last permutation's output.
-__global__ __launch_bounds__(256) void grind_search_kernel(const DevChal* st, uint64_t base, uint32_t count,
-                                                            uint32_t bits, unsigned long long* found, P2ParamsF64 p) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const uint64_t w = base + t;
+constexpr uint32_t GRIND_THREADS = 256;  // one witness per thread per chunk
+
+__device__ __forceinline__ uint32_t grind_sample(const DevChal* st, uint64_t w, const P2ParamsF64& p) {
     const uint32_t lo = uint32_t(w & 0xFFFFFFFFu) % BB_PRIME, hi = uint32_t(w >> 32) % BB_PRIME;
     const uint32_t ell = st->in_len;
     double s[16];
@@ -105,9 +103,33 @@ __global__ __launch_bounds__(256) void grind_search_kernel(const DevChal* st, ui
         s[0] = double(hi);
         Poseidon2F64::permute(s, p);
     }
-    const uint32_t v = BBF64::canonical(s[7]);
+    return BBF64::canonical(s[7]);
+}
+
+// Work queue over chunks of GRIND_THREADS witnesses.  Every workgroup leaves
+// the loop once the counter passes the end of the range or the least passing
+// witness found so far lies below its next chunk; chunks below that witness
+// were all taken before it and finish, so the grid drains with the minimum.
+__global__ __launch_bounds__(GRIND_THREADS) void grind_search_kernel(const DevChal* st, uint64_t base, uint64_t count,
+                                                                     uint32_t bits, unsigned long long* found,
+                                                                     unsigned long long* next, P2ParamsF64 p) {
+    __shared__ unsigned long long chunk;
     const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
-    if ((v & mask) == 0) atomicMin(found, (unsigned long long)w);
+    const uint64_t end = base + count;
+    while (true) {
+        if (threadIdx.x == 0) {
+            const uint64_t c = atomicAdd(next, 1ull);
+            const uint64_t start = base + c * GRIND_THREADS;
+            const uint64_t f = __hip_atomic_load(found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            chunk = (c >= (count + GRIND_THREADS - 1) / GRIND_THREADS || f < start) ? ~0ull : start;
+        }
+        __syncthreads();
+        const uint64_t start = chunk;
+        __syncthreads();  // every thread has read it before thread 0 writes the next
+        if (start == ~0ull) break;
+        const uint64_t w = start + threadIdx.x;
+        if (w < end && (grind_sample(st, w, p) & mask) == 0) atomicMin(found, (unsigned long long)w);
+    }
 }
 
 }  // namespace
@@ -118,10 +140,15 @@ int chal_run_device(DevChal* st, const ChalProgram& prog, const P2ParamsF64& p, 
     return int(hipGetLastError());
 }
 
-int grind_search_device(const DevChal* st, uint64_t base, uint32_t count, uint32_t bits,
-                        unsigned long long* found, const P2ParamsF64& p, hipStream_t s) {
+int grind_search_device(const DevChal* st, uint64_t base, uint64_t count, uint32_t bits,
+                        unsigned long long* found, unsigned long long* next, const P2ParamsF64& p,
+                        hipStream_t s) {
     if (count == 0) return 0;
-    hipLaunchKernelGGL(grind_search_kernel, dim3((count + 255) / 256), dim3(256), 0, s, st, base, count, bits, found,
+    // about as many workgroups as the device holds at once; extra ones exit
+    // at their first chunk once the search is over
+    const uint64_t chunks = (count + GRIND_THREADS - 1) / GRIND_THREADS;
+    const uint32_t grid = uint32_t(chunks < 4096 ? chunks : 4096);
+    hipLaunchKernelGGL(grind_search_kernel, dim3(grid), dim3(GRIND_THREADS), 0, s, st, base, count, bits, found, next,
                        p);
     return int(hipGetLastError());
 }

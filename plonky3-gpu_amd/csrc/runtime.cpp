@@ -406,7 +406,7 @@ struct p3h_fri {
 struct p3h_chal {
     p3h_ctx* ctx;
     DevPtr st;
-    DevPtr io;  // 64 sample words, then the 64-bit grinding minimum
+    DevPtr io;  // 64 sample words, then the 64-bit grinding minimum and the search's work counter
 };
 
 namespace {
@@ -1074,18 +1074,24 @@ ChalOp chal_op(uint32_t code, uint32_t n = 0, uint64_t arg = 0, const uint32_t* 
 
 // The least witness whose following sample has `bits` zero low bits
 // (DuplexChallenger::grind, duplex_challenger.hpp:236-254), searched on the
-// device in batches of increasing witnesses; the transcript is not changed.
+// device over increasing witnesses; the search launch stops taking work once
+// it has found the least one, so one launch covers a range of 2^36 witnesses
+// (the chance that none of them passes at bits <= 30 is below e^-64); the
+// transcript is not changed.
 uint64_t chal_search(p3h_chal* ch, uint32_t bits) {
     p3h_ctx* ctx = ch->ctx;
     if (bits >= 31) throw Err{P3H_EINVAL, "proof-of-work bits must be below 31"};
+    constexpr uint64_t RANGE = 1ull << 36;
     const unsigned long long none = ~0ull;
-    const uint32_t batch = uint32_t(std::min<uint64_t>(std::max<uint64_t>(4ull << bits, 1ull << 16), 1ull << 24));
-    for (uint64_t base = 0;; base += batch) {
-        h2d(ctx, chal_found(ch), &none, sizeof(none));
-        launch_check(grind_search_device(chal_state(ch), base, batch, bits, chal_found(ch), ctx->bbp64, ctx->stream),
+    unsigned long long* found = chal_found(ch);
+    unsigned long long* next = found + 1;  // the work counter after the minimum
+    for (uint64_t base = 0;; base += RANGE) {
+        const unsigned long long init[2] = {none, 0ull};
+        h2d(ctx, found, init, sizeof(init));
+        launch_check(grind_search_device(chal_state(ch), base, RANGE, bits, found, next, ctx->bbp64, ctx->stream),
                      "grind");
         unsigned long long w = none;
-        d2h(ctx, &w, chal_found(ch), sizeof(w));
+        d2h(ctx, &w, found, sizeof(w));
         if (w != none) return w;
     }
 }
