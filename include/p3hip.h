@@ -145,6 +145,23 @@ int p3h_tree_layer(p3h_ctx* ctx, const p3h_tree* t, uint32_t layer, void* out);
 int p3h_tree_open(p3h_ctx* ctx, const p3h_tree* t, const uint64_t* indices, uint32_t n_idx,
                   void* values_out, void* proof_out);
 int p3h_tree_free(p3h_tree* t);
+/* Verifiers for an opening, each permutation run on the device:
+ * p3_merkle_tree::MerkleTreeMmcs::verify_batch (merkle_tree_mmcs.hpp:168-254):
+ * heights/widths: the committed matrices' dimensions in commit order; values:
+ * their opened rows concatenated in that order (as p3h_tree_open writes them);
+ * proof: proof_len sibling digests, leaf level first; cap: cap_len digests.
+ * *ok = 1 when the path reproduces the cap entry, 0 when it does not (a wrong
+ * proof length, a sibling or value that does not hash to the cap); an error
+ * status only for null pointers, an unknown field, non-canonical elements or
+ * an unset Poseidon2. */
+int p3h_mmcs_verify(p3h_ctx* ctx, int field, const void* cap, uint32_t cap_len, const uint64_t* heights,
+                    const uint32_t* widths, uint32_t n_mats, uint64_t index, const void* values, const void* proof,
+                    uint32_t proof_len, int* ok);
+/* p3_merkle::MerkleTreeMmcs::verify_row (fri_merkle_tree_mmcs.hpp:146-168):
+ * leaf = sponge(row), then one compression per sibling; root: n_root digests
+ * (the commitment vector; *ok = 0 when it is empty). */
+int p3h_flat_verify(p3h_ctx* ctx, int field, const void* root, uint32_t n_root, uint64_t index, const void* row,
+                    uint32_t width, const void* proof, uint32_t proof_len, int* ok);
 
 /* ------------------------------------------------------------------- FRI */
 /* fold_matrix_cuda (fri_fold_cuda.hpp:306-348) / TwoAdicFriFolding::fold_matrix

@@ -176,13 +176,17 @@ struct HipCap {
 class HipTree {
    public:
     HipTree() = default;
-    HipTree(p3h_ctx* ctx, p3h_tree* t) : ctx_(ctx), t_(t, [](p3h_tree* p) { p3h_tree_free(p); }) {}
+    HipTree(p3h_ctx* ctx, p3h_tree* t, std::vector<size_t> widths = {})
+        : ctx_(ctx), t_(t, [](p3h_tree* p) { p3h_tree_free(p); }), widths_(std::move(widths)) {}
     p3h_tree* get() const { return t_.get(); }
     p3h_ctx* ctx() const { return ctx_; }
+    // widths of the committed matrices, in commit order
+    const std::vector<size_t>& widths() const { return widths_; }
 
    private:
     p3h_ctx* ctx_ = nullptr;
     std::shared_ptr<p3h_tree> t_;
+    std::vector<size_t> widths_;
 };
 
 template <class F, class Raw, size_t DIGEST>
@@ -217,13 +221,15 @@ class HipMerkleTreeMmcs {
         p3h_ctx* c = ctx_.get();
         std::vector<DeviceMat> dev;
         std::vector<const p3h_mat*> ptrs;
+        std::vector<size_t> widths;
         for (auto& m : matrices) {
             dev.push_back(upload<Raw>(c, m.values, m.width()));
             ptrs.push_back(dev.back().get());
+            widths.push_back(m.width());
         }
         p3h_tree* t = nullptr;
         check(c, p3h_mmcs_commit(c, ptrs.data(), uint32_t(ptrs.size()), uint32_t(cap_height_), &t));
-        HipTree tree(c, t);
+        HipTree tree(c, t, std::move(widths));
         return {Commitment{tree_cap<F, Raw, DIGEST>(tree)}, tree};
     }
 
@@ -236,9 +242,39 @@ class HipMerkleTreeMmcs {
         check(tree.ctx(), p3h_tree_open(tree.ctx(), tree.get(), &idx, 1, vals.data(), proof.data()));
         Opening op;
         op.proof = digests_of<F, Raw, DIGEST>(proof);
-        op.opened_values.push_back({});
-        for (Raw r : vals) op.opened_values.back().push_back(FieldRaw<F, Raw>::from(r));
-        return op;  // values of all matrices, concatenated in commit order
+        size_t at = 0;  // one row per committed matrix, in commit order
+        for (size_t w : tree.widths()) {
+            op.opened_values.emplace_back();
+            for (size_t k = 0; k < w; ++k) op.opened_values.back().push_back(FieldRaw<F, Raw>::from(vals[at + k]));
+            at += w;
+        }
+        return op;
+    }
+
+    // verify_batch (merkle_tree_mmcs.hpp:168-254); Dims has .width and .height
+    // (p3_matrix::Dimensions), in commit order.
+    template <class Dims>
+    bool verify_batch(const Commitment& commitment, const std::vector<Dims>& dims, size_t index,
+                      const Opening& opening) const {
+        if (dims.size() != opening.opened_values.size()) return false;
+        std::vector<uint64_t> heights;
+        std::vector<uint32_t> widths;
+        std::vector<Raw> vals, cap, proof;
+        for (size_t i = 0; i < dims.size(); ++i) {
+            if (opening.opened_values[i].size() != dims[i].width) return false;
+            heights.push_back(dims[i].height);
+            widths.push_back(uint32_t(dims[i].width));
+            for (const F& x : opening.opened_values[i]) vals.push_back(FieldRaw<F, Raw>::to(x));
+        }
+        for (const auto& d : commitment.cap)
+            for (const F& x : d) cap.push_back(FieldRaw<F, Raw>::to(x));
+        for (const auto& d : opening.proof)
+            for (const F& x : d) proof.push_back(FieldRaw<F, Raw>::to(x));
+        int ok = 0;
+        check(ctx_.get(), p3h_mmcs_verify(ctx_.get(), field_id<Raw>(), cap.data(), uint32_t(commitment.cap.size()),
+                                          heights.data(), widths.data(), uint32_t(dims.size()), index, vals.data(),
+                                          proof.data(), uint32_t(opening.proof.size()), &ok));
+        return ok != 0;
     }
 
    private:
@@ -291,6 +327,22 @@ class HipFlatMmcs {
         check(d.tree.ctx(), p3h_tree_open(d.tree.ctx(), d.tree.get(), &idx, 1, nullptr, raw.data()));
         raw.resize(size_t(n_layers - 1) * DIGEST);
         proof = digests_of<F, Raw, DIGEST>(raw);
+    }
+
+    // verify_row (fri_merkle_tree_mmcs.hpp:146-168)
+    bool verify_row(const Commitment& commit, size_t row_index, const std::vector<F>& row_vals,
+                    const OpeningProof& proof) const {
+        std::vector<Raw> root, row, pr;
+        for (const auto& d : commit)
+            for (const F& x : d) root.push_back(FieldRaw<F, Raw>::to(x));
+        for (const F& x : row_vals) row.push_back(FieldRaw<F, Raw>::to(x));
+        for (const auto& d : proof)
+            for (const F& x : d) pr.push_back(FieldRaw<F, Raw>::to(x));
+        int ok = 0;
+        check(ctx_.get(), p3h_flat_verify(ctx_.get(), field_id<Raw>(), root.data(), uint32_t(commit.size()),
+                                          row_index, row.data(), uint32_t(row.size()), pr.data(),
+                                          uint32_t(proof.size()), &ok));
+        return ok != 0;
     }
 
     size_t log_height(const ProverData& d) const { return log2_strict(d.height); }

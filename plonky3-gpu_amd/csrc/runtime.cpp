@@ -20,6 +20,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <numeric>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -1396,6 +1397,136 @@ void do_fri_commit(p3h_ctx* ctx, const p3h_mat* codeword, uint32_t log_blowup, u
 
 }  // namespace
 
+namespace {
+
+// n host states permuted on the device with the context's engine (canonical in
+// and out).
+void permute_host(p3h_ctx* ctx, int field, void* states, uint64_t n) {
+    if (field == P3H_BABYBEAR) {
+        const auto& pp = p2<HBB>(ctx);
+        DevPtr d = dalloc(ctx, std::max<uint64_t>(1, n * 16 * 4));
+        h2d(ctx, d->ptr, states, n * 16 * 4);
+        if (ctx->hash_engine == 1)
+            launch_check(permute_f64_device(dptr<uint32_t>(d), n, ctx->bbp64, ctx->stream), "permute");
+        else
+            launch_check(permute_device<BBDom>(dptr<uint32_t>(d), n, pp, ctx->stream), "permute");
+        d2h(ctx, states, d->ptr, n * 16 * 4);
+    } else if (field == P3H_GOLDILOCKS) {
+        const auto& pp = p2<HGL>(ctx);
+        DevPtr d = dalloc(ctx, std::max<uint64_t>(1, n * 8 * 8));
+        h2d(ctx, d->ptr, states, n * 8 * 8);
+        launch_check(permute_device<GLDom>(dptr<uint64_t>(d), n, pp, ctx->stream), "permute");
+        d2h(ctx, states, d->ptr, n * 8 * 8);
+    } else {
+        throw Err{P3H_EINVAL, "unknown field"};
+    }
+}
+
+// The sponge (PaddingFreeSponge::hash_iter_slices, padding_free_sponge.hpp:63-113)
+// and the 2-to-1 compression (TruncatedPermutation, truncated_permutation.hpp:61-76)
+// over host digests, each permutation run on the device.  The verifiers below
+// walk one authentication path, so they are a short chain of dependent
+// permutations.
+template <class Raw, int W, int RATE, int DIG>
+struct PathHasher {
+    using Digest = std::array<Raw, DIG>;
+    p3h_ctx* ctx;
+    int field;
+    Raw modulus;
+
+    void perm(std::array<Raw, W>& s) const { permute_host(ctx, field, s.data(), 1); }
+    // state = 0; each chunk of up to RATE elements overwrites the front of the
+    // state and is followed by a permutation; no input -> zero digest.
+    Digest hash(const std::vector<std::pair<const Raw*, uint64_t>>& parts) const {
+        std::array<Raw, W> s{};
+        uint32_t fill = 0;
+        for (const auto& [ptr, len] : parts)
+            for (uint64_t i = 0; i < len; ++i) {
+                if (ptr[i] >= modulus) throw Err{P3H_EINVAL, "opened value is not a canonical field element"};
+                s[fill++] = ptr[i];
+                if (fill == RATE) {
+                    perm(s);
+                    fill = 0;
+                }
+            }
+        if (fill) perm(s);
+        Digest d;
+        std::copy(s.begin(), s.begin() + DIG, d.begin());
+        return d;
+    }
+    Digest compress(const Digest& l, const Digest& r) const {
+        std::array<Raw, W> s{};
+        std::copy(l.begin(), l.end(), s.begin());
+        std::copy(r.begin(), r.end(), s.begin() + DIG);
+        perm(s);
+        Digest d;
+        std::copy(s.begin(), s.begin() + DIG, d.begin());
+        return d;
+    }
+    Digest digest_at(const void* p, uint64_t i) const {
+        if (!p) throw Err{P3H_EINVAL, "null digest"};
+        Digest d;
+        std::memcpy(d.data(), static_cast<const Raw*>(p) + i * DIG, sizeof(d));
+        for (Raw x : d)
+            if (x >= modulus) throw Err{P3H_EINVAL, "digest element is not a canonical field element"};
+        return d;
+    }
+
+    // MerkleTreeMmcs::verify_batch (merkle_tree_mmcs.hpp:168-254).
+    bool verify_batch(const void* cap, uint32_t cap_len, const uint64_t* heights, const uint32_t* widths,
+                      uint32_t n, uint64_t index, const Raw* values, const void* proof, uint32_t proof_len) const {
+        if (n == 0) return false;
+        std::vector<uint64_t> offset(n + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) offset[i + 1] = offset[i] + widths[i];
+        std::vector<uint32_t> order(n);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return heights[a] > heights[b]; });
+        const uint64_t max_h = heights[order[0]];
+        if (max_h == 0) return false;
+        uint32_t levels = 0;
+        for (uint64_t h = max_h; h > cap_len; h /= 2) ++levels;
+        if (proof_len != levels) return false;
+        size_t pos = 0;
+        auto group = [&](uint64_t h) {
+            std::vector<std::pair<const Raw*, uint64_t>> parts;
+            while (pos < order.size() && heights[order[pos]] == h) {
+                const uint32_t m = order[pos++];
+                parts.push_back({values + offset[m], widths[m]});
+            }
+            return parts;
+        };
+        Digest cur = hash(group(max_h));
+        uint64_t idx = index;
+        for (uint32_t step = 0; step < levels; ++step) {
+            const Digest sib = digest_at(proof, step);
+            cur = (idx & 1) == 0 ? compress(cur, sib) : compress(sib, cur);
+            idx >>= 1;
+            const auto inject = group(max_h >> (step + 1));
+            if (!inject.empty()) cur = compress(cur, hash(inject));
+        }
+        if (idx >= cap_len) return false;
+        return cur == digest_at(cap, idx);
+    }
+    // p3_merkle::MerkleTreeMmcs::verify_row (fri_merkle_tree_mmcs.hpp:146-168).
+    bool verify_row(const void* root, uint32_t n_root, uint64_t index, const Raw* row, uint32_t width,
+                    const void* proof, uint32_t proof_len) const {
+        Digest cur = hash({{row, width}});
+        uint64_t idx = index;
+        for (uint32_t step = 0; step < proof_len; ++step) {
+            const Digest sib = digest_at(proof, step);
+            cur = idx % 2 == 0 ? compress(cur, sib) : compress(sib, cur);
+            idx >>= 1;
+        }
+        if (n_root == 0) return false;
+        return cur == digest_at(root, 0);
+    }
+};
+
+using BBPath = PathHasher<uint32_t, 16, 8, 8>;
+using GLPath = PathHasher<uint64_t, 8, 4, 4>;
+
+}  // namespace
+
 extern "C" {
 
 int p3h_ctx_create(int device, uint64_t budget, p3h_ctx** out) {
@@ -1563,24 +1694,45 @@ int p3h_poseidon2_set(p3h_ctx* ctx, int field, uint32_t width, uint32_t rf, uint
 int p3h_poseidon2_permute(p3h_ctx* ctx, int field, void* states, uint64_t n) {
     return api(ctx, [&] {
         if (!states && n) throw Err{P3H_EINVAL, "null states"};
-        if (field == P3H_BABYBEAR) {
-            const auto& pp = p2<HBB>(ctx);
-            DevPtr d = dalloc(ctx, std::max<uint64_t>(1, n * 16 * 4));
-            h2d(ctx, d->ptr, states, n * 16 * 4);
-            if (ctx->hash_engine == 1)
-                launch_check(permute_f64_device(dptr<uint32_t>(d), n, ctx->bbp64, ctx->stream), "permute");
-            else
-                launch_check(permute_device<BBDom>(dptr<uint32_t>(d), n, pp, ctx->stream), "permute");
-            d2h(ctx, states, d->ptr, n * 16 * 4);
-        } else if (field == P3H_GOLDILOCKS) {
-            const auto& pp = p2<HGL>(ctx);
-            DevPtr d = dalloc(ctx, std::max<uint64_t>(1, n * 8 * 8));
-            h2d(ctx, d->ptr, states, n * 8 * 8);
-            launch_check(permute_device<GLDom>(dptr<uint64_t>(d), n, pp, ctx->stream), "permute");
-            d2h(ctx, states, d->ptr, n * 8 * 8);
-        } else {
+        permute_host(ctx, field, states, n);
+    });
+}
+
+int p3h_mmcs_verify(p3h_ctx* ctx, int field, const void* cap, uint32_t cap_len, const uint64_t* heights,
+                    const uint32_t* widths, uint32_t n_mats, uint64_t index, const void* values, const void* proof,
+                    uint32_t proof_len, int* ok) {
+    return api(ctx, [&] {
+        if (!ok || (n_mats && (!heights || !widths || !values))) throw Err{P3H_EINVAL, "null argument"};
+        *ok = 0;
+        bool r;
+        if (field == P3H_BABYBEAR)
+            r = BBPath{ctx, field, uint32_t(BB_P)}.verify_batch(cap, cap_len, heights, widths, n_mats, index,
+                                                                 static_cast<const uint32_t*>(values), proof,
+                                                                 proof_len);
+        else if (field == P3H_GOLDILOCKS)
+            r = GLPath{ctx, field, GL_P}.verify_batch(cap, cap_len, heights, widths, n_mats, index,
+                                                      static_cast<const uint64_t*>(values), proof, proof_len);
+        else
             throw Err{P3H_EINVAL, "unknown field"};
-        }
+        *ok = r ? 1 : 0;
+    });
+}
+
+int p3h_flat_verify(p3h_ctx* ctx, int field, const void* root, uint32_t n_root, uint64_t index, const void* row,
+                    uint32_t width, const void* proof, uint32_t proof_len, int* ok) {
+    return api(ctx, [&] {
+        if (!ok || (width && !row)) throw Err{P3H_EINVAL, "null argument"};
+        *ok = 0;
+        bool r;
+        if (field == P3H_BABYBEAR)
+            r = BBPath{ctx, field, uint32_t(BB_P)}.verify_row(root, n_root, index, static_cast<const uint32_t*>(row),
+                                                               width, proof, proof_len);
+        else if (field == P3H_GOLDILOCKS)
+            r = GLPath{ctx, field, GL_P}.verify_row(root, n_root, index, static_cast<const uint64_t*>(row), width,
+                                                    proof, proof_len);
+        else
+            throw Err{P3H_EINVAL, "unknown field"};
+        *ok = r ? 1 : 0;
     });
 }
 

@@ -81,6 +81,8 @@ _SIGS = {
     "p3h_tree_layer": (_i32, [_vp, _vp, _u32, _vp]),
     "p3h_tree_open": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "p3h_tree_free": (_i32, [_vp]),
+    "p3h_mmcs_verify": (_i32, [_vp, _i32, _vp, _u32, _vp, _vp, _u32, _u64, _vp, _vp, _u32, C.POINTER(C.c_int)]),
+    "p3h_flat_verify": (_i32, [_vp, _i32, _vp, _u32, _u64, _vp, _u32, _vp, _u32, C.POINTER(C.c_int)]),
     "p3h_fri_fold": (_i32, [_vp, _vp, _u32, _vp, C.POINTER(_vp)]),
     "p3h_fri_commit_phase": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, C.POINTER(_u32), C.POINTER(_vp)]),
     "p3h_fri_commit_phase_chal": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp, _vp, C.POINTER(_u32),
@@ -295,6 +297,35 @@ class Context:
         h = _vp()
         self._check(self.lib.p3h_flat_commit(self.h, m.h, C.byref(h)))
         return Tree(self, h, m.field)
+
+    def mmcs_verify(self, field: int, cap, dims, index: int, values, proof) -> bool:
+        """MerkleTreeMmcs::verify_batch (merkle_tree_mmcs.hpp:168-254).  dims:
+        (height, width) per matrix in commit order; values: their opened rows
+        concatenated in that order; proof: sibling digests, leaf level first;
+        cap: the commitment's digests."""
+        dt, dg = DTYPE[field], DIGEST[field]
+        cap = np.ascontiguousarray(cap, dtype=dt).reshape(-1, dg)
+        hs = np.ascontiguousarray([int(h) for h, _ in dims], dtype=np.uint64)
+        ws = np.ascontiguousarray([int(w) for _, w in dims], dtype=np.uint32)
+        vals = np.ascontiguousarray(values, dtype=dt).reshape(-1)
+        if vals.size != int(ws.sum()):
+            raise ValueError("opened values do not match the matrix widths")
+        pr = np.ascontiguousarray(proof, dtype=dt).reshape(-1, dg)
+        ok = C.c_int(0)
+        self._check(self.lib.p3h_mmcs_verify(self.h, field, _ptr(cap), cap.shape[0], _ptr(hs), _ptr(ws), len(dims),
+                                             index, _ptr(vals), _ptr(pr), pr.shape[0], C.byref(ok)))
+        return bool(ok.value)
+
+    def flat_verify(self, field: int, root, index: int, row, proof) -> bool:
+        """p3_merkle::MerkleTreeMmcs::verify_row (fri_merkle_tree_mmcs.hpp:146-168)."""
+        dt, dg = DTYPE[field], DIGEST[field]
+        root = np.ascontiguousarray(root, dtype=dt).reshape(-1, dg)
+        r = np.ascontiguousarray(row, dtype=dt).reshape(-1)
+        pr = np.ascontiguousarray(proof, dtype=dt).reshape(-1, dg)
+        ok = C.c_int(0)
+        self._check(self.lib.p3h_flat_verify(self.h, field, _ptr(root), root.shape[0], index, _ptr(r), r.size,
+                                             _ptr(pr), pr.shape[0], C.byref(ok)))
+        return bool(ok.value)
 
     def pcs_commit(self, mats, shifts, log_blowup: int) -> "Tree":
         arr = (_vp * len(mats))(*[m.h for m in mats])
@@ -758,14 +789,25 @@ class MerkleTreeMmcsHip:
     def commit(self, matrices):
         mats = [self.ctx.upload(m) for m in matrices]
         tree = self.ctx.mmcs_commit(mats, self.cap_height)
+        tree.widths = [m.width for m in mats]
         for m in mats:
             m.free()  # the tree keeps its own references
         return tree.cap(), tree
 
     def open_batch(self, index: int, tree: Tree):
-        """(opened_values per matrix in commit order, proof digests)."""
+        """(opened_values: one row per matrix in commit order, proof digests)."""
         vals, proof = tree.open([index])
-        return vals[0], proof[0]
+        cuts = np.cumsum(tree.widths)[:-1]
+        return list(np.split(vals[0], cuts)), proof[0]
+
+    def verify_batch(self, cap, dims, index: int, opened_values, proof) -> bool:
+        """verify_batch (merkle_tree_mmcs.hpp:168-254); dims: (height, width) per
+        matrix in commit order."""
+        if len(dims) != len(opened_values) or any(len(v) != w for v, (_, w) in zip(opened_values, dims)):
+            return False
+        field = _field_of(np.asarray(cap).dtype)
+        vals = np.concatenate([np.asarray(v).reshape(-1) for v in opened_values]) if dims else np.empty(0)
+        return self.ctx.mmcs_verify(field, cap, dims, index, vals, proof)
 
 
 class FlatMerkleTreeMmcsHip:
@@ -788,3 +830,7 @@ class FlatMerkleTreeMmcsHip:
     def open_row(self, tree: Tree, row_index: int):
         vals, proof = tree.open([row_index])
         return vals[0], proof[0]
+
+    def verify_row(self, commit, row_index: int, row_vals, proof) -> bool:
+        """verify_row (fri_merkle_tree_mmcs.hpp:146-168)."""
+        return self.ctx.flat_verify(_field_of(np.asarray(commit).dtype), commit, row_index, row_vals, proof)

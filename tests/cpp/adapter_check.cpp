@@ -57,6 +57,10 @@ int main() {
     std::vector<std::array<BB, 8>> proof;
     flat.open_row(pd, 77, proof);
     std::printf("flat_proof_len %zu first %u\n", proof.size(), proof[0][0].v);
+    auto row77 = flat.get_row(pd, 77);
+    bool flat_ok = flat.verify_row(root, 77, row77, proof);
+    row77[5] = BB(row77[5].v + 1);
+    std::printf("flat_verify %d %d\n", int(flat_ok), int(flat.verify_row(root, 77, row77, proof)));
 
     p3hip::HipMerkleTreeMmcs<BB> mmcs(ctx, 1);
     std::vector<RowMajorMatrix<BB>> mats{lde, trace};
@@ -66,7 +70,19 @@ int main() {
         for (const BB& x : dg) std::printf(" %u", x.v);
     std::printf("\n");
     auto op = mmcs.open_batch(9, tree);
-    std::printf("mmcs_open_len %zu proof_len %zu\n", op.opened_values[0].size(), op.proof.size());
+    std::printf("mmcs_open_len %zu %zu %zu proof_len %zu\n", op.opened_values.size(), op.opened_values[0].size(),
+                op.opened_values[1].size(), op.proof.size());
+    struct Dim {
+        size_t width, height;
+    };
+    std::vector<Dim> dims{{8, 2048}, {8, 1024}};
+    const bool honest = mmcs.verify_batch(cap, dims, 9, op);
+    auto bad_value = op;
+    bad_value.opened_values[1][2] = BB(bad_value.opened_values[1][2].v + 1);
+    auto bad_sibling = op;
+    bad_sibling.proof[4][0] = BB(bad_sibling.proof[4][0].v + 1);
+    std::printf("mmcs_verify %d %d %d %d\n", int(honest), int(mmcs.verify_batch(cap, dims, 9, bad_value)),
+                int(mmcs.verify_batch(cap, dims, 9, bad_sibling)), int(mmcs.verify_batch(cap, dims, 8, op)));
 
     std::vector<BB4> cw(64);
     auto vals = fill(7, 64 * 4);

@@ -24,8 +24,7 @@ def build_adapter_check():
 
 
 def test_cpp_adapters_match_oracle(orc):
-    if not os.path.exists(BIN):
-        build_adapter_check()
+    build_adapter_check()  # always: the adapters are header-only
     out = subprocess.run([BIN], check=True, capture_output=True, text=True, timeout=120).stdout
     lines = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
     consts = native.p2_consts(1, "bb", 8, 13)
@@ -40,7 +39,12 @@ def test_cpp_adapters_match_oracle(orc):
     assert int(lines["flat_proof_len"][0]) == len(proof) and int(lines["flat_proof_len"][2]) == int(proof[0][0])
     cap = orc.mmcs("bb", consts, 8, 13, [lde, trace], 1)
     assert [int(x) for x in lines["mmcs_cap"]] == [int(x) for x in cap.reshape(-1)]
-    assert lines["mmcs_open_len"] == ["16", "proof_len", "10"]
+    # one opened row per matrix, in commit order, as the reference's open_batch
+    assert lines["mmcs_open_len"] == ["2", "8", "8", "proof_len", "10"]
+    # verify_batch: the honest opening passes; a changed value, a changed sibling
+    # or another index fails
+    assert lines["mmcs_verify"] == ["1", "0", "0", "0"]
+    assert lines["flat_verify"] == ["1", "0"]
     cw = native.seeded(7, 64 * 4).reshape(64, 4)
     f = orc.fold(6, 1, np.array([3, 1, 4, 1], dtype=np.uint32), cw)
     assert [int(x) for x in lines["fold0"]] == [int(x) for x in f[0]]

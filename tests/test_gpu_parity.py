@@ -287,6 +287,72 @@ def test_mmcs_random_heights_vs_oracle(ctx, golden, orc):
             assert orc.mmcs_verify(golden["consts_bb"], RF, 13, shapes, cap, c, idx, v[0], p[0]) == 1
 
 
+def test_mmcs_verify_accepts_openings_and_rejects_tampering(ctx, golden, orc):
+    """verify_batch on the device against the oracle's verifier: honest
+    openings pass at every cap height; a changed value, sibling or index, a
+    short proof and a wrong cap entry fail in both."""
+    rng = np.random.default_rng(11)
+    shapes = [(1 << int(rng.integers(0, 10)), int(rng.integers(1, 40))) for _ in range(6)] + [(512, 8)]
+    mats = [rnd(700 + i, sh) for i, sh in enumerate(shapes)]
+    mmcs = p3hip.MerkleTreeMmcsHip(ctx, 0)
+    for cap_h in (0, 2):
+        mmcs.cap_height = cap_h
+        cap, tree = mmcs.commit(mats)
+        for idx in (0, 301, 511):
+            opened, proof = mmcs.open_batch(idx, tree)
+            assert [len(v) for v in opened] == [w for _, w in shapes]
+            assert mmcs.verify_batch(cap, shapes, idx, opened, proof)
+            flat_vals = np.concatenate(opened)
+            cases = []
+            bad = [v.copy() for v in opened]
+            bad[-1][0] = (int(bad[-1][0]) + 1) % BB_P
+            cases.append((idx, bad, proof))
+            if len(proof):
+                p2 = proof.copy()
+                p2[len(proof) // 2][3] = (int(p2[len(proof) // 2][3]) + 1) % BB_P
+                cases.append((idx, opened, p2))
+                cases.append((idx, opened, proof[:-1]))
+            cases.append((idx ^ 1, opened, proof))
+            for i, vals, pr in cases:
+                got = mmcs.verify_batch(cap, shapes, i, vals, pr)
+                want = orc.mmcs_verify(golden["consts_bb"], RF, 13, shapes, cap_h, cap, i, np.concatenate(vals), pr)
+                assert got == bool(want) and not got, (cap_h, idx, i)
+            assert orc.mmcs_verify(golden["consts_bb"], RF, 13, shapes, cap_h, cap, idx, flat_vals, proof) == 1
+        wrong_cap = cap.copy()
+        wrong_cap[0][0] = (int(wrong_cap[0][0]) + 1) % BB_P
+        opened, proof = mmcs.open_batch(0, tree)
+        assert not mmcs.verify_batch(wrong_cap, shapes, 0, opened, proof)
+    # Goldilocks: the golden multi-height tree
+    gm = mats_of(golden, "mm_gl")
+    mmcs = p3hip.MerkleTreeMmcsHip(ctx, 1)
+    cap, tree = mmcs.commit(gm)
+    dims = [m.shape for m in gm]
+    opened, proof = mmcs.open_batch(6, tree)
+    assert (np.concatenate(opened) == golden["mm_gl_vals"]).all()
+    assert mmcs.verify_batch(cap, dims, 6, opened, proof)
+    opened[0][0] = (int(opened[0][0]) + 1) % GL_P
+    assert not mmcs.verify_batch(cap, dims, 6, opened, proof)
+
+
+def test_flat_verify_row(ctx):
+    """verify_row: every opened row of a flat tree (odd layers padded with the
+    zero digest) verifies; a changed element or sibling does not."""
+    for f, dt, P in (("bb", np.uint32, BB_P), ("gl", np.uint64, GL_P)):
+        d = rnd(900, (37, 5), f)
+        mmcs = p3hip.FlatMerkleTreeMmcsHip(ctx)
+        root, tree = mmcs.commit_matrix(d.reshape(-1), 5)
+        for r in (0, 17, 36):
+            row, proof = mmcs.open_row(tree, r)
+            assert (row == d[r]).all()
+            assert mmcs.verify_row(root, r, row, proof), (f, r)
+            bad = row.copy()
+            bad[4] = (int(bad[4]) + 1) % P
+            assert not mmcs.verify_row(root, r, bad, proof)
+            p2 = proof.copy()
+            p2[0][0] = (int(p2[0][0]) + 1) % P
+            assert not mmcs.verify_row(root, r, row, p2)
+
+
 def test_config1_commit_root(ctx, golden, orc):
     """BabyBear 2^16 x 32, log_blowup 1, shift 31, flat and multi-height roots."""
     trace = native.seeded(2024, (1 << 16) * 32).reshape(1 << 16, 32)
