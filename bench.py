@@ -41,13 +41,21 @@ P2_F64_FLOPS_PER_PERM = 7216
 P2_F64_FLOPS_PER_DIGEST = 8 * 5
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
-WORKLOADS = {
-    # name: (field, log_n, width, log_blowup, poseidon2 rounds_p)
-    "bb_2^20x256_b1": ("bb", 20, 256, 1, 13),
-    "bb_2^22x64_b1": ("bb", 22, 64, 1, 13),
-    "bb_2^16x32_b1": ("bb", 16, 32, 1, 13),
-    "gl_2^22x64_b2": ("gl", 22, 64, 2, 22),
-}
+FIXTURE = os.path.join(ROOT, "tests", "golden", "configs.npz")
+METRIC = "committed field elems/s (LDE+Poseidon2 Merkle), batched vs plain; p99 ms"
+
+
+def _load_workloads():
+    spec = importlib.util.spec_from_file_location("workloads", os.path.join(ROOT, "plonky3-gpu_amd", "workloads.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["workloads"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+W = _load_workloads()
+# name: (field, log_n, width, log_blowup, poseidon2 rounds_p), inputs from W.SINGLE[name]["seed"]
+WORKLOADS = {name: (c["field"], c["log_n"], c["width"], c["added"], W.RP[c["field"]]) for name, c in W.SINGLE.items()}
 DEFAULT = "bb_2^20x256_b1"
 SPECIAL = ("batch64", "serve", "fri_bb4_2^23", "fri_bb4_2^23_chal", "grind_bb", "pcs_open_bb_2^20x256")
 
@@ -60,83 +68,100 @@ def load_p3hip():
     return mod
 
 
-def p2_constants(field: str, rp: int, seed: int = 1) -> np.ndarray:
-    rng = np.random.default_rng(seed)
-    if field == "bb":
-        return rng.integers(0, BB_P, size=8 * 16 + rp, dtype=np.uint32)
-    return rng.integers(0, GL_P, size=8 * 8 + rp, dtype=np.uint64)
+def p2_constants(field: str, rp: int) -> np.ndarray:
+    """The workloads' Poseidon2 constants (independent mt19937_64 stream)."""
+    assert rp == W.RP[field]
+    return np.array(W.p2_consts(field), dtype=np.uint32 if field == "bb" else np.uint64)
 
 
-def host_cpu_model() -> str:
+def fixture(key: str):
+    """A value of the oracle's full-size fixture (tests/golden/configs.npz), or
+    None when the file or key is absent."""
+    try:
+        with np.load(FIXTURE, allow_pickle=False) as f:
+            return f[key] if key in f.files else None
+    except OSError:
+        return None
+
+
+def host_cpu() -> dict:
+    """The host CPU: full model string from /proc/cpuinfo, nproc (logical CPUs
+    this process may run on) and the threads the multi-core baseline uses."""
+    model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                    model = line.split(":", 1)[1].strip()
+                    break
     except OSError:
         pass
-    return "unknown"
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except OSError:
+        nproc = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": nproc}
+
+
+def _median_of_3(fn):
+    """One warm-up call, then the median wall time of 3 calls (seconds)."""
+    fn()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[1], ts
 
 
 def cpu_baseline(field: str, width: int, log_blowup: int, rp: int):
-    """The reference's own CPU path (oracle/_ref, compiled from the reference
-    sources in the build container) on a bounded sample of the same workload:
-    2^15 rows x width, same blowup, LDE + single-matrix Poseidon2 commit, one
-    thread.  Falls back to the oracle restatement when the reference build is
-    absent."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import native  # test infrastructure; used here only as the CPU baseline
-
-    lib = native.reference()
-    kind = "reference"
-    if lib is None:
-        lib, kind = native.oracle(), "port"
-    log_n = 15 if field == "bb" else 13
-    n = 1 << log_n
-    rng = np.random.default_rng(3)
-    if field == "bb":
-        trace = rng.integers(0, BB_P, size=(n, width), dtype=np.uint32)
-    else:
-        trace = rng.integers(0, GL_P, size=(n, width), dtype=np.uint64)
-    consts = p2_constants(field, rp)
-    t0 = time.perf_counter()
-    lde = lib.lde(field, trace, log_blowup, 31 if field == "bb" else 7)
-    lib.flat_commit(field, consts, 8, rp, lde)
-    dt = time.perf_counter() - t0
-    elems = (n << log_blowup) * width
-    name = "BabyBear" if field == "bb" else "Goldilocks"
-    return {"value": elems / dt, "unit": "committed field elems/s", "cores": 1, "kind": kind,
-            "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
-                      f"{dt:.2f} s on 1 host thread", "cpu_model": host_cpu_model()}
-
-
-def cpu_baseline_threads(field: str, width: int, log_blowup: int, rp: int, threads: int = 0):
-    """The oracle restatement on several host threads (columns split for the
-    LDE, rows and tree nodes for the commit; the root equals the single-thread
-    one), on a sample 4x the single-thread one.  Threads: 16, the host cores
-    that go with one GPU of an eight-GPU MI355X node, or fewer if the host has
-    fewer."""
+    """The oracle's restatement of the reference CPU path (oracle/liboracle.so,
+    C++ -O3; the reference itself is single-threaded) on one host thread: LDE +
+    single-matrix Poseidon2 commit of a bounded sample of the same workload
+    (2^14 rows x width for BabyBear, 2^12 for Goldilocks; same blowup and
+    constants), one warm-up then the median of 3, scaled to elems/s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import native  # test infrastructure; used here only as the CPU baseline
 
     lib = native.oracle()
-    threads = threads or min(16, os.cpu_count() or 1)
-    log_n = 17 if field == "bb" else 15
+    log_n = 14 if field == "bb" else 12
     n = 1 << log_n
-    rng = np.random.default_rng(5)
-    if field == "bb":
-        trace = rng.integers(0, BB_P, size=(n, width), dtype=np.uint32)
-    else:
-        trace = rng.integers(0, GL_P, size=(n, width), dtype=np.uint64)
+    trace = native.seeded(3, n * width, field).reshape(n, width)
     consts = p2_constants(field, rp)
-    t0 = time.perf_counter()
-    lib.lde_commit_mt(field, consts, 8, rp, trace, log_blowup, 31 if field == "bb" else 7, threads)
-    dt = time.perf_counter() - t0
+    shift = W.SHIFT[field]
+    med, ts = _median_of_3(lambda: lib.flat_commit(field, consts, 8, rp, lib.lde(field, trace, log_blowup, shift)))
     elems = (n << log_blowup) * width
     name = "BabyBear" if field == "bb" else "Goldilocks"
-    return {"value": elems / dt, "unit": "committed field elems/s", "cores": threads, "kind": "port",
-            "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit, "
-                      f"{dt:.2f} s on {threads} host threads", "cpu_model": host_cpu_model()}
+    return {"value": elems / med, "unit": "committed field elems/s", "cores": 1, "kind": "port",
+            "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit on 1 host "
+                      f"thread, median of 3 after a warm-up ({', '.join(f'{t:.2f}' for t in ts)} s)",
+            **host_cpu()}
+
+
+def cpu_baseline_threads(field: str, width: int, log_blowup: int, rp: int, threads: int = 0):
+    """The same restatement on several host threads (column slabs for the LDE,
+    row and node ranges for the commit; the root equals the single-thread one),
+    on a sample 4x the single-thread one, median of 3 after a warm-up.
+    Threads: 16, the host share of one GPU of an eight-GPU MI355X node, or
+    nproc if smaller."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import native  # test infrastructure; used here only as the CPU baseline
+
+    lib = native.oracle()
+    cpu = host_cpu()
+    threads = threads or min(16, cpu["nproc"])
+    log_n = 16 if field == "bb" else 14
+    n = 1 << log_n
+    trace = native.seeded(5, n * width, field).reshape(n, width)
+    consts = p2_constants(field, rp)
+    med, ts = _median_of_3(lambda: lib.lde_commit_mt(field, consts, 8, rp, trace, log_blowup, W.SHIFT[field],
+                                                     threads))
+    elems = (n << log_blowup) * width
+    name = "BabyBear" if field == "bb" else "Goldilocks"
+    return {"value": elems / med, "unit": "committed field elems/s", "cores": threads, "kind": "port",
+            "sample": f"{name} 2^{log_n} x {width}, log_blowup {log_blowup}: LDE + Poseidon2 flat commit on "
+                      f"{threads} host threads, median of 3 after a warm-up ({', '.join(f'{t:.2f}' for t in ts)} s)",
+            **cpu}
 
 
 def main():
@@ -146,6 +171,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + list(SPECIAL))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload only: skip the batched (64 traces) and served comparisons")
     ap.add_argument("--hash-engine", type=int, default=1, help="BabyBear Poseidon2: 0 uint32, 1 binary64")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning-file", default=None,
@@ -162,9 +189,9 @@ def main():
         return calibrate_copy()
     p3hip = load_p3hip()
     if args.workload == "batch64":
-        return bench_batch(p3hip, args)
+        return print(json.dumps(bench_batch(p3hip, args)))
     if args.workload == "serve":
-        return bench_serve(p3hip, args)
+        return print(json.dumps(bench_serve(p3hip, args)))
     if args.workload == "fri_bb4_2^23":
         return bench_fri(p3hip, args)
     if args.workload == "fri_bb4_2^23_chal":
@@ -180,10 +207,7 @@ def main():
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(fid, p2_constants(field, rp), 8, rp)
     ctx.set_hash_engine(args.hash_engine)
-    rng = np.random.default_rng(2024)
-    p = BB_P if field == "bb" else GL_P
-    dtype = np.uint32 if field == "bb" else np.uint64
-    trace = rng.integers(0, p, size=(n, width), dtype=dtype)
+    trace = p3hip.seeded_fill(fid, W.SINGLE[args.workload]["seed"], n * width).reshape(n, width)
     m = ctx.upload(trace)
     if args.tuning_file and os.path.exists(args.tuning_file):
         ctx.tuning_load(args.tuning_file)
@@ -212,6 +236,9 @@ def main():
     phases = {ph: ctx.phase_stats(ph) for ph in ("lde", "leaf_hash", "compress")}
     ctx.set_profiling(False)
     assert all((r == roots[0]).all() for r in roots), "non-deterministic root"
+    pinned_root = fixture(f"{args.workload}/root")
+    if pinned_root is not None:
+        assert (roots[0] == pinned_root).all(), "root differs from the oracle's full-size fixture"
     # host-handed input (the reference API's RowMajorMatrix): upload + step, outside the timed region
     t0 = time.perf_counter()
     m2 = ctx.upload(trace)
@@ -252,7 +279,7 @@ def main():
                     "traffic": traffic.get("hash_rows_dense_kernel"), "algorithmic_bytes": hash_bytes,
                     "avg_launch_ms": hash_ms}
     out = {
-        "metric": "committed field elems/s (LDE+Poseidon2 Merkle)",
+        "metric": METRIC,
         "value": value,
         "unit": "field elems/s",
         "n_gpus": 1,
@@ -263,7 +290,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32" if field == "bb" else "u64",
-        "data": "synthetic (seeded uniform field elements, seeded Poseidon2 constants)",
+        "data": "synthetic (std::mt19937_64 field elements and Poseidon2 constants, seeds of workloads.py)",
         "config": {"workload": f"{'BabyBear' if field == 'bb' else 'Goldilocks'} trace 2^{log_n} x {width}, "
                                f"log_blowup {lb}, Poseidon2 W{16 if field == 'bb' else 8} RP{rp}, "
                                f"single-matrix MMCS root",
@@ -276,14 +303,28 @@ def main():
         "pcie_inclusive": {"value": elems / (host_ms / 1e3), "unit": "field elems/s", "upload_ms": upload_ms,
                            "note": "trace handed over in pageable host memory: upload + LDE + commit + root"},
         "root": [int(x) for x in roots[0]],
+        "root_equals_oracle_fixture": None if pinned_root is None else True,
         "hash_engine": "binary64" if args.hash_engine == 1 and field == "bb" else "uint32",
         "lde_tuning": tuned,
     }
+    ctx.close()
+    if args.workload == DEFAULT and not args.no_extras:
+        # the metric's other two parts, on the same GPU: the 64-trace batched
+        # commit against the plain per-trace loop, and the served form's p50/p99
+        xargs = argparse.Namespace(**vars(args))
+        xargs.steps, xargs.warmup = min(args.steps, 5), min(max(args.warmup, 1), 2)
+        b = bench_batch(p3hip, xargs)
+        out["batched"] = {k: b[k] for k in ("value", "ms_per_step", "plain_loop", "batched_over_plain", "config",
+                                             "cap_equals_oracle_fixture", "autotuned")}
+        out["batched_over_plain"] = b["batched_over_plain"]
+        sv = bench_serve(p3hip, xargs)
+        out["serving"] = sv["serving"]
+        out["p50_ms"], out["p99_ms"] = sv["p50_ms"], sv["p99_ms"]
+        out["served_over_plain"] = sv["vs_plain"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field, width, lb, rp)
         out["cpu_baseline_threads"] = cpu_baseline_threads(field, width, lb, rp)
     print(json.dumps(out))
-    ctx.close()
 
 
 def load_traffic(workload: str) -> dict:
@@ -309,15 +350,6 @@ def calibrate_copy(nbytes: int = 1 << 30, reps: int = 3):
                       "reps": reps}))
 
 
-def batch_traces(seed: int = 7, count: int = 64):
-    """64 BabyBear traces: heights 2^h, h uniform in [12, 20]; widths uniform in
-    [8, 128]; log_blowup 1 or 2; uniform field elements; all from one seed."""
-    rng = np.random.default_rng(seed)
-    shapes = [(int(rng.integers(12, 21)), int(rng.integers(8, 129)), int(rng.integers(1, 3))) for _ in range(count)]
-    traces = [rng.integers(0, BB_P, size=(1 << h, w), dtype=np.uint32) for h, w, _ in shapes]
-    return shapes, traces
-
-
 def bench_batch(p3hip, args):
     """64 BabyBear traces (heights 2^12..2^20, widths 8..128, blowup 2 or 4): one
     multi-height MMCS over 64 LDEs, planned and replayed as a
@@ -326,14 +358,15 @@ def bench_batch(p3hip, args):
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(p3hip.BABYBEAR, p2_constants("bb", 13), 8, 13)
     ctx.set_hash_engine(args.hash_engine)
-    shapes, traces = batch_traces()
-    mats = [ctx.upload(t) for t in traces]
-    del traces
-    ks = [k for _, _, k in shapes]
-    elems = sum((1 << (h + k)) * w for h, w, k in shapes)
+    shapes = W.batch_shapes()
+    mats = [ctx.upload(p3hip.seeded_fill(p3hip.BABYBEAR, s, (1 << h) * w).reshape(1 << h, w))
+            for h, w, _, s in shapes]
+    ks = [k for _, _, k, _ in shapes]
+    cap_height = W.BATCH["cap_height"]
+    elems = W.committed_elems(shapes)
     # autotune the pass shape of the largest LDEs (they dominate the time)
     tuned = {}
-    for h, w, k in sorted(shapes, key=lambda s: -(1 << s[0]) * s[1])[:6]:
+    for h, w, k, _ in sorted(shapes, key=lambda s: -(1 << s[0]) * s[1])[:6]:
         if (h, w, k) not in tuned:
             tuned[(h, w, k)] = ctx.autotune_lde(p3hip.BABYBEAR, h, w, k, iters=1)
 
@@ -356,7 +389,7 @@ def bench_batch(p3hip, args):
     ctx.sync()
     plain_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
-    plan = ctx.batch_plan(mats, ks, 31, cap_height=0, use_graph=True)
+    plan = ctx.batch_plan(mats, ks, 31, cap_height=cap_height, use_graph=True)
     caps = [plan.run() for _ in range(max(1, args.warmup))]
     ctx.sync()
     t0 = time.perf_counter()
@@ -365,23 +398,30 @@ def bench_batch(p3hip, args):
     ctx.sync()
     batch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     assert all((c == caps[0]).all() for c in caps), "non-deterministic batched root"
+    pinned_cap = fixture("batch64/cap")
+    if pinned_cap is not None:
+        assert (caps[0] == pinned_cap).all(), "batched cap differs from the oracle's full-size fixture"
     out = {
-        "metric": "committed field elems/s (LDE+Poseidon2 Merkle), batched vs plain",
+        "metric": METRIC,
         "value": elems / (batch_ms / 1e3),
         "unit": "field elems/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": batch_ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-        "data": "synthetic (64 seeded traces, seeded Poseidon2 constants)",
-        "config": {"workload": "64 BabyBear traces, h in [12,20], w in [8,128], log_blowup 1|2, one multi-height "
-                               "MMCS, hipGraph replay", "committed_elems": elems},
+        "data": "synthetic (64 std::mt19937_64 traces and Poseidon2 constants, seeds of workloads.py)",
+        "config": {"workload": f"64 BabyBear traces, h in [12,20], w in [8,128], log_blowup 1|2, one multi-height "
+                               f"MMCS with a cap of 2^{cap_height}, hipGraph replay; plain: per-trace LDE + "
+                               f"single-matrix commit + root", "committed_elems": elems},
         "plain_loop": {"value": elems / (plain_ms / 1e3), "ms_per_step": plain_ms},
         "batched_over_plain": plain_ms / batch_ms,
         "autotuned": {f"2^{h}x{w}_b{k}": v for (h, w, k), v in tuned.items()},
-        "cap": [int(x) for x in caps[0][0]],
+        "cap0": [int(x) for x in caps[0][0]],
+        "cap_equals_oracle_fixture": None if pinned_cap is None else True,
     }
-    print(json.dumps(out))
     plan.free()
+    for m in mats:
+        m.free()
     ctx.close()
+    return out
 
 
 def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3, challenger: bool = False):
@@ -396,10 +436,11 @@ def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(p3hip.BABYBEAR, consts, 8, 13)
     ctx.set_hash_engine(args.hash_engine)
-    rng = np.random.default_rng(5)
-    cw = rng.integers(0, BB_P, size=(1 << log_len, 4), dtype=np.uint32)
+    fc = W.FRI_BB4
+    assert (log_len, log_blowup, lfpl) == (fc["log_len"], fc["log_blowup"], fc["log_final_poly_len"])
+    cw = p3hip.seeded_fill(p3hip.BABYBEAR, fc["seed"], (1 << log_len) * 4).reshape(-1, 4)
     rounds = log_len - log_blowup - lfpl
-    betas = rng.integers(0, BB_P, size=(rounds, 4), dtype=np.uint32)
+    betas = p3hip.seeded_fill(p3hip.BABYBEAR, fc["seed"] + 1, 4 * rounds).reshape(-1, 4)
     m = ctx.upload(cw)
 
     def step():
@@ -428,6 +469,9 @@ def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3
     phases = {ph: ctx.phase_stats(ph) for ph in ("fold", "leaf_hash", "compress")}
     ctx.set_profiling(False)
     assert all((o[0] == outs[0][0]).all() and (o[1] == outs[0][1]).all() for o in outs), "non-deterministic"
+    pinned = fixture("fri_bb4/roots")
+    if pinned is not None and not challenger:
+        assert (outs[0][0] == pinned).all(), "FRI layer roots differ from the oracle's full-size fixture"
     ms = dt * 1e3 / args.steps
     layer_elems = sum(1 << l for l in range(lfpl + log_blowup + 1, log_len + 1))
     fold_ms = phases["fold"][1] / args.steps
@@ -436,7 +480,8 @@ def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3
         "metric": "committed BabyBear^4 elems/s (FRI commit phase)", "value": layer_elems / (ms / 1e3),
         "unit": "ext field elems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-        "data": "synthetic (seeded codeword, challenges and Poseidon2 constants)",
+        "data": "synthetic (std::mt19937_64 codeword, challenges and Poseidon2 constants)",
+        "roots_equal_oracle_fixture": None if pinned is None or challenger else True,
         "config": {"workload": f"FRI commit phase, BabyBear^4 codeword 2^{log_len}, arity 2, log_blowup {log_blowup}, "
                                f"final poly 2^{lfpl}, {rounds} layer commits, "
                                + ("challenges from the duplex transcript" if challenger else "seeded challenges"),
@@ -453,9 +498,7 @@ def bench_fri(p3hip, args, log_len: int = 23, log_blowup: int = 1, lfpl: int = 3
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import native  # test infrastructure; used here only as the CPU baseline
 
-        lib, kind = native.reference(), "reference"
-        if lib is None:
-            lib, kind = native.oracle(), "port"
+        lib, kind = native.oracle(), "port"
         sl = 18
         t0 = time.perf_counter()
         lib.fri_commit(consts, 8, 13, cw[: 1 << sl], log_blowup, lfpl, 1, betas[: sl - log_blowup - lfpl])
@@ -501,9 +544,7 @@ def bench_grind(p3hip, args, bits: int = 24, cpu_bits: int = 13):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import native  # test infrastructure; used here only as the CPU baseline
 
-        lib, kind = native.reference(), "reference"
-        if lib is None:
-            lib, kind = native.oracle(), "port"
+        lib, kind = native.oracle(), "port"
         ctries, ctime = 0, 0.0
         for rep in range(8):
             ops = [(native.OBSERVE, int(v)) for v in prefix] + [(native.OBSERVE, rep), (native.GRIND, cpu_bits)]
@@ -575,9 +616,7 @@ def bench_pcs_open(p3hip, args, log_n: int = 20, width: int = 256, n_points: int
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import native  # test infrastructure; used here only as the CPU baseline
 
-        lib, kind = native.reference(), "reference"
-        if lib is None:
-            lib, kind = native.oracle(), "port"
+        lib, kind = native.oracle(), "port"
         sl = 12
         m = rng.integers(0, BB_P, size=(1 << sl, width), dtype=np.uint32)
         t0 = time.perf_counter()
@@ -596,7 +635,7 @@ def _pct(xs, q):
     return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))]
 
 
-def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 32):
+def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24):
     """Open-loop Poisson commit requests (BabyBear, log_blowup 1, heights
     2^[12,20], widths [8,128], 64 leaf openings each) at three arrival rates
     plus a saturating burst.  The served form (continuous batching over 4
@@ -605,13 +644,15 @@ def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 
     context, FIFO: upload, LDE, commit, root, openings, free) on the same
     arrivals.  Latency is scheduled arrival -> root on the host.  The request
     pool sits in page-locked host memory for both paths (uploads at the PCIe
-    DMA rate); one burst from pageable copies of the pool is reported beside."""
+    DMA rate); one burst from pageable copies of the pool is reported beside.
+    The HBM budget is the whole device (288 GB on MI355X)."""
     import threading
 
     consts = p2_constants("bb", 13)
     rng = np.random.default_rng(11)
-    shapes = [(int(rng.integers(12, 21)), int(rng.integers(8, 129))) for _ in range(pool)]
-    pageable = [rng.integers(0, BB_P, size=(1 << h, w), dtype=np.uint32) for h, w in shapes]
+    pool_shapes = W.batch_shapes(0x51, pool, blowups=(1,))
+    shapes = [(h, w) for h, w, _, _ in pool_shapes]
+    pageable = [p3hip.seeded_fill(p3hip.BABYBEAR, s, (1 << h) * w).reshape(1 << h, w) for h, w, _, s in pool_shapes]
     pinned = []
     for t in pageable:
         pinned.append(p3hip.pinned_empty(t.shape, np.uint32))
@@ -653,7 +694,7 @@ def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 
         return lat, time.perf_counter() - start
 
     def served_run(arrivals):
-        with p3hip.Server(consts, 8, 13, hbm_budget=budget_gib << 30, n_slots=4, max_batch=8) as sv:
+        with p3hip.Server(consts, 8, 13, hbm_budget=0, n_slots=4, max_batch=8) as sv:
             ids, lag = [None] * n_req, [0.0] * n_req
 
             def submitter():
@@ -696,7 +737,8 @@ def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 
     _, burst_pageable = burst()
     traces = pinned
     span_p, burst_pinned = burst()
-    res = {"budget_gib": budget_gib, "committed_elems": elems, "host_buffers": "pinned", "h2d": h2d,
+    budget = ctx.memory()["budget"]
+    res = {"budget_bytes": budget, "committed_elems": elems, "host_buffers": "pinned", "h2d": h2d,
            "uploaded_bytes": int(sum(pageable[i].nbytes for i in order)),
            "burst": burst_pinned, "burst_pageable": burst_pageable, "rates": []}
     svc = span_p / n_req  # plain per-request service time
@@ -713,20 +755,20 @@ def bench_serve(p3hip, args, n_req: int = 96, pool: int = 24, budget_gib: int = 
         })
     top = res["rates"][-1]
     out = {
-        "metric": "committed field elems/s (served, Poisson arrivals at 1.6x plain capacity)",
+        "metric": METRIC + " (served, Poisson arrivals at 1.6x plain capacity)",
         "value": top["served"]["elems_per_s"], "unit": "field elems/s",
         "n_gpus": 1, "steps": n_req, "warmup": n_req, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (seeded traces, Poisson arrivals)",
         "config": {"workload": f"{n_req} Poisson commit requests over {pool} BabyBear traces, h in [12,20], "
                                f"w in [8,128], log_blowup 1, 64 openings each; 4 slots, max batch 8, "
-                               f"{budget_gib} GiB HBM budget; traces in pinned host memory"},
+                               f"HBM budget {budget / 1e9:.0f} GB (whole device); traces in pinned host memory"},
         "p50_ms": top["served"]["p50_ms"], "p99_ms": top["served"]["p99_ms"],
         "vs_plain": top["served"]["elems_per_s"] / top["plain"]["elems_per_s"],
         "serving": res,
     }
-    print(json.dumps(out))
     ctx.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -61,6 +61,16 @@ enum {
 int p3h_host_alloc(uint64_t bytes, void** out);
 int p3h_host_free(void* p);
 
+/* ------------------------------------------------------ synthetic inputs */
+/* Host-side fill of n canonical field elements: element i is the i-th output
+ * of std::mt19937_64(seed) reduced mod p (uint32 for BabyBear, uint64 for
+ * Goldilocks), row-major when read as a matrix.  This is the synthetic input
+ * stream of the benchmark configurations; the reference's examples draw their
+ * matrices the same way from a 32-bit std::mt19937, rng() % p
+ * (examples/merkle_tree_cuda_example.cu:53-66).
+ * No device work. */
+int p3h_seeded_fill(int field, uint64_t seed, void* out, uint64_t n);
+
 /* ---------------------------------------------------------------- context */
 /* hbm_budget_bytes = 0 means "all device memory". */
 int p3h_ctx_create(int device, uint64_t hbm_budget_bytes, p3h_ctx** out);

@@ -48,8 +48,20 @@ struct GL {
     static uint64_t sub(uint64_t a, uint64_t b) {
         return uint64_t((static_cast<unsigned __int128>(a) + P - b) % P);
     }
+    // reduce128 (goldilocks.hpp:135-162): with x = lo + 2^64 hl + 2^96 hh and
+    // 2^64 = 2^32 - 1, 2^96 = -1 (mod p), x = lo - hh + hl (2^32 - 1); each
+    // wrap of the 64-bit word is worth 2^32 - 1; the result is made canonical.
     static uint64_t mul(uint64_t a, uint64_t b) {
-        return uint64_t((static_cast<unsigned __int128>(a) * b) % P);
+        constexpr uint64_t E = 0xFFFFFFFFull;  // 2^64 mod p
+        const unsigned __int128 x = static_cast<unsigned __int128>(a) * b;
+        const uint64_t lo = uint64_t(x), hi = uint64_t(x >> 64);
+        const uint64_t hh = hi >> 32, hl = hi & E;
+        uint64_t t0 = lo - hh;
+        if (lo < hh) t0 -= E;
+        const uint64_t t1 = hl * E;
+        uint64_t t2 = t0 + t1;
+        if (t2 < t1) t2 += E;
+        return t2 >= P ? t2 - P : t2;
     }
     static uint64_t from_u64(uint64_t v) { return v % P; }
     // Not defined by the reference: generator 7, omega_{2^32} = 7^((p-1)/2^32).
@@ -389,26 +401,46 @@ void lde_brute(const typename F::Raw* in, size_t log_h, size_t w, size_t added,
     }
 }
 
+// Runs body(lo, hi) over [0, n) split into `threads` contiguous ranges.  The
+// threaded entry points below use it only for work whose parts are
+// independent (columns of a transform, leaves, nodes of one tree level, rows of
+// a fold), so their results equal the single-threaded ones.
+template <class Body>
+void parallel_for(size_t n, int threads, Body body) {
+    const size_t t = std::max<size_t>(1, std::min<size_t>(size_t(threads), n));
+    if (t == 1) {
+        body(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t i = 0; i < t; ++i) pool.emplace_back(body, n * i / t, n * (i + 1) / t);
+    for (auto& th : pool) th.join();
+}
+
 // ------------------------------------------------------------------- MMCS ---
 // p3_merkle::MerkleTreeMmcs::commit_matrix (fri_merkle_tree_mmcs.hpp:55-102):
 // leaf i = sponge(row i); each layer pairs (2j, 2j+1), a missing right child of
 // an odd-sized layer is the all-zero digest; stop at one node.
 template <class F>
 std::vector<std::vector<Digest<F>>> flat_tree(const Perm<F, Hcfg<F>::W>& perm,
-                                              const typename F::Raw* d, size_t h, size_t w) {
+                                              const typename F::Raw* d, size_t h, size_t w, int threads = 1) {
     if (w == 0 || h == 0) throw std::invalid_argument("empty matrix");
     std::vector<std::vector<Digest<F>>> tree(1);
     tree[0].resize(h);
-    for (size_t i = 0; i < h; ++i) tree[0][i] = sponge<F>(perm, d + i * w, w);
+    parallel_for(h, threads, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) tree[0][i] = sponge<F>(perm, d + i * w, w);
+    });
     while (tree.back().size() > 1) {
-        const auto prev = tree.back();
+        const auto& prev = tree.back();
         size_t nxt = (prev.size() + 1) / 2;
         std::vector<Digest<F>> layer(nxt);
-        for (size_t j = 0; j < nxt; ++j) {
-            Digest<F> r{};
-            if (2 * j + 1 < prev.size()) r = prev[2 * j + 1];
-            layer[j] = compress<F>(perm, prev[2 * j], r);
-        }
+        parallel_for(nxt, nxt >= 4096 ? threads : 1, [&](size_t j0, size_t j1) {
+            for (size_t j = j0; j < j1; ++j) {
+                Digest<F> r{};
+                if (2 * j + 1 < prev.size()) r = prev[2 * j + 1];
+                layer[j] = compress<F>(perm, prev[2 * j], r);
+            }
+        });
         tree.push_back(std::move(layer));
     }
     return tree;
@@ -426,7 +458,7 @@ struct MultiTree {
 
 template <class F>
 MultiTree<F> multi_tree(const Perm<F, Hcfg<F>::W>& perm, size_t n, const size_t* hs, const size_t* ws,
-                        const typename F::Raw* const* mats, size_t cap_height) {
+                        const typename F::Raw* const* mats, size_t cap_height, int threads = 1) {
     using R = typename F::Raw;
     if (n == 0) throw std::invalid_argument("no matrices");
     std::vector<size_t> order(n);
@@ -448,16 +480,21 @@ MultiTree<F> multi_tree(const Perm<F, Hcfg<F>::W>& perm, size_t n, const size_t*
     MultiTree<F> t;
     auto top = take(maxh);
     t.layers.emplace_back(maxh);
-    for (size_t i = 0; i < maxh; ++i) t.layers[0][i] = hash_rows(i, top);
+    parallel_for(maxh, threads, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) t.layers[0][i] = hash_rows(i, top);
+    });
     size_t level = maxh, cap = size_t(1) << cap_height;
     while (level > cap) {
         level /= 2;
         std::vector<Digest<F>> nl(level);
         const auto& prev = t.layers.back();
-        for (size_t i = 0; i < level; ++i) nl[i] = compress<F>(perm, prev[2 * i], prev[2 * i + 1]);
         auto inj = take(level);
-        if (!inj.empty())
-            for (size_t i = 0; i < level; ++i) nl[i] = compress<F>(perm, nl[i], hash_rows(i, inj));
+        parallel_for(level, level >= 1024 ? threads : 1, [&](size_t i0, size_t i1) {
+            for (size_t i = i0; i < i1; ++i) {
+                nl[i] = compress<F>(perm, prev[2 * i], prev[2 * i + 1]);
+                if (!inj.empty()) nl[i] = compress<F>(perm, nl[i], hash_rows(i, inj));
+            }
+        });
         t.layers.push_back(std::move(nl));
     }
     return t;
@@ -573,11 +610,14 @@ E4 fold_row(size_t i, size_t log_height, size_t log_arity, const E4& beta, const
     return res;
 }
 
-std::vector<E4> fold(size_t log_height, size_t log_arity, const E4& beta, const std::vector<E4>& cur) {
+std::vector<E4> fold(size_t log_height, size_t log_arity, const E4& beta, const std::vector<E4>& cur,
+                     int threads = 1) {
     size_t arity = size_t(1) << log_arity;
     if (cur.size() % arity) throw std::invalid_argument("fold: size");
     std::vector<E4> out(cur.size() / arity);
-    for (size_t i = 0; i < out.size(); ++i) out[i] = fold_row(i, log_height, log_arity, beta, &cur[i * arity]);
+    parallel_for(out.size(), out.size() >= 1024 ? threads : 1, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) out[i] = fold_row(i, log_height, log_arity, beta, &cur[i * arity]);
+    });
     return out;
 }
 
@@ -690,55 +730,73 @@ std::vector<E4> final_poly(std::vector<E4> cur, size_t lfpl) {
     return res;
 }
 
-// LDE + flat commit on `threads` host threads: the CPU baseline's multi-core
-// figure.  Every column's transform is independent, so the LDE runs on column
-// slabs (one per thread, lde() above on each); leaf hashes split the rows and
-// each wide tree level splits its nodes.  The arithmetic is the single-threaded
-// code's, so the root is the same (tests/test_oracle.py checks it).
-template <class Body>
-void parallel_for(size_t n, int threads, Body body) {
-    const size_t t = std::max<size_t>(1, std::min<size_t>(size_t(threads), n));
-    if (t == 1) {
-        body(size_t(0), n);
-        return;
-    }
-    std::vector<std::thread> pool;
-    for (size_t i = 0; i < t; ++i) pool.emplace_back(body, n * i / t, n * (i + 1) / t);
-    for (auto& th : pool) th.join();
-}
-
+// LDE on `threads` host threads: every column's transform is independent, so
+// the columns are split into slabs, one per thread, each run through lde()
+// above.  Natural row order, N x w.
 template <class F>
-Digest<F> lde_commit_mt(const typename F::Raw* c, int rf, int rp, const typename F::Raw* in, size_t log_h,
-                        size_t w, size_t added, typename F::Raw shift, int threads) {
+std::vector<typename F::Raw> lde_mt(const typename F::Raw* in, size_t log_h, size_t w, size_t added,
+                                    typename F::Raw shift, int threads) {
     using R = typename F::Raw;
     if (w == 0) throw std::invalid_argument("empty matrix");
     const size_t h = size_t(1) << log_h, N = h << added;
-    std::vector<R> lde_m(N * w);
+    std::vector<R> out(N * w);
     parallel_for(w, threads, [&](size_t c0, size_t c1) {
         if (c0 == c1) return;
         const size_t wc = c1 - c0;
         std::vector<R> slab(h * wc);
         for (size_t i = 0; i < h; ++i) std::copy(in + i * w + c0, in + i * w + c1, slab.begin() + i * wc);
         Mat<F> m = lde<F>(slab.data(), log_h, wc, added, shift);
-        for (size_t i = 0; i < N; ++i) std::copy(m.row(i), m.row(i) + wc, lde_m.begin() + i * w + c0);
+        slab = std::vector<R>();
+        for (size_t i = 0; i < N; ++i) std::copy(m.row(i), m.row(i) + wc, out.begin() + i * w + c0);
     });
+    return out;
+}
+
+// LDE + flat commit on `threads` host threads: the CPU baseline's multi-core
+// figure and the source of the full-size fixtures.  The arithmetic is the
+// single-threaded code's, so the root is the same (tests/test_oracle.py checks
+// it).
+template <class F>
+Digest<F> lde_commit_mt(const typename F::Raw* c, int rf, int rp, const typename F::Raw* in, size_t log_h,
+                        size_t w, size_t added, typename F::Raw shift, int threads) {
+    auto lde_m = lde_mt<F>(in, log_h, w, added, shift, threads);
     const Perm<F, Hcfg<F>::W> perm(c, rf, rp);
-    std::vector<Digest<F>> layer(N);
-    parallel_for(N, threads, [&](size_t r0, size_t r1) {
-        for (size_t i = r0; i < r1; ++i) layer[i] = sponge<F>(perm, lde_m.data() + i * w, w);
-    });
-    while (layer.size() > 1) {
-        std::vector<Digest<F>> nxt((layer.size() + 1) / 2);
-        parallel_for(nxt.size(), nxt.size() >= 4096 ? threads : 1, [&](size_t j0, size_t j1) {
-            for (size_t j = j0; j < j1; ++j) {
-                Digest<F> r{};
-                if (2 * j + 1 < layer.size()) r = layer[2 * j + 1];
-                nxt[j] = compress<F>(perm, layer[2 * j], r);
-            }
-        });
-        layer = std::move(nxt);
+    return flat_tree<F>(perm, lde_m.data(), (size_t(1) << log_h) << added, w, threads).back()[0];
+}
+
+// Opening of a flat tree (open_row, fri_merkle_tree_mmcs.hpp:129-141): the row
+// and one sibling per level below the root, leaf level first.
+template <class F>
+void flat_open_rows(const std::vector<std::vector<Digest<F>>>& t, const typename F::Raw* m, size_t w,
+                    const uint64_t* idx, size_t n_idx, typename F::Raw* values, typename F::Raw* proof) {
+    constexpr size_t DIG = Hcfg<F>::DIG;
+    const size_t steps = t.size() - 1, h = t[0].size();
+    for (size_t q = 0; q < n_idx; ++q) {
+        if (idx[q] >= h) throw std::invalid_argument("index out of range");
+        std::copy(m + idx[q] * w, m + (idx[q] + 1) * w, values + q * w);
+        size_t cur = idx[q];
+        for (size_t s = 0; s < steps; ++s) {
+            Digest<F> sib{};
+            if ((cur ^ 1) < t[s].size()) sib = t[s][cur ^ 1];
+            std::copy(sib.begin(), sib.end(), proof + (q * steps + s) * DIG);
+            cur >>= 1;
+        }
     }
-    return layer[0];
+}
+
+// LDE + flat commit on `threads` host threads with openings at n_idx LDE rows:
+// values_out n_idx x w (natural row order), proof_out n_idx x (log_h + added)
+// digests, leaf level first.
+template <class F>
+int lde_commit_open_entry(const typename F::Raw* c, int rf, int rp, const typename F::Raw* in, size_t log_h,
+                          size_t w, size_t added, typename F::Raw shift, int threads, const uint64_t* idx,
+                          size_t n_idx, typename F::Raw* root, typename F::Raw* values, typename F::Raw* proof) {
+    auto lde_m = lde_mt<F>(in, log_h, w, added, shift % F::P, threads);
+    const Perm<F, Hcfg<F>::W> perm(c, rf, rp);
+    auto t = flat_tree<F>(perm, lde_m.data(), (size_t(1) << log_h) << added, w, threads);
+    std::copy(t.back()[0].begin(), t.back()[0].end(), root);
+    flat_open_rows<F>(t, lde_m.data(), w, idx, n_idx, values, proof);
+    return 0;
 }
 
 template <class Fn>
@@ -929,6 +987,71 @@ int orc_gl_lde_commit_mt(const uint64_t* c, int rf, int rp, const uint64_t* in, 
     });
 }
 
+int orc_bb_lde_commit_open_mt(const uint32_t* c, int rf, int rp, const uint32_t* in, size_t log_h, size_t w,
+                              size_t added, uint32_t shift, int threads, const uint64_t* idx, size_t n_idx,
+                              uint32_t* root, uint32_t* values, uint32_t* proof) {
+    return guarded([&] {
+        return lde_commit_open_entry<BB>(c, rf, rp, in, log_h, w, added, shift, threads, idx, n_idx, root, values,
+                                         proof);
+    });
+}
+
+int orc_gl_lde_commit_open_mt(const uint64_t* c, int rf, int rp, const uint64_t* in, size_t log_h, size_t w,
+                              size_t added, uint64_t shift, int threads, const uint64_t* idx, size_t n_idx,
+                              uint64_t* root, uint64_t* values, uint64_t* proof) {
+    return guarded([&] {
+        return lde_commit_open_entry<GL>(c, rf, rp, in, log_h, w, added, shift, threads, idx, n_idx, root, values,
+                                         proof);
+    });
+}
+
+// Batched commit of BabyBear traces: each trace's coset LDE (coset_lde_batch,
+// radix2_dit.hpp:221-233, with its own added bits and the common shift), then
+// one multi-height MMCS over the LDEs in trace order (build_merkle_tree,
+// merkle_tree.hpp:82-185) with openings (open_batch,
+// merkle_tree_mmcs.hpp:120-152) at n_idx indices of the tallest LDE.
+// values_out: per index, the opened row of every LDE in trace order;
+// proof_out: per index, *proof_len digests.
+int orc_bb_batch_commit_mt(const uint32_t* c, int rf, int rp, size_t n, const size_t* log_hs, const size_t* ws,
+                           const size_t* adds, const uint32_t* const* traces, uint32_t shift, size_t cap_height,
+                           int threads, const uint64_t* idx, size_t n_idx, uint32_t* cap_out, uint32_t* values_out,
+                           uint32_t* proof_out, size_t* proof_len) {
+    return guarded([&] {
+        std::vector<std::vector<uint32_t>> ldes(n);
+        std::vector<size_t> hs(n);
+        std::vector<const uint32_t*> ptrs(n);
+        for (size_t m = 0; m < n; ++m) {
+            ldes[m] = lde_mt<BB>(traces[m], log_hs[m], ws[m], adds[m], shift % BB::P, threads);
+            hs[m] = (size_t(1) << log_hs[m]) << adds[m];
+            ptrs[m] = ldes[m].data();
+        }
+        const Perm<BB, 16> perm(c, rf, rp);
+        auto t = multi_tree<BB>(perm, n, hs.data(), ws, ptrs.data(), cap_height, threads);
+        size_t k = 0;
+        for (const auto& d : t.layers.back())
+            for (size_t i = 0; i < 8; ++i) cap_out[k++] = d[i];
+        const size_t maxh = t.layers[0].size(), steps = t.layers.size() - 1;
+        size_t tw = 0;
+        for (size_t m = 0; m < n; ++m) tw += ws[m];
+        for (size_t q = 0; q < n_idx; ++q) {
+            if (idx[q] >= maxh) return -2;
+            size_t v = q * tw;
+            for (size_t m = 0; m < n; ++m) {
+                const size_t row = idx[q] / (maxh / hs[m]);
+                std::copy(ptrs[m] + row * ws[m], ptrs[m] + (row + 1) * ws[m], values_out + v);
+                v += ws[m];
+            }
+            size_t cur = idx[q];
+            for (size_t s = 0; s < steps; ++s) {
+                std::copy(t.layers[s][cur ^ 1].begin(), t.layers[s][cur ^ 1].end(), proof_out + (q * steps + s) * 8);
+                cur >>= 1;
+            }
+        }
+        *proof_len = steps;
+        return 0;
+    });
+}
+
 // MerkleTreeMmcs::open_row (fri_merkle_tree_mmcs.hpp:129-141): siblings, with a
 // zero digest where the sibling falls past an odd layer's end.
 int orc_bb_flat_open(const uint32_t* c, int rf, int rp, const uint32_t* d, size_t h, size_t w,
@@ -985,9 +1108,10 @@ int orc_bb4_fold(size_t log_height, size_t log_arity, const uint32_t* beta, cons
 // (len/arity) x arity matrix of BabyBear^4 flattened to 4*arity base elements
 // per row, then the flat MMCS.  Final polynomial (fri_prover.hpp:147-178):
 // truncate to 2^log_final_poly_len, bit-reverse, inverse DFT (naive).
-int orc_bb4_fri_commit(const uint32_t* c, int rf, int rp, const uint32_t* codeword, size_t log_len,
-                       size_t log_blowup, size_t log_final_poly_len, size_t max_log_arity,
-                       const uint32_t* betas, uint32_t* roots_out, uint32_t* final_out, size_t* n_rounds) {
+int orc_bb4_fri_commit_mt(const uint32_t* c, int rf, int rp, const uint32_t* codeword, size_t log_len,
+                          size_t log_blowup, size_t log_final_poly_len, size_t max_log_arity,
+                          const uint32_t* betas, int threads, uint32_t* roots_out, uint32_t* final_out,
+                          size_t* n_rounds) {
     return guarded([&] {
         Perm<BB, 16> perm(c, rf, rp);
         auto cur = load_e4(codeword, size_t(1) << log_len);
@@ -1000,10 +1124,10 @@ int orc_bb4_fri_commit(const uint32_t* c, int rf, int rp, const uint32_t* codewo
             std::vector<uint32_t> flat(cur.size() * 4);
             for (size_t i = 0; i < cur.size(); ++i)
                 for (int k = 0; k < 4; ++k) flat[4 * i + k] = cur[i][k];
-            auto t = flat_tree<BB>(perm, flat.data(), rows, 4 * arity);
+            auto t = flat_tree<BB>(perm, flat.data(), rows, 4 * arity, threads);
             std::copy(t.back()[0].begin(), t.back()[0].end(), roots_out + 8 * round);
             const uint32_t* bp = betas + 4 * round;
-            cur = fold(lh, la, E4{bp[0] % BB::P, bp[1] % BB::P, bp[2] % BB::P, bp[3] % BB::P}, cur);
+            cur = fold(lh, la, E4{bp[0] % BB::P, bp[1] % BB::P, bp[2] % BB::P, bp[3] % BB::P}, cur, threads);
             ++round;
         }
         size_t n = size_t(1) << log_final_poly_len;
@@ -1021,6 +1145,13 @@ int orc_bb4_fri_commit(const uint32_t* c, int rf, int rp, const uint32_t* codewo
         *n_rounds = round;
         return 0;
     });
+}
+
+int orc_bb4_fri_commit(const uint32_t* c, int rf, int rp, const uint32_t* codeword, size_t log_len,
+                       size_t log_blowup, size_t log_final_poly_len, size_t max_log_arity,
+                       const uint32_t* betas, uint32_t* roots_out, uint32_t* final_out, size_t* n_rounds) {
+    return orc_bb4_fri_commit_mt(c, rf, rp, codeword, log_len, log_blowup, log_final_poly_len, max_log_arity, betas,
+                                 1, roots_out, final_out, n_rounds);
 }
 
 // TwoAdicFriPcs::commit (two_adic_fri_pcs.hpp:207-266).

@@ -72,6 +72,24 @@ int orc_bb_lde_commit_mt(const uint32_t* consts, int rf, int rp, const uint32_t*
                          size_t added, uint32_t shift, int threads, uint32_t* root8);
 int orc_gl_lde_commit_mt(const uint64_t* consts, int rf, int rp, const uint64_t* in, size_t log_h, size_t w,
                          size_t added, uint64_t shift, int threads, uint64_t* root4);
+/* the same on `threads` host threads, with openings at n_idx LDE rows:
+ * values n_idx x w (natural row order), proof n_idx x (log_h + added) digests,
+ * leaf level first */
+int orc_bb_lde_commit_open_mt(const uint32_t* consts, int rf, int rp, const uint32_t* in, size_t log_h, size_t w,
+                              size_t added, uint32_t shift, int threads, const uint64_t* idx, size_t n_idx,
+                              uint32_t* root8, uint32_t* values, uint32_t* proof);
+int orc_gl_lde_commit_open_mt(const uint64_t* consts, int rf, int rp, const uint64_t* in, size_t log_h, size_t w,
+                              size_t added, uint64_t shift, int threads, const uint64_t* idx, size_t n_idx,
+                              uint64_t* root4, uint64_t* values, uint64_t* proof);
+/* batched commit on `threads` host threads: LDE of every BabyBear trace (its
+ * own added bits, common shift), one multi-height MMCS over the LDEs in trace
+ * order with a cap of 2^cap_height digests, openings at n_idx indices of the
+ * tallest LDE (values: every LDE's row in trace order; proof: *proof_len
+ * digests per index) */
+int orc_bb_batch_commit_mt(const uint32_t* consts, int rf, int rp, size_t n, const size_t* log_hs, const size_t* ws,
+                           const size_t* adds, const uint32_t* const* traces, uint32_t shift, size_t cap_height,
+                           int threads, const uint64_t* idx, size_t n_idx, uint32_t* cap_out, uint32_t* values_out,
+                           uint32_t* proof_out, size_t* proof_len);
 /* single-matrix opening: row values and log2(height) sibling digests */
 int orc_bb_flat_open(const uint32_t* consts, int rf, int rp, const uint32_t* data, size_t height,
                      size_t width, size_t index, uint32_t* row_out, uint32_t* proof_out);
@@ -99,6 +117,12 @@ int orc_bb4_fri_commit(const uint32_t* consts, int rf, int rp, const uint32_t* c
                        size_t log_len, size_t log_blowup, size_t log_final_poly_len,
                        size_t max_log_arity, const uint32_t* betas, uint32_t* roots_out,
                        uint32_t* final_out, size_t* n_rounds);
+
+/* the FRI commit phase with the folds and layer trees on `threads` host threads */
+int orc_bb4_fri_commit_mt(const uint32_t* consts, int rf, int rp, const uint32_t* codeword,
+                          size_t log_len, size_t log_blowup, size_t log_final_poly_len,
+                          size_t max_log_arity, const uint32_t* betas, int threads, uint32_t* roots_out,
+                          uint32_t* final_out, size_t* n_rounds);
 
 /* PCS commit: LDE every matrix (added bits = log_blowup, shift per matrix),
  * concatenate rows cyclically, zero-pad the width to a power of two, flat commit */

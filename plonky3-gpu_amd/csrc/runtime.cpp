@@ -21,6 +21,7 @@
 #include <functional>
 #include <map>
 #include <numeric>
+#include <random>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -1599,6 +1600,21 @@ int p3h_host_alloc(uint64_t bytes, void** out) {
 int p3h_host_free(void* p) {
     if (!p) return P3H_OK;
     return hipHostFree(p) == hipSuccess ? P3H_OK : P3H_EDEVICE;
+}
+
+int p3h_seeded_fill(int field, uint64_t seed, void* out, uint64_t n) {
+    if (!out && n) return P3H_EINVAL;
+    std::mt19937_64 rng(seed);
+    if (field == P3H_BABYBEAR) {
+        uint32_t* o = static_cast<uint32_t*>(out);
+        for (uint64_t i = 0; i < n; ++i) o[i] = uint32_t(rng() % 0x78000001ull);
+    } else if (field == P3H_GOLDILOCKS) {
+        uint64_t* o = static_cast<uint64_t*>(out);
+        for (uint64_t i = 0; i < n; ++i) o[i] = rng() % 0xFFFFFFFF00000001ull;
+    } else {
+        return P3H_EINVAL;
+    }
+    return P3H_OK;
 }
 
 int p3h_ctx_trim(p3h_ctx* ctx) {

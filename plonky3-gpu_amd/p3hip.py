@@ -64,6 +64,7 @@ _SIGS = {
     "p3h_ctx_set_hash_engine": (_i32, [_vp, _i32]),
     "p3h_host_alloc": (_i32, [_u64, C.POINTER(_vp)]),
     "p3h_host_free": (_i32, [_vp]),
+    "p3h_seeded_fill": (_i32, [_i32, _u64, _vp, _u64]),
     "p3h_poseidon2_set": (_i32, [_vp, _i32, _u32, _u32, _u32, _vp]),
     "p3h_poseidon2_permute": (_i32, [_vp, _i32, _vp, _u64]),
     "p3h_mat_upload": (_i32, [_vp, _i32, _vp, _u64, _u32, C.POINTER(_vp)]),
@@ -174,6 +175,21 @@ def pinned_empty(shape, dtype) -> PinnedArray:
     # every numpy view of the memory keeps `buf` alive, and `buf` the block
     buf._p3h_owner = _PinnedBlock(lib, p.value)
     return np.frombuffer(buf, dtype=dtype, count=nbytes // dtype.itemsize).reshape(shape).view(PinnedArray)
+
+
+def seeded_fill(field: int, seed: int, n: int, out: np.ndarray | None = None) -> np.ndarray:
+    """n canonical field elements, element i = the i-th std::mt19937_64(seed)
+    output mod p (p3h_seeded_fill; host only, no device work)."""
+    lib = load_library()
+    dt = DTYPE[field]
+    if out is None:
+        out = np.empty(n, dtype=dt)
+    if out.dtype != dt or out.size < n or not out.flags.c_contiguous:
+        raise ValueError("seeded_fill: output must be a contiguous array of the field's dtype")
+    rc = lib.p3h_seeded_fill(field, seed, out.ctypes.data_as(_vp), n)
+    if rc != _OK:
+        raise ValueError(f"p3h_seeded_fill failed with status {rc}")
+    return out
 
 
 def _ptr(a: np.ndarray):

@@ -138,6 +138,52 @@ class Native:
                  "lde_commit_mt")
         return root
 
+    def lde_commit_open_mt(self, f, consts, rf, rp, mat, added, shift, threads, indices):
+        """LDE + flat commit on `threads` host threads, with openings: (root,
+        values n_idx x w, proofs n_idx x levels x DIG)."""
+        m = np.ascontiguousarray(mat, dtype=DT[f])
+        h, w = m.shape
+        c = np.ascontiguousarray(consts, dtype=DT[f])
+        idx = np.ascontiguousarray(indices, dtype=U64)
+        levels = (h << added).bit_length() - 1
+        root = np.zeros(DIG[f], dtype=DT[f])
+        vals = np.zeros(max(1, idx.size * w), dtype=DT[f])
+        proof = np.zeros(max(1, idx.size * levels * DIG[f]), dtype=DT[f])
+        sh = C.c_uint32(shift) if f == "bb" else C.c_uint64(shift)
+        self._rc(self.fn(f"{f}_lde_commit_open_mt")(_p(c), C.c_int(rf), C.c_int(rp), _p(m),
+                                                    C.c_size_t(h.bit_length() - 1), C.c_size_t(w),
+                                                    C.c_size_t(added), sh, C.c_int(threads), _p(idx),
+                                                    C.c_size_t(idx.size), _p(root), _p(vals), _p(proof)),
+                 "lde_commit_open_mt")
+        return (root, vals[: idx.size * w].reshape(idx.size, w),
+                proof[: idx.size * levels * DIG[f]].reshape(idx.size, levels, DIG[f]))
+
+    def batch_commit_mt(self, consts, rf, rp, traces, adds, shift, cap_height, threads, indices):
+        """LDE of every BabyBear trace, one multi-height MMCS over them, with
+        openings: (cap, values n_idx x total_width, proofs n_idx x levels x 8)."""
+        ts = [np.ascontiguousarray(t, dtype=U32) for t in traces]
+        n = len(ts)
+        lhs = (C.c_size_t * n)(*[t.shape[0].bit_length() - 1 for t in ts])
+        ws = (C.c_size_t * n)(*[t.shape[1] for t in ts])
+        ks = (C.c_size_t * n)(*[int(k) for k in adds])
+        ptrs = (C.c_void_p * n)(*[t.ctypes.data for t in ts])
+        c = np.ascontiguousarray(consts, dtype=U32)
+        idx = np.ascontiguousarray(indices, dtype=U64)
+        tw = sum(t.shape[1] for t in ts)
+        maxn = max(t.shape[0] << int(k) for t, k in zip(ts, adds))
+        levels = maxn.bit_length() - 1 - cap_height
+        cap = np.zeros((1 << cap_height) * 8, dtype=U32)
+        vals = np.zeros(max(1, idx.size * tw), dtype=U32)
+        proof = np.zeros(max(1, idx.size * levels * 8), dtype=U32)
+        plen = C.c_size_t(0)
+        self._rc(self.fn("bb_batch_commit_mt")(_p(c), C.c_int(rf), C.c_int(rp), C.c_size_t(n), lhs, ws, ks, ptrs,
+                                               C.c_uint32(shift), C.c_size_t(cap_height), C.c_int(threads), _p(idx),
+                                               C.c_size_t(idx.size), _p(cap), _p(vals), _p(proof), C.byref(plen)),
+                 "batch_commit_mt")
+        assert plen.value == levels
+        return (cap.reshape(-1, 8), vals[: idx.size * tw].reshape(idx.size, tw),
+                proof[: idx.size * levels * 8].reshape(idx.size, levels, 8))
+
     def mmcs(self, f, consts, rf, rp, mats, cap_height=0, open_index=-1):
         ms = [np.ascontiguousarray(m, dtype=DT[f]) for m in mats]
         n = len(ms)
@@ -202,7 +248,7 @@ class Native:
                                      C.c_size_t(c.size // 4), _p(out)), "fold")
         return out.reshape(-1, 4)
 
-    def fri_commit(self, consts, rf, rp, codeword, log_blowup, log_final_poly_len, max_log_arity, betas):
+    def fri_commit(self, consts, rf, rp, codeword, log_blowup, log_final_poly_len, max_log_arity, betas, threads=1):
         cw = np.ascontiguousarray(codeword, dtype=U32).reshape(-1)
         c = np.ascontiguousarray(consts, dtype=U32)
         b = np.ascontiguousarray(betas, dtype=U32).reshape(-1)
@@ -210,10 +256,16 @@ class Native:
         roots = np.zeros(64 * 8, dtype=U32)
         final = np.zeros(4 << log_final_poly_len, dtype=U32)
         nr = C.c_size_t(0)
-        self._rc(self.fn("bb4_fri_commit")(_p(c), C.c_int(rf), C.c_int(rp), _p(cw), C.c_size_t(log_len),
-                                           C.c_size_t(log_blowup), C.c_size_t(log_final_poly_len),
-                                           C.c_size_t(max_log_arity), _p(b), _p(roots), _p(final), C.byref(nr)),
-                 "fri_commit")
+        if threads > 1:
+            self._rc(self.fn("bb4_fri_commit_mt")(_p(c), C.c_int(rf), C.c_int(rp), _p(cw), C.c_size_t(log_len),
+                                                  C.c_size_t(log_blowup), C.c_size_t(log_final_poly_len),
+                                                  C.c_size_t(max_log_arity), _p(b), C.c_int(threads), _p(roots),
+                                                  _p(final), C.byref(nr)), "fri_commit_mt")
+        else:
+            self._rc(self.fn("bb4_fri_commit")(_p(c), C.c_int(rf), C.c_int(rp), _p(cw), C.c_size_t(log_len),
+                                               C.c_size_t(log_blowup), C.c_size_t(log_final_poly_len),
+                                               C.c_size_t(max_log_arity), _p(b), _p(roots), _p(final), C.byref(nr)),
+                     "fri_commit")
         return roots[: nr.value * 8].reshape(-1, 8), final.reshape(-1, 4)
 
     # Challenger ------------------------------------------------------------

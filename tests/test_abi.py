@@ -41,3 +41,18 @@ def test_host_alloc_rejects_bad_arguments_before_any_device_call():
     assert out.value is None  # cleared on failure
     assert lib.p3h_host_alloc(64, None) == 1
     assert lib.p3h_host_free(None) == 0  # freeing nothing is a no-op
+
+
+def test_seeded_fill_matches_the_oracle_stream():
+    """p3h_seeded_fill is host-only: the benchmark's synthetic inputs equal the
+    oracle's std::mt19937_64 stream element for element."""
+    import native
+    import numpy as np
+
+    for field, name in ((p3hip.BABYBEAR, "bb"), (p3hip.GOLDILOCKS, "gl")):
+        got = p3hip.seeded_fill(field, 0x2A, 5000)
+        assert (got == native.seeded(0x2A, 5000, name)).all()
+    lib = p3hip.load_library()
+    assert lib.p3h_seeded_fill(7, 1, None, 0) == 1  # unknown field: P3H_EINVAL
+    assert lib.p3h_seeded_fill(0, 1, None, 4) == 1  # null output
+    assert np.asarray(p3hip.seeded_fill(p3hip.BABYBEAR, 1, 0)).size == 0

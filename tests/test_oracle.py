@@ -12,7 +12,8 @@ What pins each oracle function (a plausible mistake in it fails at least one):
 |                           | brute-force DFT; idft(dft(x)) == x                                         |
 | coset LDE (BB and GL)     | golden vectors; O(n*N) polynomial evaluation (independent of the FFT)      |
 | flat MMCS commit / open   | golden roots (odd heights exercise the zero-digest padding)               |
-| threaded LDE + commit     | equals the single-threaded LDE and commit (baseline use only)             |
+| threaded LDE + commit     | equals the single-threaded LDE and commit, openings included              |
+| threaded batch / FRI      | equal the single-threaded multi-height MMCS and the golden FRI roots      |
 | multi-height MMCS         | golden caps, openings and proofs (cap 0/1, height ties, injection)         |
 | verify_batch              | accepts golden openings, rejects single-element tampering                 |
 | FRI fold                  | golden folds (arity 2/4/8); arity-2 closed form in pure Python integers    |
@@ -68,6 +69,21 @@ def test_goldilocks_two_adic_structure(orc):
         assert pow(7, (GL_P - 1) // q, GL_P) != 1
     for a, b in ((GL_P - 1, GL_P - 1), (2**63, 2**63 + 5), (12345, 2**64 - 2**32)):
         assert orc.lib.orc_gl_mul(a % GL_P, b % GL_P) == (a % GL_P) * (b % GL_P) % GL_P
+
+
+def test_goldilocks_mul_against_python_integers(orc):
+    """The oracle's reduce128 against exact Python integers: random operands,
+    and operands near 0, 2^32, 2^63 and p whose products hit each wrap of the
+    reduction (a borrow in lo - hh, a carry in t0 + t1, a result in [p, 2^64))."""
+    rng = np.random.default_rng(99)
+    edge = [0, 1, 2, 2**32 - 1, 2**32, 2**32 + 1, 2**63 - 1, 2**63, GL_P - 2, GL_P - 1, 2**64 - 2**33,
+            (GL_P - 1) // 2, 0xFFFFFFFE00000002, 0x00000001FFFFFFFF]
+    vals = [int(v) for v in rng.integers(0, GL_P, size=400, dtype=U64)] + edge
+    for a in edge:
+        for b in vals:
+            assert orc.lib.orc_gl_mul(a, b) == a * b % GL_P, (a, b)
+    for a, b in zip(vals[:200], vals[200:400]):
+        assert orc.lib.orc_gl_mul(a, b) == a * b % GL_P, (a, b)
 
 
 def test_bb4_mul_known_answers(orc, golden):
@@ -176,6 +192,70 @@ def test_threaded_lde_commit_matches_single_thread(orc, golden, f, lh, w, added,
     shift = 31 if f == "bb" else 7
     want = orc.flat_commit(f, consts, RF, rp, orc.lde(f, m, added, shift))
     assert (orc.lde_commit_mt(f, consts, RF, rp, m, added, shift, threads) == want).all()
+
+
+@pytest.mark.parametrize("f,lh,w,added,threads", [("bb", 9, 19, 1, 5), ("gl", 8, 7, 2, 3)])
+def test_threaded_openings_match_single_thread(orc, golden, f, lh, w, added, threads):
+    """The threaded LDE + commit with openings (the source of the full-size
+    fixtures) equals the single-threaded LDE, flat tree and openings."""
+    consts, rp = (golden["consts_bb"], 13) if f == "bb" else (golden["consts_gl"], 22)
+    m = native.seeded(lh * 31 + w, (1 << lh) * w, f).reshape(1 << lh, w)
+    shift = 31 if f == "bb" else 7
+    lde = orc.lde(f, m, added, shift)
+    want_root, layers = orc.flat_commit(f, consts, RF, rp, lde, with_tree=True)
+    idx = [0, 1, 5, (1 << (lh + added)) - 1, 77]
+    root, vals, proofs = orc.lde_commit_open_mt(f, consts, RF, rp, m, added, shift, threads, idx)
+    assert (root == want_root).all()
+    for q, i in enumerate(idx):
+        assert (vals[q] == lde[i]).all()
+        cur = i
+        for s_, sib in enumerate(proofs[q]):
+            assert (sib == layers[s_][cur ^ 1]).all()
+            cur >>= 1
+
+
+def test_threaded_batch_commit_matches_mmcs(orc, golden):
+    """The threaded batched commit equals per-trace LDEs followed by the
+    single-threaded multi-height MMCS and its openings (height ties, two
+    blowups, cap 2)."""
+    shapes = [(6, 5, 1), (8, 3, 2), (6, 9, 2), (9, 4, 1), (4, 11, 1), (7, 2, 2)]
+    traces = [native.seeded(100 + i, (1 << h) * w).reshape(1 << h, w) for i, (h, w, _) in enumerate(shapes)]
+    ks = [k for _, _, k in shapes]
+    ldes = [orc.lde("bb", t, k, 31) for t, k in zip(traces, ks)]
+    idx = [0, 3, 511, 1023, 600]
+    cap, vals, proofs = orc.batch_commit_mt(golden["consts_bb"], RF, 13, traces, ks, 31, 2, 4, idx)
+    for q, i in enumerate(idx):
+        wcap, wv, wp = orc.mmcs("bb", golden["consts_bb"], RF, 13, ldes, cap_height=2, open_index=i)
+        assert (cap == wcap).all()
+        assert (vals[q] == wv).all()
+        assert (proofs[q] == wp).all()
+
+
+def test_threaded_fri_commit_matches_golden(orc, golden):
+    for case in range(3):
+        log_len, lb, lfpl, mla = (int(v) for v in golden[f"fri_{case}_params"])
+        roots, final = orc.fri_commit(golden["consts_bb"], RF, 13, golden[f"fri_{case}_cw"], lb, lfpl, mla,
+                                      golden[f"fri_{case}_betas"], threads=4)
+        assert (roots == golden[f"fri_{case}_roots"]).all(), case
+        assert (final == golden[f"fri_{case}_final"]).all(), case
+
+
+def test_seeded_streams(orc):
+    """The synthetic input stream: the oracle's fill, the pure-Python
+    mt19937_64 of the workload definitions, and the C++ standard's known answer
+    (the 10000th output of a default-constructed mt19937_64,
+    [rand.predef]/6: 9981545732273789042)."""
+    import workloads
+
+    g = workloads.MT19937_64(5489)
+    for _ in range(9999):
+        g()
+    assert g() == 9981545732273789042
+    for f in ("bb", "gl"):
+        want = np.array(workloads.field_elems(424242, 1000, f), dtype=native.DT[f])
+        assert (native.seeded(424242, 1000, f) == want).all()
+    assert (native.p2_consts(1, "bb", RF, 13) == np.array(workloads.p2_consts("bb"), dtype=U32)).all()
+    assert (native.p2_consts(1, "gl", RF, 22) == np.array(workloads.p2_consts("gl"), dtype=U64)).all()
 
 
 def test_flat_open_consistent_with_tree(orc, golden):
