@@ -117,6 +117,13 @@ int p3h_mat_info(const p3h_mat* m, int* field, uint64_t* height, uint32_t* width
 /* download rows [row0, row0 + nrows) in natural order */
 int p3h_mat_download_rows(p3h_ctx* ctx, const p3h_mat* m, uint64_t row0, uint64_t nrows, void* host);
 int p3h_mat_free(p3h_mat* m);
+/* Device address of the matrix's first element, for callers that launch
+ * their own work on it or pass it to the device-pointer entry points.  Valid
+ * while the handle lives.  Rows are in natural order for uploaded and
+ * allocated matrices; LDE outputs are stored coset by coset (stored row
+ * r * n + i is LDE row i * 2^k + r for 2^k cosets of height n). */
+int p3h_mat_device_ptr(const p3h_mat* m, void** out);
+
 /* overwrite a natural-order matrix with new host contents (same shape) */
 int p3h_mat_write(p3h_ctx* ctx, p3h_mat* m, const void* host);
 
@@ -145,6 +152,12 @@ int p3h_flat_commit(p3h_ctx* ctx, const p3h_mat* m, p3h_tree** out);
 /* cap (or root) digests; cap_len receives the number of digests */
 int p3h_tree_cap(p3h_ctx* ctx, const p3h_tree* t, void* cap_out, uint32_t* cap_len);
 int p3h_tree_info(const p3h_tree* t, uint32_t* n_layers, uint64_t* n_leaves, uint32_t* total_width);
+/* The committed matrices of a tree in commit order: their count, and, when
+ * the arrays are not null, each one's height and width (what verify_batch
+ * takes as dimensions, merkle_tree_mmcs.hpp:168-254).  A multi-height tree
+ * lists every matrix; a single-matrix tree lists its one matrix (for a PCS
+ * commit, the zero-padded row concatenation). */
+int p3h_tree_matrices(const p3h_tree* t, uint32_t* n_mats, uint64_t* heights, uint32_t* widths);
 /* copy digest layer `layer` (0 = leaves) to the host */
 int p3h_tree_layer(p3h_ctx* ctx, const p3h_tree* t, uint32_t layer, void* out);
 /* Openings for n_idx row indices at once: open_batch (merkle_tree_mmcs.hpp:120-152)
@@ -179,6 +192,15 @@ int p3h_flat_verify(p3h_ctx* ctx, int field, const void* root, uint32_t n_root, 
  * holding BabyBear^4 values; beta = 4 canonical coefficients; log_arity 1..3. */
 int p3h_fri_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uint32_t* beta,
                  p3h_mat** out);
+/* fold_matrix_cuda_device (fri_fold_cuda.hpp:250-277) on device buffers:
+ * d_in holds n * 2^log_arity BabyBear^4 values (4 canonical words each),
+ * d_out receives n; row i folds d_in[i * arity .. (i + 1) * arity) over the
+ * domain of size 2^log_height, as p3h_fri_fold.  n <= 2^(log_height -
+ * log_arity).  Like the reference it copies nothing to or from the host and
+ * returns when the fold is done; the arity-2 table is cached in the context
+ * (allocated on the first call of each height). */
+int p3h_fri_fold_device(p3h_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, const uint32_t* beta,
+                        uint32_t log_arity, uint32_t log_height, uint64_t n);
 /* prove_fri commit loop (fri_prover.hpp:83-178) for one input codeword of
  * BabyBear^4 (2^log_len x 4 matrix), with the Fiat-Shamir challenges supplied
  * by the caller (betas: 4 words per round).  Each round commits the current

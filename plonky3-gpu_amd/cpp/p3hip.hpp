@@ -7,6 +7,7 @@
 //   HipMerkleTreeMmcs<F, D>   ~ p3_merkle_tree::MerkleTreeMmcs (merkle_tree_mmcs.hpp:75-254)
 //   HipFlatMmcs<F, D>         ~ p3_merkle::MerkleTreeMmcs      (fri_merkle_tree_mmcs.hpp:29-168)
 //   hip_fold_matrix<EF>       ~ fold_matrix_cuda                (fri_fold_cuda.hpp:306-348)
+//   hip_fold_matrix_device    ~ fold_matrix_cuda_device         (fri_fold_cuda.hpp:250-277)
 //   HipDuplexChallenger<F>    ~ p3_challenger::DuplexChallenger (duplex_challenger.hpp:42-268)
 //
 // Matrix types: any type with `values` (std::vector<F>), `width()` and
@@ -176,17 +177,21 @@ struct HipCap {
 class HipTree {
    public:
     HipTree() = default;
-    HipTree(p3h_ctx* ctx, p3h_tree* t, std::vector<size_t> widths = {})
-        : ctx_(ctx), t_(t, [](p3h_tree* p) { p3h_tree_free(p); }), widths_(std::move(widths)) {}
+    HipTree(p3h_ctx* ctx, p3h_tree* t) : ctx_(ctx), t_(t, [](p3h_tree* p) { p3h_tree_free(p); }) {}
     p3h_tree* get() const { return t_.get(); }
     p3h_ctx* ctx() const { return ctx_; }
-    // widths of the committed matrices, in commit order
-    const std::vector<size_t>& widths() const { return widths_; }
+    // widths of the committed matrices in commit order, as the native tree records them
+    std::vector<size_t> widths() const {
+        uint32_t n = 0;
+        check(ctx_, p3h_tree_matrices(t_.get(), &n, nullptr, nullptr));
+        std::vector<uint32_t> w(n);
+        check(ctx_, p3h_tree_matrices(t_.get(), &n, nullptr, w.data()));
+        return std::vector<size_t>(w.begin(), w.end());
+    }
 
    private:
     p3h_ctx* ctx_ = nullptr;
     std::shared_ptr<p3h_tree> t_;
-    std::vector<size_t> widths_;
 };
 
 template <class F, class Raw, size_t DIGEST>
@@ -221,15 +226,13 @@ class HipMerkleTreeMmcs {
         p3h_ctx* c = ctx_.get();
         std::vector<DeviceMat> dev;
         std::vector<const p3h_mat*> ptrs;
-        std::vector<size_t> widths;
         for (auto& m : matrices) {
             dev.push_back(upload<Raw>(c, m.values, m.width()));
             ptrs.push_back(dev.back().get());
-            widths.push_back(m.width());
         }
         p3h_tree* t = nullptr;
         check(c, p3h_mmcs_commit(c, ptrs.data(), uint32_t(ptrs.size()), uint32_t(cap_height_), &t));
-        HipTree tree(c, t, std::move(widths));
+        HipTree tree(c, t);
         return {Commitment{tree_cap<F, Raw, DIGEST>(tree)}, tree};
     }
 
@@ -307,7 +310,10 @@ class HipFlatMmcs {
         return {tree_cap<F, Raw, DIGEST>(pd.tree), pd};
     }
 
-    std::vector<F> get_row(const ProverData& d, size_t row) const {
+    // get_row_as_vals (fri_merkle_tree_mmcs.hpp:115-118): the committed row,
+    // read from the device-resident matrix; ExtensionMmcs::get_row calls it.
+    std::vector<F> get_row_as_vals(const ProverData& d, size_t row) const {
+        if (row >= d.height) throw std::invalid_argument("get_row_as_vals: row out of range");
         std::vector<Raw> vals(d.width), proof;
         uint32_t n_layers = 0;
         check(d.tree.ctx(), p3h_tree_info(d.tree.get(), &n_layers, nullptr, nullptr));
@@ -318,6 +324,8 @@ class HipFlatMmcs {
         for (Raw r : vals) out.push_back(FieldRaw<F, Raw>::from(r));
         return out;
     }
+    // get_row: the reference's alias of get_row_as_vals (fri_merkle_tree_mmcs.hpp:120-124)
+    std::vector<F> get_row(const ProverData& d, size_t row) const { return get_row_as_vals(d, row); }
 
     void open_row(const ProverData& d, size_t row_index, OpeningProof& proof) const {
         uint32_t n_layers = 0;
@@ -379,6 +387,21 @@ std::vector<EF> hip_fold_matrix(const Context& ctx, const EF& beta, size_t log_a
     for (size_t i = 0; i < res.size(); ++i)
         for (int k = 0; k < 4; ++k) res[i].coeffs[k] = vals[4 * i + k];
     return res;
+}
+
+// fold_matrix_cuda_device (fri_fold_cuda.hpp:250-277): fold n rows of
+// 2^log_arity values from device buffer d_input into d_output, both already
+// on the device; nothing is copied to or from the host.  EF must be laid out
+// as 4 canonical uint32 coefficients (BinomialExtensionField<BabyBear, 4>).
+template <class Val, class EF>
+void hip_fold_matrix_device(const Context& ctx, const EF* d_input, EF* d_output, const EF& beta, size_t log_arity,
+                            size_t log_height, size_t n) {
+    static_assert(sizeof(EF) == 4 * sizeof(uint32_t), "EF must be four 32-bit coefficients");
+    uint32_t b[4];
+    for (int i = 0; i < 4; ++i) b[i] = FieldRaw<Val, uint32_t>::to(beta.coeffs[i]);
+    check(ctx.get(), p3h_fri_fold_device(ctx.get(), reinterpret_cast<const uint32_t*>(d_input),
+                                         reinterpret_cast<uint32_t*>(d_output), b, uint32_t(log_arity),
+                                         uint32_t(log_height), uint64_t(n)));
 }
 
 // ------------------------------------------------------------ challenger ---

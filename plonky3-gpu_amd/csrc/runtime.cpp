@@ -1811,6 +1811,12 @@ int p3h_mat_info(const p3h_mat* m, int* field, uint64_t* h, uint32_t* w) {
     return P3H_OK;
 }
 
+int p3h_mat_device_ptr(const p3h_mat* m, void** out) {
+    if (!m || !out) return P3H_EINVAL;
+    *out = m->field == P3H_BABYBEAR ? static_cast<void*>(m->data<uint32_t>()) : static_cast<void*>(m->data<uint64_t>());
+    return P3H_OK;
+}
+
 int p3h_mat_free(p3h_mat* m) {
     if (!m) return P3H_EINVAL;
     mat_unref(m);
@@ -1869,6 +1875,16 @@ int p3h_tree_info(const p3h_tree* t, uint32_t* n_layers, uint64_t* n_leaves, uin
     return P3H_OK;
 }
 
+int p3h_tree_matrices(const p3h_tree* t, uint32_t* n_mats, uint64_t* heights, uint32_t* widths) {
+    if (!t || !n_mats) return P3H_EINVAL;
+    *n_mats = uint32_t(t->widths.size());
+    for (size_t i = 0; i < t->widths.size(); ++i) {
+        if (heights) heights[i] = t->heights[i];
+        if (widths) widths[i] = t->widths[i];
+    }
+    return P3H_OK;
+}
+
 int p3h_tree_layer(p3h_ctx* ctx, const p3h_tree* t, uint32_t layer, void* out) {
     return api(ctx, [&] {
         if (!t || !out || layer >= t->layers.size()) throw Err{P3H_EINVAL, "bad layer"};
@@ -1897,6 +1913,28 @@ int p3h_fri_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uin
     return api(ctx, [&] {
         if (!cur || !beta || !out) throw Err{P3H_EINVAL, "null argument"};
         *out = do_fold(ctx, cur, log_arity, beta);
+    });
+}
+
+int p3h_fri_fold_device(p3h_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, const uint32_t* beta, uint32_t log_arity,
+                        uint32_t log_height, uint64_t n) {
+    return api(ctx, [&] {
+        if (!beta || (n && (!d_in || !d_out))) throw Err{P3H_EINVAL, "null argument"};
+        if (log_arity < 1 || log_arity > 3) throw Err{P3H_EINVAL, "log_arity must be 1, 2, or 3"};
+        if (log_height < log_arity || log_height > 27)
+            throw Err{P3H_EINVAL, "log_height must lie in [log_arity, 27]"};
+        if (n > (uint64_t(1) << (log_height - log_arity)))
+            throw Err{P3H_EINVAL, "n exceeds the folded height 2^(log_height - log_arity)"};
+        for (int q = 0; q < 4; ++q)
+            if (beta[q] >= BB_P) throw Err{P3H_EINVAL, "beta is not canonical"};
+        if (n == 0) return;
+        const uint32_t* tinv = log_arity == 1 ? fri_tinv(ctx, log_height) : nullptr;
+        Phase ph(ctx, "fold");
+        launch_check(fold_device(d_in, d_out, n, log_height, log_arity, beta, nullptr, tinv,
+                                 bb_mont(bb_root(log_height)), bb_mont(bb_root(log_arity)),
+                                 bb_mont(bb_pow(bb_inv(2), log_arity)), ctx->stream),
+                     "fold");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
 }
 

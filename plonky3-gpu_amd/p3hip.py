@@ -73,12 +73,15 @@ _SIGS = {
     "p3h_mat_download_rows": (_i32, [_vp, _vp, _u64, _u64, _vp]),
     "p3h_mat_info": (_i32, [_vp, C.POINTER(_i32), C.POINTER(_u64), C.POINTER(_u32)]),
     "p3h_mat_free": (_i32, [_vp]),
+    "p3h_mat_device_ptr": (_i32, [_vp, C.POINTER(_vp)]),
+    "p3h_fri_fold_device": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _u64]),
     "p3h_dft": (_i32, [_vp, _i32, _vp, _u64, C.POINTER(_vp)]),
     "p3h_coset_lde": (_i32, [_vp, _vp, _u32, _u64, C.POINTER(_vp)]),
     "p3h_mmcs_commit": (_i32, [_vp, C.POINTER(_vp), _u32, _u32, C.POINTER(_vp)]),
     "p3h_flat_commit": (_i32, [_vp, _vp, C.POINTER(_vp)]),
     "p3h_tree_cap": (_i32, [_vp, _vp, _vp, C.POINTER(_u32)]),
     "p3h_tree_info": (_i32, [_vp, C.POINTER(_u32), C.POINTER(_u64), C.POINTER(_u32)]),
+    "p3h_tree_matrices": (_i32, [_vp, C.POINTER(_u32), _vp, _vp]),
     "p3h_tree_layer": (_i32, [_vp, _vp, _u32, _vp]),
     "p3h_tree_open": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp]),
     "p3h_tree_free": (_i32, [_vp]),
@@ -420,6 +423,12 @@ class Context:
         self._check(self.lib.p3h_fri_fold(self.h, cur.h, log_arity, _ptr(b), C.byref(h)))
         return Mat(self, h)
 
+    def fri_fold_device(self, d_in: int, d_out: int, beta, log_arity: int, log_height: int, n: int):
+        """fold_matrix_cuda_device: fold n rows from device address d_in into
+        device address d_out (BabyBear^4, 4 words per value)."""
+        b = np.ascontiguousarray(beta, dtype=np.uint32).reshape(4)
+        self._check(self.lib.p3h_fri_fold_device(self.h, d_in, d_out, _ptr(b), log_arity, log_height, n))
+
     def fri_commit_phase(self, codeword: "Mat", log_blowup: int, log_final_poly_len: int, max_log_arity: int,
                          betas: np.ndarray, keep: bool = False):
         """Returns (roots [rounds x 8], final_poly [2^lfpl x 4], Fri or None)."""
@@ -483,6 +492,12 @@ class Mat:
             raise ValueError("shape mismatch")
         self.ctx._check(self.ctx.lib.p3h_mat_write(self.ctx.h, self.h, _ptr(a)))
 
+    def device_ptr(self) -> int:
+        """Device address of the first element (p3h_mat_device_ptr)."""
+        p = _vp()
+        self.ctx._check(self.ctx.lib.p3h_mat_device_ptr(self.h, C.byref(p)))
+        return p.value
+
     def download_rows(self, row0: int, nrows: int) -> np.ndarray:
         out = np.empty((nrows, self.width), dtype=DTYPE[self.field])
         self.ctx._check(self.ctx.lib.p3h_mat_download_rows(self.ctx.h, self.h, row0, nrows, _ptr(out)))
@@ -519,6 +534,15 @@ class Tree:
 
     def root(self) -> np.ndarray:
         return self.cap()[0]
+
+    def dims(self):
+        """(height, width) of each committed matrix in commit order (p3h_tree_matrices)."""
+        n = _u32()
+        self.ctx._check(self.ctx.lib.p3h_tree_matrices(self.h, C.byref(n), None, None))
+        hs = np.empty(n.value, dtype=np.uint64)
+        ws = np.empty(n.value, dtype=np.uint32)
+        self.ctx._check(self.ctx.lib.p3h_tree_matrices(self.h, C.byref(n), _ptr(hs), _ptr(ws)))
+        return [(int(h), int(w)) for h, w in zip(hs, ws)]
 
     def layer(self, i: int) -> np.ndarray:
         cnt = self.n_leaves
@@ -805,7 +829,6 @@ class MerkleTreeMmcsHip:
     def commit(self, matrices):
         mats = [self.ctx.upload(m) for m in matrices]
         tree = self.ctx.mmcs_commit(mats, self.cap_height)
-        tree.widths = [m.width for m in mats]
         for m in mats:
             m.free()  # the tree keeps its own references
         return tree.cap(), tree
@@ -813,7 +836,7 @@ class MerkleTreeMmcsHip:
     def open_batch(self, index: int, tree: Tree):
         """(opened_values: one row per matrix in commit order, proof digests)."""
         vals, proof = tree.open([index])
-        cuts = np.cumsum(tree.widths)[:-1]
+        cuts = np.cumsum([w for _, w in tree.dims()])[:-1]
         return list(np.split(vals[0], cuts)), proof[0]
 
     def verify_batch(self, cap, dims, index: int, opened_values, proof) -> bool:

@@ -380,6 +380,52 @@ def test_fold_golden(ctx, golden):
         assert (out == golden[f"fold_a{la}_out"]).all(), la
 
 
+def test_fold_device_pointers(ctx, golden, orc):
+    """p3h_fri_fold_device (fold_matrix_cuda_device) on device buffers: the
+    golden folds at every arity, a partial fold (n below the folded height:
+    rows 0..n-1 only, the rest of the output untouched), and a 2^16 fold
+    against the oracle; argument errors raise."""
+    for la in (1, 2, 3):
+        cw = golden[f"fold_a{la}_in"]
+        src = ctx.upload(cw)
+        lh = cw.shape[0].bit_length() - 1
+        want = golden[f"fold_a{la}_out"]
+        dst = ctx.upload(np.full_like(want, 7))
+        ctx.fri_fold_device(src.device_ptr(), dst.device_ptr(), golden[f"fold_a{la}_beta"], la, lh, want.shape[0])
+        assert (dst.download() == want).all(), la
+        part = max(1, want.shape[0] // 2 - 1)
+        dst2 = ctx.upload(np.full_like(want, 7))
+        ctx.fri_fold_device(src.device_ptr(), dst2.device_ptr(), golden[f"fold_a{la}_beta"], la, lh, part)
+        got = dst2.download()
+        assert (got[:part] == want[:part]).all() and (got[part:] == 7).all(), la
+    cw = rnd(21, (1 << 16, 4))
+    beta = rnd(22, (4,))
+    src, dst = ctx.upload(cw), ctx.alloc(p3hip.BABYBEAR, 1 << 15, 4)
+    ctx.fri_fold_device(src.device_ptr(), dst.device_ptr(), beta, 1, 16, 1 << 15)
+    assert (dst.download() == orc.fold(16, 1, beta, cw)).all()
+    with pytest.raises(ValueError):
+        ctx.fri_fold_device(src.device_ptr(), dst.device_ptr(), beta, 4, 16, 1 << 12)
+    with pytest.raises(ValueError):
+        ctx.fri_fold_device(src.device_ptr(), dst.device_ptr(), beta, 1, 16, (1 << 15) + 1)
+    with pytest.raises(ValueError):
+        ctx.fri_fold_device(src.device_ptr(), dst.device_ptr(), [BB_P, 0, 0, 0], 1, 16, 8)
+
+
+def test_tree_matrices_and_open_batch(ctx, golden, orc):
+    """p3h_tree_matrices reports the committed dimensions in commit order for
+    trees from any commit call, and open_batch splits the opened row by them."""
+    mats = [rnd(40 + i, (h, w)) for i, (h, w) in enumerate([(64, 5), (256, 3), (64, 9), (128, 2)])]
+    tree = ctx.mmcs_commit([ctx.upload(m) for m in mats], 1)
+    assert tree.dims() == [(m.shape[0], m.shape[1]) for m in mats]
+    mm = p3hip.MerkleTreeMmcsHip(ctx, 1)
+    vals, proof = mm.open_batch(200, tree)
+    for m, v in zip(mats, vals):
+        assert (v == m[200 // (256 // m.shape[0])]).all()
+    assert mm.verify_batch(tree.cap(), tree.dims(), 200, vals, proof)
+    flat = ctx.flat_commit(ctx.upload(mats[1]))
+    assert flat.dims() == [(256, 3)]
+
+
 def test_fri_commit_golden(ctx, golden):
     for case in range(3):
         log_len, lb, lfpl, mla = (int(v) for v in golden[f"fri_{case}_params"])
