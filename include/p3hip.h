@@ -201,6 +201,16 @@ int p3h_fri_fold(p3h_ctx* ctx, const p3h_mat* cur, uint32_t log_arity, const uin
  * (allocated on the first call of each height). */
 int p3h_fri_fold_device(p3h_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, const uint32_t* beta,
                         uint32_t log_arity, uint32_t log_height, uint64_t n);
+/* No counterpart in the reference: p3h_fri_fold_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result; it
+ * calls no hipMalloc/hipFree, keeps the arity-2 table in a static 16 MiB
+ * device array per GPU (set up at p3h_ctx_create), and returns once the work
+ * is enqueued on the context's stream (p3h_ctx_sync waits for it).  Limit: at
+ * arity 2, n <= 2^22 (P3H_EINVAL beyond).  Calls that overlap on one GPU
+ * share that array and race. */
+int p3h_fri_fold_device_noalloc(p3h_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, const uint32_t* beta,
+                                uint32_t log_arity, uint32_t log_height, uint64_t n);
 /* prove_fri commit loop (fri_prover.hpp:83-178) for one input codeword of
  * BabyBear^4 (2^log_len x 4 matrix), with the Fiat-Shamir challenges supplied
  * by the caller (betas: 4 words per round).  Each round commits the current

@@ -12,6 +12,8 @@
 #include "fri.h"
 #include "ext4.h"
 
+#include <mutex>
+
 namespace p3h {
 
 namespace {
@@ -88,7 +90,61 @@ __global__ void fold_general_kernel(const uint32_t* in, uint32_t* out, uint64_t 
     for (int q = 0; q < 4; ++q) out[4 * i + q] = res.c[q];
 }
 
+// Arity-2 table of the allocation-free fold: tinv_m[i] =
+// omega_{2^(bits+1)}^-bitrev(i, bits) / 2 in Montgomery form (base_m =
+// omega^-1 and half_m = 1/2, both Montgomery), the entries fri_tinv caches.
+__global__ void fold_tinv_kernel(uint32_t* tinv_m, uint64_t n, uint32_t bits, uint32_t base_m, uint32_t half_m) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tinv_m[i] = BB::mont(bb_pow_m(base_m, dbitrev(uint32_t(i), bits)), half_m);
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[MAX_DEVICES] = {};
+
 }  // namespace
+
+// Scratch of the allocation-free fold: a file-scope device array, one copy per
+// GPU, held until the process ends.  Its name is a host-side stand-in in host
+// code; the device address comes from fold_scratch_setup().
+constexpr uint64_t FOLD_SCRATCH_WORDS = uint64_t(1) << 22;  // 16 MiB
+static __device__ __align__(256) uint32_t scratch[FOLD_SCRATCH_WORDS];
+
+uint64_t fold_scratch_words() { return FOLD_SCRATCH_WORDS; }
+
+int fold_scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return int(hipErrorInvalidDevice);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return int(hipSuccess);
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return int(e);
+    g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    return int(hipSuccess);
+}
+
+uint32_t* fold_scratch_ptr(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+int fold_noalloc_device(const uint32_t* in, uint32_t* out, uint64_t n_out, uint32_t log_h, uint32_t log_arity,
+                        const uint32_t* beta, uint32_t* scratch_ptr, uint32_t tinv_base_m, uint32_t half_m,
+                        uint32_t omega_h_m, uint32_t w_m, uint32_t arity_inv_m, hipStream_t st) {
+    if (n_out == 0) return 0;
+    if (log_arity == 1) {
+        if (!scratch_ptr || n_out > FOLD_SCRATCH_WORDS || log_h < 1) return int(hipErrorInvalidValue);
+        uint32_t blocks = uint32_t((n_out + 255) / 256);
+        hipLaunchKernelGGL(fold_tinv_kernel, dim3(blocks), dim3(256), 0, st, scratch_ptr, n_out, log_h - 1,
+                           tinv_base_m, half_m);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return int(e);
+    }
+    return fold_device(in, out, n_out, log_h, log_arity, beta, nullptr, log_arity == 1 ? scratch_ptr : nullptr,
+                       omega_h_m, w_m, arity_inv_m, st);
+}
 
 int fold_device(const uint32_t* in, uint32_t* out, uint64_t n_out, uint32_t log_h, uint32_t log_arity,
                 const uint32_t* beta, const uint32_t* beta_dev, const uint32_t* tinv_m, uint32_t omega_h_m,

@@ -1541,6 +1541,7 @@ int p3h_ctx_create(int device, uint64_t budget, p3h_ctx** out) {
         if (device < 0 || device >= n) throw Err{P3H_EINVAL, "no such device"};
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipError_t(ntt_prepare_device()), "NTT kernel LDS limits");
+        hip_check(hipError_t(fold_scratch_setup(device)), "hipGetSymbolAddress(scratch)");
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         size_t fr = 0, tot = 0;
         hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
@@ -1935,6 +1936,30 @@ int p3h_fri_fold_device(p3h_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, con
                                  bb_mont(bb_pow(bb_inv(2), log_arity)), ctx->stream),
                      "fold");
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int p3h_fri_fold_device_noalloc(p3h_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, const uint32_t* beta,
+                                uint32_t log_arity, uint32_t log_height, uint64_t n) {
+    return api(ctx, [&] {
+        if (!ctx || !beta || (n && (!d_in || !d_out))) throw Err{P3H_EINVAL, "null argument"};
+        if (log_arity < 1 || log_arity > 3) throw Err{P3H_EINVAL, "log_arity must be 1, 2, or 3"};
+        if (log_height < log_arity || log_height > 27)
+            throw Err{P3H_EINVAL, "log_height must lie in [log_arity, 27]"};
+        if (n > (uint64_t(1) << (log_height - log_arity)))
+            throw Err{P3H_EINVAL, "n exceeds the folded height 2^(log_height - log_arity)"};
+        if (log_arity == 1 && n > fold_scratch_words())
+            throw Err{P3H_EINVAL, "arity-2 fold of more than 2^22 rows: beyond the allocation-free scratch"};
+        for (int q = 0; q < 4; ++q)
+            if (beta[q] >= BB_P) throw Err{P3H_EINVAL, "beta is not canonical"};
+        if (n == 0) return;
+        uint32_t* sp = fold_scratch_ptr(ctx->device);
+        if (!sp) throw Err{P3H_ESTATE, "scratch address not set up for this device"};
+        launch_check(fold_noalloc_device(d_in, d_out, n, log_height, log_arity, beta, sp,
+                                         bb_mont(bb_inv(bb_root(log_height))), bb_mont(bb_inv(2)),
+                                         bb_mont(bb_root(log_height)), bb_mont(bb_root(log_arity)),
+                                         bb_mont(bb_pow(bb_inv(2), log_arity)), ctx->stream),
+                     "fold (no allocation)");
     });
 }
 

@@ -75,6 +75,7 @@ _SIGS = {
     "p3h_mat_free": (_i32, [_vp]),
     "p3h_mat_device_ptr": (_i32, [_vp, C.POINTER(_vp)]),
     "p3h_fri_fold_device": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _u64]),
+    "p3h_fri_fold_device_noalloc": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _u64]),
     "p3h_dft": (_i32, [_vp, _i32, _vp, _u64, C.POINTER(_vp)]),
     "p3h_coset_lde": (_i32, [_vp, _vp, _u32, _u64, C.POINTER(_vp)]),
     "p3h_mmcs_commit": (_i32, [_vp, C.POINTER(_vp), _u32, _u32, C.POINTER(_vp)]),
