@@ -524,10 +524,13 @@ def bench_grind(p3hip, args, bits: int = 24, cpu_bits: int = 13):
         ch = ctx.challenger()
         ch.observe(prefix)
         ch.observe([rep])
+        chk = ch.clone()
         t0 = time.perf_counter()
         w = ch.grind(bits)
         dt = time.perf_counter() - t0
+        assert chk.check_witness(bits, w), "grind returned a witness that does not pass"
         ch.free()
+        chk.free()
         if rep >= args.warmup:
             runs.append((w, dt))
     tries = sum(w + 1 for w, _ in runs)

@@ -79,6 +79,15 @@ const char* p3h_last_error(const p3h_ctx* ctx);
 int p3h_ctx_sync(p3h_ctx* ctx);
 /* bytes held by live handles, the budget, and bytes parked in the allocator cache */
 int p3h_ctx_memory(const p3h_ctx* ctx, uint64_t* in_use, uint64_t* budget, uint64_t* cached);
+/* The part of in_use held by the context's table caches (twiddles, coset
+ * twists, row scales, fold tables), which live until p3h_ctx_destroy; with
+ * no handle alive, in_use equals it. */
+int p3h_ctx_table_bytes(const p3h_ctx* ctx, uint64_t* bytes);
+/* Fault injection: when the environment variable P3H_FAIL_ALLOC_AFTER=N is
+ * set at p3h_ctx_create, every device allocation of that context after its
+ * first N fails with P3H_ENOMEM, as a failed hipMalloc would; tests use it to
+ * check that every failure path returns a clean status and frees what it
+ * allocated. */
 /* release cached device blocks back to HIP */
 int p3h_ctx_trim(p3h_ctx* ctx);
 /* per-phase HIP-event timing on the context stream (off by default) */
@@ -345,6 +354,10 @@ int p3h_serve_release(p3h_serve* sv, uint64_t request_id);
 int p3h_serve_stats(p3h_serve* sv, uint64_t* completed, uint64_t* batches, uint64_t* reserved_bytes,
                     uint64_t* committed_elems);
 const char* p3h_serve_last_error(p3h_serve* sv);
+/* Requests still queued fail with P3H_ESTATE (their waiters wake), batches
+ * being processed complete, calls in progress on other threads return; then
+ * every resident result is freed.  No call on the server may start once
+ * destroy has begun (submissions racing with it get P3H_ESTATE). */
 int p3h_serve_destroy(p3h_serve* sv);
 
 /* ------------------------------------------------------------ autotuner */

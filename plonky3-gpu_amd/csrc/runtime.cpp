@@ -18,6 +18,7 @@
 #include <thread>
 #include <unordered_map>
 #include <cstring>
+#include <cstdlib>
 #include <functional>
 #include <map>
 #include <numeric>
@@ -29,6 +30,7 @@
 #include <vector>
 
 #include "fri.h"
+#include "host_plan.h"
 #include "challenger.h"
 #include "pcs.h"
 #include "merkle.h"
@@ -169,6 +171,10 @@ struct p3h_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint64_t budget = 0, in_use = 0, cached = 0;
+    uint64_t table_bytes = 0;  // part of in_use held by the table caches below
+    // allocation-failure injection (P3H_FAIL_ALLOC_AFTER): -1 = off
+    int64_t fail_after = -1;
+    uint64_t n_allocs = 0;
     std::multimap<size_t, void*> free_blocks;
     std::string err;
     bool p2_set[2] = {false, false};
@@ -206,17 +212,6 @@ DevBlock::~DevBlock() {
 
 namespace {
 
-size_t size_class(size_t bytes) {
-    if (bytes == 0) bytes = 1;
-    if (bytes <= (size_t(2) << 20)) {
-        size_t c = 256;
-        while (c < bytes) c <<= 1;
-        return c;
-    }
-    size_t g = size_t(2) << 20;
-    return (bytes + g - 1) / g * g;
-}
-
 void trim(p3h_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
@@ -226,6 +221,9 @@ void trim(p3h_ctx* ctx) {
 
 DevPtr dalloc(p3h_ctx* ctx, size_t bytes) {
     size_t cls = size_class(bytes);
+    if (ctx->fail_after >= 0 && ctx->n_allocs >= uint64_t(ctx->fail_after))
+        throw Err{P3H_ENOMEM, "injected allocation failure (P3H_FAIL_ALLOC_AFTER=" +
+                                  std::to_string(ctx->fail_after) + ")"};
     if (ctx->in_use + cls > ctx->budget)
         throw Err{P3H_EBUDGET, "request of " + std::to_string(cls) + " bytes exceeds the HBM budget (" +
                                    std::to_string(ctx->in_use) + " of " + std::to_string(ctx->budget) + " in use)"};
@@ -250,6 +248,7 @@ DevPtr dalloc(p3h_ctx* ctx, size_t bytes) {
     blk->ctx = ctx;
     blk->cls = cls;
     ctx->in_use += cls;
+    ++ctx->n_allocs;
     return blk;
 }
 
@@ -487,6 +486,7 @@ const typename H::T* twiddles(p3h_ctx* ctx, uint32_t L) {
                      "twiddles");
     }
     ctx->twiddles[key] = buf;
+    ctx->table_bytes += buf->cls;
     return dptr<T>(buf);
 }
 
@@ -511,6 +511,7 @@ const typename H::T* lde_twist(p3h_ctx* ctx, uint32_t L, uint32_t k, typename H:
                  "lde twist");
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
     ctx->twists[key] = buf;
+    ctx->table_bytes += buf->cls;
     return dptr<T>(buf);
 }
 
@@ -535,6 +536,7 @@ const typename H::T* row_scale(p3h_ctx* ctx, uint32_t L, int kind, typename H::T
     launch_check(powers_device<typename H::F>(dptr<T>(buf), n, H::c(base), H::c(mult), H::c(1), ctx->stream),
                  "row scale");
     ctx->row_scales[key] = buf;
+    ctx->table_bytes += buf->cls;
     return dptr<T>(buf);
 }
 
@@ -1027,6 +1029,7 @@ const uint32_t* fri_tinv(p3h_ctx* ctx, uint32_t LH) {
                  "fri tinv");
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
     ctx->fri_tinv[LH] = buf;
+    ctx->table_bytes += buf->cls;
     return dptr<uint32_t>(buf);
 }
 
@@ -1546,6 +1549,11 @@ int p3h_ctx_create(int device, uint64_t budget, p3h_ctx** out) {
         size_t fr = 0, tot = 0;
         hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
         ctx->budget = budget ? budget : tot;
+        if (const char* f = std::getenv("P3H_FAIL_ALLOC_AFTER")) {
+            char* end = nullptr;
+            const long long v = std::strtoll(f, &end, 10);
+            if (end != f && *end == '\0' && v >= 0) ctx->fail_after = v;
+        }
     });
     if (rc != P3H_OK) {
         delete ctx;
@@ -1615,6 +1623,12 @@ int p3h_seeded_fill(int field, uint64_t seed, void* out, uint64_t n) {
     } else {
         return P3H_EINVAL;
     }
+    return P3H_OK;
+}
+
+int p3h_ctx_table_bytes(const p3h_ctx* ctx, uint64_t* bytes) {
+    if (!ctx || !bytes) return P3H_EINVAL;
+    *bytes = ctx->table_bytes;
     return P3H_OK;
 }
 
@@ -2578,23 +2592,12 @@ struct p3h_serve {
     std::unordered_map<uint64_t, std::unique_ptr<ServeReq>> reqs;
     uint64_t next_id = 1;
     bool stop = false;
+    int callers = 0;  // wait/poll/open/release calls in progress (destroy waits for them)
     uint64_t completed = 0, batches = 0, committed_elems = 0;
     std::string err;
 };
 
 namespace {
-
-size_t cls_bytes(size_t b) { return size_class(b); }
-
-uint64_t serve_footprint(uint64_t h, uint32_t w, uint32_t k) {
-    uint64_t N = h << k;
-    uint64_t tree = 0;
-    for (uint64_t c = N; c >= 1; c >>= 1) {
-        tree += cls_bytes(c * 32);
-        if (c == 1) break;
-    }
-    return cls_bytes(h * w * 4) + cls_bytes(N * w * 4) + tree + cls_bytes(N * 4) + 4096;
-}
 
 void serve_process(p3h_serve* sv, int slot, std::vector<ServeReq*>& batch) {
     p3h_ctx* ctx = sv->slots[slot];
@@ -2668,17 +2671,35 @@ void serve_process(p3h_serve* sv, int slot, std::vector<ServeReq*>& batch) {
     sv->cv_work.notify_all();
 }
 
+// A wait/poll/open/release call in progress: p3h_serve_destroy waits until
+// none is left before it frees the server.
+struct ServeCall {
+    p3h_serve* sv;
+    explicit ServeCall(p3h_serve* s) : sv(s) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        ++sv->callers;
+    }
+    ~ServeCall() {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        --sv->callers;
+        sv->cv_done.notify_all();
+    }
+};
+
 void serve_worker(p3h_serve* sv, int slot) {
     (void)hipSetDevice(sv->device);
     std::unique_lock<std::mutex> lk(sv->mu);
+    std::vector<uint64_t> fp;
     while (true) {
         sv->cv_work.wait(lk, [&] {
-            return sv->stop || (!sv->queue.empty() && sv->reserved + sv->queue.front()->reserve <= sv->budget);
+            return sv->stop || (!sv->queue.empty() && sv->queue.front()->reserve <= sv->budget - sv->reserved);
         });
         if (sv->stop) return;
+        fp.clear();
+        for (size_t i = 0; i < sv->queue.size() && i < sv->max_batch; ++i) fp.push_back(sv->queue[i]->reserve);
+        const size_t take = serve_admit(sv->reserved, sv->budget, sv->max_batch, fp.data(), fp.size());
         std::vector<ServeReq*> batch;
-        while (!sv->queue.empty() && batch.size() < sv->max_batch &&
-               sv->reserved + sv->queue.front()->reserve <= sv->budget) {
+        for (size_t i = 0; i < take; ++i) {
             ServeReq* r = sv->queue.front();
             sv->queue.pop_front();
             sv->reserved += r->reserve;
@@ -2733,6 +2754,10 @@ int p3h_serve_submit(p3h_serve* sv, const uint32_t* trace, uint64_t height, uint
     r->shift = uint32_t(shift);
     r->reserve = serve_footprint(height, width, added_bits);
     std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) {
+        sv->err = "the server is being destroyed";
+        return P3H_ESTATE;
+    }
     if (r->reserve > sv->budget) {
         sv->err = "request footprint exceeds the server's HBM budget";
         return P3H_EBUDGET;
@@ -2748,6 +2773,7 @@ int p3h_serve_submit(p3h_serve* sv, const uint32_t* trace, uint64_t height, uint
 
 int p3h_serve_wait(p3h_serve* sv, uint64_t id, uint32_t* root_out, double* latency_ms) {
     if (!sv) return P3H_EINVAL;
+    const ServeCall call(sv);
     std::unique_lock<std::mutex> lk(sv->mu);
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return P3H_EINVAL;
@@ -2764,6 +2790,7 @@ int p3h_serve_wait(p3h_serve* sv, uint64_t id, uint32_t* root_out, double* laten
 
 int p3h_serve_poll(p3h_serve* sv, uint64_t id, int* done) {
     if (!sv || !done) return P3H_EINVAL;
+    const ServeCall call(sv);
     std::lock_guard<std::mutex> lk(sv->mu);
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return P3H_EINVAL;
@@ -2774,6 +2801,7 @@ int p3h_serve_poll(p3h_serve* sv, uint64_t id, int* done) {
 int p3h_serve_open(p3h_serve* sv, uint64_t id, const uint64_t* indices, uint32_t n, uint32_t* values,
                    uint32_t* proof) {
     if (!sv) return P3H_EINVAL;
+    const ServeCall call(sv);
     ServeReq* r = nullptr;
     {
         std::unique_lock<std::mutex> lk(sv->mu);
@@ -2794,6 +2822,7 @@ int p3h_serve_open(p3h_serve* sv, uint64_t id, const uint64_t* indices, uint32_t
 
 int p3h_serve_release(p3h_serve* sv, uint64_t id) {
     if (!sv) return P3H_EINVAL;
+    const ServeCall call(sv);
     std::unique_ptr<ServeReq> r;
     {
         std::unique_lock<std::mutex> lk(sv->mu);
@@ -2832,11 +2861,25 @@ const char* p3h_serve_last_error(p3h_serve* sv) { return sv ? sv->err.c_str() : 
 int p3h_serve_destroy(p3h_serve* sv) {
     if (!sv) return P3H_EINVAL;
     {
+        // requests still queued fail; their waiters wake with P3H_ESTATE
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->stop = true;
+        for (ServeReq* r : sv->queue) {
+            r->done = true;
+            r->status = P3H_ESTATE;
+            r->err = "the server was destroyed before the request ran";
+            sv->reserved -= r->reserve;
+            r->reserve = 0;
+        }
+        sv->queue.clear();
     }
     sv->cv_work.notify_all();
-    for (auto& t : sv->workers) t.join();
+    sv->cv_done.notify_all();
+    for (auto& t : sv->workers) t.join();  // batches in flight complete and publish
+    {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        sv->cv_done.wait(lk, [&] { return sv->callers == 0; });
+    }
     for (auto& kv : sv->reqs)
         if (kv.second->tree) p3h_tree_free(kv.second->tree);
     sv->reqs.clear();

@@ -58,6 +58,7 @@ _SIGS = {
     "p3h_ctx_sync": (_i32, [_vp]),
     "p3h_ctx_memory": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "p3h_ctx_trim": (_i32, [_vp]),
+    "p3h_ctx_table_bytes": (_i32, [_vp, C.POINTER(_u64)]),
     "p3h_ctx_set_profiling": (_i32, [_vp, _i32]),
     "p3h_phase_stats": (_i32, [_vp, C.c_char_p, C.POINTER(_u64), C.POINTER(C.c_double)]),
     "p3h_ctx_set_ntt_tuning": (_i32, [_vp, _u32, _u32, _u32]),
@@ -248,6 +249,12 @@ class Context:
         a, b, c = _u64(), _u64(), _u64()
         self._check(self.lib.p3h_ctx_memory(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"in_use": a.value, "budget": b.value, "cached": c.value}
+
+    def table_bytes(self) -> int:
+        """Bytes of in_use held by the context's table caches."""
+        v = _u64()
+        self._check(self.lib.p3h_ctx_table_bytes(self.h, C.byref(v)))
+        return v.value
 
     def trim(self):
         self._check(self.lib.p3h_ctx_trim(self.h))
@@ -765,9 +772,12 @@ class Server:
         return {"completed": a.value, "batches": b.value, "reserved_bytes": c.value, "committed_elems": d.value}
 
     def close(self):
-        if getattr(self, "h", None):
-            self.lib.p3h_serve_destroy(self.h)  # joins the workers first
+        """Destroy the server: queued requests fail, running batches finish,
+        calls in progress on other threads return first."""
+        h = getattr(self, "h", None)
+        if h:
             self.h = None
+            self.lib.p3h_serve_destroy(h)
             self._held.clear()
 
     def __enter__(self):

@@ -124,3 +124,29 @@ def test_challenger_errors(ctx):
     ch.observe(np.zeros(0, dtype=np.uint32))  # empty slice: allowed, no effect
     ch2 = ctx.challenger()
     assert (ch.sample(3) == ch2.sample(3)).all()
+
+
+def test_grind_least_witness_beyond_the_first_wave():
+    """19-bit grinds whose least witness lies past the first 2^20 witnesses
+    (the first wave of 4096 chunks of 256): the minimum is found after later
+    chunks are claimed and the remaining workgroups drain on the early exit.
+    The witnesses are the oracle's (tests/test_oracle.py LATE_GRINDS)."""
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "plonky3-gpu_amd"))
+    import workloads
+    from test_oracle import LATE_GRINDS
+
+    c = p3hip.Context(0)
+    c.set_poseidon2(p3hip.BABYBEAR, np.array(workloads.p2_consts("bb"), dtype=np.uint32), RF, 13)
+    for seed, want in LATE_GRINDS.items():
+        prefix = native.seeded(seed, 11)
+        ch = c.challenger()
+        ch.observe(prefix)
+        chk = ch.clone()
+        assert ch.grind(19) == want, seed
+        assert chk.check_witness(19, want)
+        ch.free()
+        chk.free()
+    c.close()

@@ -84,3 +84,28 @@ def test_pinned_host_buffers(ctx, golden, orc):
     tail = pt[-4:]
     del pt
     assert (tail == t[-4:]).all()
+
+
+def test_destroy_with_requests_queued_and_a_waiter(golden):
+    """Destroying the server while requests are still queued and another thread
+    waits on the last one: the waiter returns -- with the root, or with
+    P3H_ESTATE when its request never ran -- and nothing hangs."""
+    import ctypes
+    import threading
+
+    reqs = _requests(13, 24, 11, 12)
+    sv = p3hip.Server(golden["consts_bb"], 8, 13, n_slots=1, max_batch=1)
+    ids = [sv.submit(t, 1, 31) for t in reqs]
+    h, out = sv.h, {}
+    root = np.zeros(8, dtype=np.uint32)
+
+    def waiter():
+        out["rc"] = sv.lib.p3h_serve_wait(h, ids[-1], root.ctypes.data_as(ctypes.c_void_p), None)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    sv.wait(ids[0])
+    sv.close()
+    th.join(timeout=120)
+    assert not th.is_alive(), "a waiter hung across destroy"
+    assert out["rc"] in (0, 5), out  # P3H_OK or P3H_ESTATE

@@ -469,3 +469,20 @@ def test_random_cross_check_against_reference(orc, ref):
             else:
                 ops.append((native.GRIND, int(rng.integers(0, 8))))
         assert (orc.challenger_run(cb, RF, 13, ops) == ref.challenger_run(cb, RF, 13, ops)).all()
+
+
+# Transcripts whose least 19-bit proof-of-work witness lies past the first
+# 2^20 witnesses: the prefix is native.seeded(seed, 11) under the workload
+# constants.  The GPU grind test compares with these values.
+LATE_GRINDS = {7004: 1971144, 7007: 1487619, 7009: 1161396}
+
+
+def test_late_grind_witnesses():
+    import workloads
+
+    orc = native.oracle()
+    cb = np.array(workloads.p2_consts("bb"), dtype=U32)
+    for seed, want in LATE_GRINDS.items():
+        prefix = [int(v) for v in native.seeded(seed, 11)]
+        ops = [(native.OBSERVE, v) for v in prefix] + [(native.GRIND, 19)]
+        assert int(orc.challenger_run(cb, RF, 13, ops)[0]) == want, seed
