@@ -43,6 +43,28 @@ struct BabyBear {
     __device__ __forceinline__ static uint32_t mul_c(uint32_t x, uint32_t c_mont) { return mont(x, c_mont); }
     __device__ __forceinline__ static uint32_t to_mont(uint32_t x) { return mont(x, R2_MOD_P); }
     __device__ __forceinline__ static uint32_t from_mont(uint32_t x) { return mont(x, 1u); }
+
+    // NTT butterflies (ntt.hip) on canonical words; w in multiply-constant form.
+    // dit: (a, b) -> (a + w b, a - w b); dif: (a, b) -> (a + b, (a - b) w);
+    // the *1 forms are the same with w = 1.
+    __device__ __forceinline__ static void dit(uint32_t& a, uint32_t& b, uint32_t w) {
+        const uint32_t t = mul_c(b, w), x = a;
+        a = add(x, t);
+        b = sub(x, t);
+    }
+    __device__ __forceinline__ static void dif(uint32_t& a, uint32_t& b, uint32_t w) {
+        const uint32_t x = a, y = b;
+        a = add(x, y);
+        b = mul_c(sub(x, y), w);
+    }
+    __device__ __forceinline__ static void bf1(uint32_t& a, uint32_t& b) {
+        const uint32_t x = a, y = b;
+        a = add(x, y);
+        b = sub(x, y);
+    }
+    // NTT words are canonical throughout
+    __device__ __forceinline__ static uint32_t mul_ntt(uint32_t x, uint32_t c) { return mul_c(x, c); }
+    __device__ __forceinline__ static uint32_t store_form(uint32_t x) { return x; }
 };
 
 struct Goldilocks {
@@ -109,6 +131,118 @@ struct Goldilocks {
         return canon(t2);
     }
     __device__ __forceinline__ static uint64_t mul_c(uint64_t x, uint64_t c) { return mul(x, c); }
+
+    // ---- NTT words: lazily reduced (any value below 2^64 congruent mod p) ----
+    // The carry chains are written in gfx950 assembly, one block per
+    // operation with VCC as the carry: the compiler's own lowering of
+    // __builtin_addc / __builtin_subc chains folds a borrow into the next
+    // addition as an added 0xFFFFFFFF and then takes that addition's carry,
+    // which is wrong after a borrow (seen on canon_lz(sub_lz(a, b))).  Every
+    // instruction here is a 32-bit VALU op (v_add_co / v_addc / v_sub_co /
+    // v_subb / v_cndmask); no 64-bit compare or (x, 0) register pair.
+    __device__ __forceinline__ static uint64_t pack(uint32_t lo, uint32_t hi) { return (uint64_t(hi) << 32) | lo; }
+    // canonical form: x + (2^32 - 1) carries out of 64 bits exactly when x >= p,
+    // and is then x - p
+    __device__ __forceinline__ static uint64_t canon_lz(uint64_t x) {
+        uint32_t l, h;
+        asm("v_add_co_u32 %0, vcc, -1, %2\n\t"
+            "v_addc_co_u32 %1, vcc, 0, %3, vcc\n\t"
+            "v_cndmask_b32 %0, %2, %0, vcc\n\t"
+            "v_cndmask_b32 %1, %3, %1, vcc"
+            : "=&v"(l), "=&v"(h)
+            : "v"(uint32_t(x)), "v"(uint32_t(x >> 32))
+            : "vcc");
+        return pack(l, h);
+    }
+    // a lazy, b canonical: a + b, lazy.  A carry out of 2^64 is worth
+    // 2^32 - 1; the wrapped sum is below p, so the correction cannot wrap.
+    __device__ __forceinline__ static uint64_t add_lz(uint64_t a, uint64_t b) {
+        uint32_t l, h, m;
+        asm("v_add_co_u32 %0, vcc, %3, %5\n\t"
+            "v_addc_co_u32 %1, vcc, %4, %6, vcc\n\t"
+            "v_cndmask_b32_e64 %2, 0, -1, vcc\n\t"
+            "v_add_co_u32 %0, vcc, %0, %2\n\t"
+            "v_addc_co_u32 %1, vcc, 0, %1, vcc"
+            : "=&v"(l), "=&v"(h), "=&v"(m)
+            : "v"(uint32_t(a)), "v"(uint32_t(a >> 32)), "v"(uint32_t(b)), "v"(uint32_t(b >> 32))
+            : "vcc");
+        return pack(l, h);
+    }
+    // a lazy, b canonical: a - b, lazy.  A borrow (a < b) leaves a - b + 2^64,
+    // which is at least 2^64 - p + 1 = 2^32, so subtracting 2^32 - 1 cannot wrap.
+    __device__ __forceinline__ static uint64_t sub_lz(uint64_t a, uint64_t b) {
+        uint32_t l, h, m;
+        asm("v_sub_co_u32 %0, vcc, %3, %5\n\t"
+            "v_subb_co_u32 %1, vcc, %4, %6, vcc\n\t"
+            "v_cndmask_b32_e64 %2, 0, -1, vcc\n\t"
+            "v_sub_co_u32 %0, vcc, %0, %2\n\t"
+            "v_subbrev_co_u32 %1, vcc, 0, %1, vcc"
+            : "=&v"(l), "=&v"(h), "=&v"(m)
+            : "v"(uint32_t(a)), "v"(uint32_t(a >> 32)), "v"(uint32_t(b)), "v"(uint32_t(b >> 32))
+            : "vcc");
+        return pack(l, h);
+    }
+    // a, b any below 2^64: a * b mod p, lazy.  Four 32 x 32 -> 64 products
+    // give the 128-bit product r3 r2 r1 r0 (32-bit limbs); with 2^64 = 2^32 - 1
+    // and 2^96 = -1 (mod p), x = (r1 r0) - r3 + r2 (2^32 - 1).  A borrow of
+    // the subtraction adds 2^64 - (2^32 - 1) (the difference is then at least
+    // 2^64 - 2^32 + 1, so it cannot wrap); a carry of the addition adds
+    // 2^32 - 1 (the wrapped sum is below 2^64 - 2^33 + 2, so it cannot wrap).
+    __device__ __forceinline__ static uint64_t mul_lz(uint64_t a, uint64_t b) {
+        const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+        const uint64_t p00 = uint64_t(a0) * b0, p01 = uint64_t(a0) * b1, p10 = uint64_t(a1) * b0,
+                       p11 = uint64_t(a1) * b1;
+        uint32_t r1, r2, r3, tl, th, ul, uh, m;
+        asm("v_add_co_u32 %0, vcc, %8, %9\n\t"          // r1 = h00 + l01
+            "v_addc_co_u32 %1, vcc, %10, %11, vcc\n\t"   // r2 = h01 + l11 + c
+            "v_addc_co_u32 %2, vcc, 0, %12, vcc\n\t"     // r3 = h11 + c
+            "v_add_co_u32 %0, vcc, %0, %13\n\t"          // r1 += l10
+            "v_addc_co_u32 %1, vcc, %1, %14, vcc\n\t"    // r2 += h10 + c
+            "v_addc_co_u32 %2, vcc, 0, %2, vcc\n\t"      // r3 += c
+            "v_sub_co_u32 %3, vcc, %15, %2\n\t"          // t = (r1 r0) - r3
+            "v_subbrev_co_u32 %4, vcc, 0, %0, vcc\n\t"
+            "v_cndmask_b32_e64 %7, 0, -1, vcc\n\t"       // borrow: t += 0xFFFFFFFF00000001
+            "v_addc_co_u32 %3, vcc, 0, %3, vcc\n\t"
+            "v_addc_co_u32 %4, vcc, %7, %4, vcc\n\t"
+            "v_sub_co_u32 %5, vcc, 0, %1\n\t"            // u = r2 2^32 - r2
+            "v_subbrev_co_u32 %6, vcc, 0, %1, vcc\n\t"
+            "v_add_co_u32 %3, vcc, %3, %5\n\t"           // t += u
+            "v_addc_co_u32 %4, vcc, %4, %6, vcc\n\t"
+            "v_cndmask_b32_e64 %7, 0, -1, vcc\n\t"       // carry: t += 2^32 - 1
+            "v_add_co_u32 %3, vcc, %3, %7\n\t"
+            "v_addc_co_u32 %4, vcc, 0, %4, vcc"
+            : "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(tl), "=&v"(th), "=&v"(ul), "=&v"(uh), "=&v"(m)
+            : "v"(uint32_t(p00 >> 32)), "v"(uint32_t(p01)), "v"(uint32_t(p01 >> 32)), "v"(uint32_t(p11)),
+              "v"(uint32_t(p11 >> 32)), "v"(uint32_t(p10)), "v"(uint32_t(p10 >> 32)), "v"(uint32_t(p00))
+            : "vcc");
+        return pack(tl, th);
+    }
+
+    // NTT butterflies (ntt.hip) on lazy words, results lazy; w canonical.
+    // The operand added to or subtracted from a is made canonical first, as
+    // add_lz / sub_lz require.  Words are made canonical when stored to global
+    // memory (store_form).
+    __device__ __forceinline__ static void dit(uint64_t& a, uint64_t& b, uint64_t w) {
+        const uint64_t t = canon_lz(mul_lz(b, w)), x = a;
+        a = add_lz(x, t);
+        b = sub_lz(x, t);
+    }
+    __device__ __forceinline__ static void dif(uint64_t& a, uint64_t& b, uint64_t w) {
+        const uint64_t y = canon_lz(b), x = a;
+        a = add_lz(x, y);
+        b = mul_lz(sub_lz(x, y), w);
+    }
+    __device__ __forceinline__ static void bf1(uint64_t& a, uint64_t& b) {
+        const uint64_t y = canon_lz(b), x = a;
+        a = add_lz(x, y);
+        b = sub_lz(x, y);
+    }
+    __device__ __forceinline__ static uint64_t mul_ntt(uint64_t x, uint64_t c) { return mul_lz(x, c); }
+    __device__ __forceinline__ static uint64_t store_form(uint64_t x) { return canon_lz(x); }
 };
+
+// Global-memory form of an NTT word (canonical), by word type.
+__device__ __forceinline__ uint32_t ntt_store_form(uint32_t x) { return BabyBear::store_form(x); }
+__device__ __forceinline__ uint64_t ntt_store_form(uint64_t x) { return Goldilocks::store_form(x); }
 
 }  // namespace p3h

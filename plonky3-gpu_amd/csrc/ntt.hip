@@ -108,10 +108,8 @@ __device__ __forceinline__ void butterflies(typename F::T* x, const typename F::
                 constexpr bool a0 = A0 && Q0 == 0;
                 const bool one = a0 && (j == 0 || (j == 1 && (m & 1u) == 0));
                 const T w = twb[(1u << (Q0 + j)) - 1 + ((m & ((1u << j) - 1)) << Q0)];
-                const T yw = one ? x[m | (1u << j)] : F::mul_c(x[m | (1u << j)], w);
-                const T xv = x[m];
-                x[m] = F::add(xv, yw);
-                x[m | (1u << j)] = F::sub(xv, yw);
+                if (one) F::bf1(x[m], x[m | (1u << j)]);
+                else F::dit(x[m], x[m | (1u << j)], w);
             }
         }
     } else {
@@ -124,9 +122,8 @@ __device__ __forceinline__ void butterflies(typename F::T* x, const typename F::
                 constexpr bool a0 = A0 && Q0 == 0;
                 const bool one = a0 && (j == 0 || (j == 1 && (m & 1u) == 0));
                 const T w = twb[(1u << (Q0 + j)) - 1 + ((m & ((1u << j) - 1)) << Q0)];
-                const T xv = x[m], yv = x[m | (1u << j)];
-                x[m] = F::add(xv, yv);
-                x[m | (1u << j)] = one ? F::sub(xv, yv) : F::mul_c(F::sub(xv, yv), w);
+                if (one) F::bf1(x[m], x[m | (1u << j)]);
+                else F::dif(x[m], x[m | (1u << j)], w);
             }
         }
     }
@@ -192,7 +189,7 @@ struct GlobalIO {
         T* p = dst + uint64_t(tb) * tstride + c;
         const uint64_t step = tstride << Q0;
 #pragma unroll
-        for (uint32_t m = 0; m < (1u << R); ++m) p[m * step] = x[m];
+        for (uint32_t m = 0; m < (1u << R); ++m) p[m * step] = ntt_store_form(x[m]);
     }
 };
 
@@ -243,7 +240,7 @@ struct BufIO {
         const uint32_t vo = tb * tstride_b + c * uint32_t(sizeof(T));
         const uint32_t step = tstride_b << Q0;
 #pragma unroll
-        for (uint32_t m = 0; m < (1u << R); ++m) buf_st(x[m], dst, vo, m * step);
+        for (uint32_t m = 0; m < (1u << R); ++m) buf_st(ntt_store_form(x[m]), dst, vo, m * step);
     }
 };
 
@@ -429,7 +426,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
             if (active) {
                 T y[M];
 #pragma unroll
-                for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_c(x[m], twist[m]);
+                for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_ntt(x[m], twist[m]);
                 butterflies<F, R0, 0, true, true>(y, twf);
                 if constexpr (R1 > 0) lio.template store<R0, 0>(g.c, g.tb, y);
                 else gio_r.template store<R0, 0>(g.c, g.tb, y);
@@ -447,7 +444,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
             if (col >= w) return T(0);
             const uint64_t row = bitrev(uint32_t(pos(t)), A.L);
             T v = src[row * ld + col];
-            return A.scale ? F::mul_c(v, A.scale[row]) : v;
+            return A.scale ? F::mul_ntt(v, A.scale[row]) : v;
         };
         auto stf = [&](uint32_t, uint32_t, T) {};
         const auto gather = elem_io<T>(ldf, stf);
@@ -465,7 +462,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
             const uint32_t col = c0 + c;
             if (col >= w) return;
             const uint64_t row = bitrev(uint32_t(pos(t)), A.L);
-            dst[row * ld + col] = A.scale ? F::mul_c(v, A.scale[row]) : v;
+            dst[row * ld + col] = ntt_store_form(A.scale ? F::mul_ntt(v, A.scale[row]) : v);
         };
         const auto scatter = elem_io<T>(ldf, stf);
         if constexpr (R1 > 0) {
