@@ -39,6 +39,17 @@ FP64_PEAK_TFLOPS = 78.6  # vector and matrix FP64 are the same figure on MI355X
 # each digest lane's final canonical reduction adds 5 flops.
 P2_F64_FLOPS_PER_PERM = 7216
 P2_F64_FLOPS_PER_DIGEST = 8 * 5
+# The Goldilocks leaf hash (integer engine, width 8, 8 full + 22 partial
+# rounds) issues 5.647e10 wave64 VALU instructions for 2^24 leaves x 16
+# permutations (SQ_INSTS_VALU, profiles/valu_counters_gl_2^22x64_b2.md), i.e.
+# 13467 per permutation and lane.  Its VALU roofline is that instruction
+# stream against one wave64 instruction per SIMD every 2 cycles (1024 SIMDs at
+# 2.4 GHz), the issue rate of MI355X's 157.3 TFLOP/s FP32 vector peak; its
+# 64-bit and multiply instructions take longer, so the fraction is a lower
+# bound on how busy the VALU is.
+GL_HASH_VALU_PER_PERM = 13467
+SIMD_COUNT, CLOCK_HZ = 1024, 2.4e9
+VALU_ISSUE_PEAK = SIMD_COUNT * CLOCK_HZ / 2  # wave64 instructions per second
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 FIXTURE = os.path.join(ROOT, "tests", "golden", "configs.npz")
@@ -235,6 +246,7 @@ def main():
     dt = time.perf_counter() - t0
     phases = {ph: ctx.phase_stats(ph) for ph in ("lde", "leaf_hash", "compress")}
     ctx.set_profiling(False)
+    probe_rate = ctx.probe_butterflies(fid)  # the NTT butterflies' VALU ceiling, outside the timed region
     assert all((r == roots[0]).all() for r in roots), "non-deterministic root"
     pinned_root = fixture(f"{args.workload}/root")
     if pinned_root is not None:
@@ -273,11 +285,24 @@ def main():
                     "traffic": traffic.get("hash_rows_dense_f64_kernel"), "algorithmic_flops": flops,
                     "algorithmic_bytes": hash_bytes, "hbm_GBps": hash_bytes / (hash_ms / 1e3) / 1e9,
                     "avg_launch_ms": hash_ms}
+    elif field == "gl":
+        issued = perms * GL_HASH_VALU_PER_PERM / 64 / (hash_ms / 1e3)  # wave64 instructions per second
+        roofline = {"bound": "valu", "kernel": "hash_rows_dense_kernel<GLDom> (Poseidon2 W8 leaf sponge, integer)",
+                    "achieved": issued / 1e9, "peak": VALU_ISSUE_PEAK / 1e9,
+                    "unit": "G wave64 VALU instructions/s", "frac": issued / VALU_ISSUE_PEAK,
+                    "traffic": traffic.get("hash_rows_dense_kernel<GLDom>"), "algorithmic_bytes": hash_bytes,
+                    "permutations_per_s": perms / (hash_ms / 1e3), "valu_instructions_per_perm": GL_HASH_VALU_PER_PERM,
+                    "avg_launch_ms": hash_ms}
     else:
         roofline = {"bound": "valu", "kernel": "hash_rows_dense_kernel (Poseidon2 leaf sponge, integer)",
                     "achieved": perms / (hash_ms / 1e3) / 1e9, "unit": "Gperm/s", "peak": None, "frac": None,
                     "traffic": traffic.get("hash_rows_dense_kernel"), "algorithmic_bytes": hash_bytes,
                     "avg_launch_ms": hash_ms}
+    # The LDE's radix-2 butterflies: the inverse transform and one forward
+    # transform per coset, log_n stages of n/2 butterflies per column each.
+    lde_butterflies = (1 + (1 << lb)) * log_n * (n // 2) * width
+    bf_rate, bf_probe_ms = probe_rate
+    lde_valu_floor_ms = lde_butterflies / bf_rate * 1e3
     out = {
         "metric": METRIC,
         "value": value,
@@ -299,7 +324,18 @@ def main():
         "roofline": roofline,
         "roofline_lde": {"bound": "hbm", "achieved": lde_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": lde_gbs / HBM_PEAK_GBS, "traffic": traffic.get("lde_phase"),
-                         "algorithmic_bytes": lde_bytes, "avg_phase_ms": lde_ms},
+                         "algorithmic_bytes": lde_bytes, "avg_phase_ms": lde_ms,
+                         "hbm_floor_ms": lde_bytes / (HBM_PEAK_GBS * 1e9) * 1e3,
+                         "valu": {"butterflies": lde_butterflies,
+                                  "achieved_butterflies_per_s": lde_butterflies / (lde_ms / 1e3),
+                                  "ceiling_butterflies_per_s": bf_rate,
+                                  "frac": lde_butterflies / (lde_ms / 1e3) / bf_rate,
+                                  "floor_ms": lde_valu_floor_ms,
+                                  "ceiling_from": "p3h_probe_butterflies: the passes' own butterflies on registers, "
+                                                  "no memory traffic, 16384 x 256 threads",
+                                  "probe_ms": bf_probe_ms},
+                         "limiting_floor": "valu" if lde_valu_floor_ms > lde_bytes / (HBM_PEAK_GBS * 1e9) * 1e3
+                         else "hbm"},
         "pcie_inclusive": {"value": elems / (host_ms / 1e3), "unit": "field elems/s", "upload_ms": upload_ms,
                            "note": "trace handed over in pageable host memory: upload + LDE + commit + root"},
         "root": [int(x) for x in roots[0]],

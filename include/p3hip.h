@@ -360,6 +360,16 @@ const char* p3h_serve_last_error(p3h_serve* sv);
  * destroy has begun (submissions racing with it get P3H_ESTATE). */
 int p3h_serve_destroy(p3h_serve* sv);
 
+/* ----------------------------------------------------------- measurement */
+/* The VALU ceiling of the NTT arithmetic: `blocks` x 256 threads each take 16
+ * words through iters x 32 of the NTT passes' own radix-2 butterflies (a
+ * modular product, sum and difference) in registers, with no memory traffic
+ * in the timed loop.  *ms: the kernel time (HIP events on the context's
+ * stream); *butterflies: the butterflies it ran.  No counterpart in the
+ * reference; the benchmark reports the rate as the passes' VALU roofline. */
+int p3h_probe_butterflies(p3h_ctx* ctx, int field, uint32_t blocks, uint32_t iters, double* ms,
+                          double* butterflies);
+
 /* ------------------------------------------------------------ autotuner */
 /* Time the LDE of one shape under candidate pass shapes (columns per LDS tile,
  * stages per pass, column slab) and record the fastest for that (field,

@@ -118,5 +118,11 @@ int lde_twist_device(typename F::T* out, uint32_t L, uint32_t n_cosets, const ty
 
 template <class T>
 int coset_to_natural_device(const T* in, T* out, uint32_t L, uint32_t k, uint32_t w, hipStream_t st);
+// VALU ceiling probe of the NTT butterflies: blocks x 256 threads, each
+// `iters` x 32 butterflies on registers (out: blocks x 256 words; tw: 16
+// canonical twiddles in the field's multiply-constant form).
+template <class F>
+int butterfly_probe_device(typename F::T* out, const typename F::T* tw, uint32_t blocks, uint32_t iters,
+                           hipStream_t st);
 
 }  // namespace p3h

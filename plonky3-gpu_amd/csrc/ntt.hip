@@ -1037,6 +1037,51 @@ int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typenam
     return int(hipGetLastError());
 }
 
+namespace {
+
+// VALU ceiling of the NTT arithmetic: each thread takes 16 words through the
+// passes' own DIT butterfly (F::dit) in radix-2^4 register rounds -- 4 stages
+// of 8 butterflies, twiddles held in registers -- `iters` times, with no
+// memory traffic inside the loop; the words are xor-folded into one store so
+// the work cannot be dropped.
+template <class F>
+__global__ __launch_bounds__(256) void butterfly_probe_kernel(typename F::T* out, const typename F::T* tw,
+                                                              uint32_t iters) {
+    using T = typename F::T;
+    constexpr int R = 4, M = 1 << R;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    T x[M], w[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        x[m] = T(tid * 31u + uint32_t(m)) % T(F::P);
+        w[m] = tw[m];
+    }
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if (m & (1 << j)) continue;
+                F::dit(x[m], x[m | (1 << j)], w[(m + j) & (M - 1)]);
+            }
+    }
+    T acc = 0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc ^= x[m];
+    out[tid] = acc;
+}
+
+}  // namespace
+
+template <class F>
+int butterfly_probe_device(typename F::T* out, const typename F::T* tw, uint32_t blocks, uint32_t iters,
+                           hipStream_t st) {
+    hipLaunchKernelGGL((butterfly_probe_kernel<F>), dim3(blocks), dim3(256), 0, st, out, tw, iters);
+    return int(hipGetLastError());
+}
+template int butterfly_probe_device<BabyBear>(uint32_t*, const uint32_t*, uint32_t, uint32_t, hipStream_t);
+template int butterfly_probe_device<Goldilocks>(uint64_t*, const uint64_t*, uint32_t, uint32_t, hipStream_t);
+
 template std::vector<LdePassGroup<uint32_t>> plan_lde_batch<BabyBear>(const std::vector<LdeJob<uint32_t>>&);
 template std::vector<LdePassGroup<uint64_t>> plan_lde_batch<Goldilocks>(const std::vector<LdeJob<uint64_t>>&);
 template int launch_lde_group<BabyBear>(const LdePassGroup<uint32_t>&, const NttWork<uint32_t>*, hipStream_t);

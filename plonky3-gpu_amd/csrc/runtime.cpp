@@ -1626,6 +1626,47 @@ int p3h_seeded_fill(int field, uint64_t seed, void* out, uint64_t n) {
     return P3H_OK;
 }
 
+int p3h_probe_butterflies(p3h_ctx* ctx, int field, uint32_t blocks, uint32_t iters, double* ms,
+                          double* butterflies) {
+    return api(ctx, [&] {
+        if (!ms || !butterflies || blocks == 0 || iters == 0) throw Err{P3H_EINVAL, "bad probe arguments"};
+        if (field != P3H_BABYBEAR && field != P3H_GOLDILOCKS) throw Err{P3H_EINVAL, "unknown field"};
+        const size_t eb = field == P3H_BABYBEAR ? 4 : 8;
+        DevPtr out = dalloc(ctx, size_t(blocks) * 256 * eb), tw = dalloc(ctx, 16 * eb);
+        uint64_t host[16];
+        for (int i = 0; i < 16; ++i) host[i] = 1000003ull * uint64_t(i + 1) + 17;
+        if (field == P3H_BABYBEAR) {
+            uint32_t h32[16];
+            for (int i = 0; i < 16; ++i) h32[i] = bb_mont(uint32_t(host[i] % BB_P));
+            h2d(ctx, tw->ptr, h32, sizeof(h32));
+        } else {
+            h2d(ctx, tw->ptr, host, sizeof(host));
+        }
+        auto launch = [&](uint32_t it) {
+            launch_check(field == P3H_BABYBEAR
+                             ? butterfly_probe_device<BabyBear>(dptr<uint32_t>(out), dptr<uint32_t>(tw), blocks, it,
+                                                                ctx->stream)
+                             : butterfly_probe_device<Goldilocks>(dptr<uint64_t>(out), dptr<uint64_t>(tw), blocks,
+                                                                  it, ctx->stream),
+                         "butterfly probe");
+        };
+        launch(2);  // warm the clocks and the code object
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+        hip_check(hipEventRecord(a, ctx->stream), "hipEventRecord");
+        launch(iters);
+        hip_check(hipEventRecord(b, ctx->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+        float t = 0;
+        hip_check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *ms = t;
+        *butterflies = double(blocks) * 256.0 * double(iters) * 32.0;
+    });
+}
+
 int p3h_ctx_table_bytes(const p3h_ctx* ctx, uint64_t* bytes) {
     if (!ctx || !bytes) return P3H_EINVAL;
     *bytes = ctx->table_bytes;

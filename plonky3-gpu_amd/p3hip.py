@@ -59,6 +59,7 @@ _SIGS = {
     "p3h_ctx_memory": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "p3h_ctx_trim": (_i32, [_vp]),
     "p3h_ctx_table_bytes": (_i32, [_vp, C.POINTER(_u64)]),
+    "p3h_probe_butterflies": (_i32, [_vp, _i32, _u32, _u32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "p3h_ctx_set_profiling": (_i32, [_vp, _i32]),
     "p3h_phase_stats": (_i32, [_vp, C.c_char_p, C.POINTER(_u64), C.POINTER(C.c_double)]),
     "p3h_ctx_set_ntt_tuning": (_i32, [_vp, _u32, _u32, _u32]),
@@ -249,6 +250,13 @@ class Context:
         a, b, c = _u64(), _u64(), _u64()
         self._check(self.lib.p3h_ctx_memory(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"in_use": a.value, "budget": b.value, "cached": c.value}
+
+    def probe_butterflies(self, field: int, blocks: int = 16384, iters: int = 200):
+        """(butterflies per second, kernel ms): the NTT butterflies' VALU
+        ceiling (p3h_probe_butterflies)."""
+        ms, n = C.c_double(), C.c_double()
+        self._check(self.lib.p3h_probe_butterflies(self.h, field, blocks, iters, C.byref(ms), C.byref(n)))
+        return n.value / (ms.value / 1e3), ms.value
 
     def table_bytes(self) -> int:
         """Bytes of in_use held by the context's table caches."""
