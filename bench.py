@@ -276,7 +276,8 @@ def main():
     # leaf hashing: one launch absorbs every LDE row
     perms = N * ((width + rate - 1) // rate)
     hash_bytes = eb * width * N + eb * dig * N
-    traffic = load_traffic(args.workload)
+    shape = tuned_shape(tuned, args.tuning_file, field, log_n, width, lb)
+    traffic = load_traffic(args.workload, shape)
     if field == "bb" and args.hash_engine == 1:
         flops = perms * P2_F64_FLOPS_PER_PERM + N * P2_F64_FLOPS_PER_DIGEST
         ach = flops / (hash_ms / 1e3) / 1e12
@@ -363,14 +364,31 @@ def main():
     print(json.dumps(out))
 
 
-def load_traffic(workload: str) -> dict:
+def load_traffic(workload: str, shape) -> dict:
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-    (each pass calibrated against a 1 GiB copy), kept as a profile summary."""
+    (each pass calibrated against a 1 GiB copy), kept as a profile summary,
+    for the LDE pass shape (log_cols, max_stages, log_slab) this run uses;
+    empty when that shape was not measured."""
+    if shape is None:
+        return {}
     try:
         with open(TRAFFIC_FILE) as f:
-            return json.load(f).get(workload, {})
+            return json.load(f).get(workload, {}).get(" ".join(str(int(v)) for v in shape), {})
     except (OSError, ValueError):
         return {}
+
+
+def tuned_shape(tuned, tuning_file, field: str, log_n: int, width: int, lb: int):
+    """(log_cols, max_stages, log_slab) the LDE of this run uses."""
+    if tuned and "log_cols" in tuned:
+        return tuned["log_cols"], tuned["max_stages"], tuned["log_slab"]
+    if tuning_file and os.path.exists(tuning_file):
+        key = [0 if field == "bb" else 1, log_n, width, lb]
+        for line in open(tuning_file):
+            v = [int(x) for x in line.split()]
+            if v[:4] == key:
+                return v[4], v[5], v[6]
+    return None
 
 
 def calibrate_copy(nbytes: int = 1 << 30, reps: int = 3):
