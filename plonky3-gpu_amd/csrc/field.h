@@ -23,6 +23,9 @@ struct BabyBear {
     static constexpr uint32_t NEG_PINV = 0x77FFFFFFu;  // -p^-1 mod 2^32
     static constexpr uint32_t R_MOD_P = 0x0FFFFFFEu;   // 2^32 mod p
     static constexpr uint32_t R2_MOD_P = 0x45DDDDE3u;  // 2^64 mod p
+    // no power-of-two roots of unity to exploit in the NTT
+    static constexpr bool POW2_ROOTS = false;
+    static constexpr int LOG2_ROOT64_FWD = 0, LOG2_ROOT64_INV = 0;
 
     // [0, 2p) -> [0, p): for x < p, x - p wraps above 2^32 - p > x.
     __device__ __forceinline__ static uint32_t reduce2p(uint32_t x) { return min(x, x - P); }
@@ -71,6 +74,12 @@ struct Goldilocks {
     using T = uint64_t;
     static constexpr uint64_t P = 0xFFFFFFFF00000001ull;
     static constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
+    // 2 has order 192 (2^96 = -1), so every 64th root of unity is a power of
+    // two.  With generator 7 (omega_{2^32} = 7^((p-1)/2^32)) the NTT's
+    // omega_64 is 2^39 and omega_64^-1 = 2^153; omega_{2^(j+1)}^u is then
+    // 2^(39 u 2^(5-j) mod 192) for j <= 5.
+    static constexpr bool POW2_ROOTS = true;
+    static constexpr int LOG2_ROOT64_FWD = 39, LOG2_ROOT64_INV = 153;
 
     __device__ __forceinline__ static uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
     __device__ __forceinline__ static uint64_t add(uint64_t a, uint64_t b) {
@@ -216,6 +225,128 @@ struct Goldilocks {
               "v"(uint32_t(p11 >> 32)), "v"(uint32_t(p10)), "v"(uint32_t(p10 >> 32)), "v"(uint32_t(p00))
             : "vcc");
         return pack(tl, th);
+    }
+
+    // x lazy, 0 <= SH < 96: x * 2^SH mod p, lazy.  With x * 2^(SH mod 32) =
+    // q2 q1 q0 (32-bit limbs; q2 < 2^31) and 2^64 = 2^32 - 1, 2^96 = -1:
+    //   SH < 32: (q1 q0) + q2 (2^32 - 1)
+    //   SH < 64: (q0 0) + q1 (2^32 - 1) - q2
+    //   SH < 96: q0 (2^32 - 1) - (q2 q1)
+    // u = q (2^32 - 1) is q 2^32 - q: (q - [q > 0], -q).  A carry out of 2^64
+    // adds 2^32 - 1 and a borrow subtracts it; in each case the corrected word
+    // stays inside [0, 2^64) (the wrapped sum is below 2^64 - 2^32, the wrapped
+    // difference at least 2^63), so one correction suffices.
+    template <int SH>
+    __device__ __forceinline__ static uint64_t mul_pow2_lz(uint64_t x) {
+        static_assert(SH >= 0 && SH < 96, "shift below 96");
+        if constexpr (SH == 0) {
+            return x;
+        } else {
+            constexpr int S1 = SH & 31;
+            const uint32_t x0 = uint32_t(x), x1 = uint32_t(x >> 32);
+            uint32_t q0, q1, q2;
+            if constexpr (S1 == 0) {
+                q0 = x0;
+                q1 = x1;
+                q2 = 0;
+            } else {
+                q0 = x0 << S1;
+                q1 = (x1 << S1) | (x0 >> (32 - S1));
+                q2 = x1 >> (32 - S1);
+            }
+            uint32_t tl, th, ul, uh, m;
+            if constexpr (SH < 32) {
+                asm("v_sub_co_u32 %2, vcc, 0, %7\n\t"          // u = q2 2^32 - q2
+                    "v_subbrev_co_u32 %3, vcc, 0, %7, vcc\n\t"
+                    "v_add_co_u32 %0, vcc, %5, %2\n\t"         // t = (q1 q0) + u
+                    "v_addc_co_u32 %1, vcc, %6, %3, vcc\n\t"
+                    "v_cndmask_b32_e64 %4, 0, -1, vcc\n\t"     // carry: t += 2^32 - 1
+                    "v_add_co_u32 %0, vcc, %0, %4\n\t"
+                    "v_addc_co_u32 %1, vcc, 0, %1, vcc"
+                    : "=&v"(tl), "=&v"(th), "=&v"(ul), "=&v"(uh), "=&v"(m)
+                    : "v"(q0), "v"(q1), "v"(q2)
+                    : "vcc");
+            } else if constexpr (SH < 64 && S1 == 0) {
+                asm("v_sub_co_u32 %2, vcc, 0, %6\n\t"          // u = q1 2^32 - q1
+                    "v_subbrev_co_u32 %3, vcc, 0, %6, vcc\n\t"
+                    "v_add_co_u32 %1, vcc, %3, %5\n\t"         // t = (q0 0) + u
+                    "v_cndmask_b32_e64 %4, 0, -1, vcc\n\t"     // carry: t += 2^32 - 1
+                    "v_add_co_u32 %0, vcc, %2, %4\n\t"
+                    "v_addc_co_u32 %1, vcc, 0, %1, vcc"
+                    : "=&v"(tl), "=&v"(th), "=&v"(ul), "=&v"(uh), "=&v"(m)
+                    : "v"(q0), "v"(q1)
+                    : "vcc");
+            } else if constexpr (SH < 64) {
+                asm("v_sub_co_u32 %2, vcc, 0, %6\n\t"          // u = q1 2^32 - q1
+                    "v_subbrev_co_u32 %3, vcc, 0, %6, vcc\n\t"
+                    "v_add_co_u32 %1, vcc, %3, %5\n\t"         // t = (q0 0) + u
+                    "v_cndmask_b32_e64 %4, 0, -1, vcc\n\t"     // carry: t += 2^32 - 1
+                    "v_add_co_u32 %0, vcc, %2, %4\n\t"
+                    "v_addc_co_u32 %1, vcc, 0, %1, vcc\n\t"
+                    "v_sub_co_u32 %0, vcc, %0, %7\n\t"         // t -= q2
+                    "v_subbrev_co_u32 %1, vcc, 0, %1, vcc\n\t"
+                    "v_cndmask_b32_e64 %4, 0, -1, vcc\n\t"     // borrow: t -= 2^32 - 1
+                    "v_sub_co_u32 %0, vcc, %0, %4\n\t"
+                    "v_subbrev_co_u32 %1, vcc, 0, %1, vcc"
+                    : "=&v"(tl), "=&v"(th), "=&v"(ul), "=&v"(uh), "=&v"(m)
+                    : "v"(q0), "v"(q1), "v"(q2)
+                    : "vcc");
+            } else if constexpr (S1 == 0) {
+                asm("v_sub_co_u32 %2, vcc, 0, %5\n\t"          // u = q0 2^32 - q0
+                    "v_subbrev_co_u32 %3, vcc, 0, %5, vcc\n\t"
+                    "v_sub_co_u32 %0, vcc, %2, %6\n\t"         // t = u - q1
+                    "v_subbrev_co_u32 %1, vcc, 0, %3, vcc\n\t"
+                    "v_cndmask_b32_e64 %4, 0, -1, vcc\n\t"     // borrow: t -= 2^32 - 1
+                    "v_sub_co_u32 %0, vcc, %0, %4\n\t"
+                    "v_subbrev_co_u32 %1, vcc, 0, %1, vcc"
+                    : "=&v"(tl), "=&v"(th), "=&v"(ul), "=&v"(uh), "=&v"(m)
+                    : "v"(q0), "v"(q1)
+                    : "vcc");
+            } else {
+                asm("v_sub_co_u32 %2, vcc, 0, %5\n\t"          // u = q0 2^32 - q0
+                    "v_subbrev_co_u32 %3, vcc, 0, %5, vcc\n\t"
+                    "v_sub_co_u32 %0, vcc, %2, %6\n\t"         // t = u - (q2 q1)
+                    "v_subb_co_u32 %1, vcc, %3, %7, vcc\n\t"
+                    "v_cndmask_b32_e64 %4, 0, -1, vcc\n\t"     // borrow: t -= 2^32 - 1
+                    "v_sub_co_u32 %0, vcc, %0, %4\n\t"
+                    "v_subbrev_co_u32 %1, vcc, 0, %1, vcc"
+                    : "=&v"(tl), "=&v"(th), "=&v"(ul), "=&v"(uh), "=&v"(m)
+                    : "v"(q0), "v"(q1), "v"(q2)
+                    : "vcc");
+            }
+            return pack(tl, th);
+        }
+    }
+
+    // Butterflies whose twiddle is 2^E (0 <= E < 192, 2^96 = -1), lazy words
+    // as dit / dif.  A twiddle -2^(E-96) swaps the sum and the difference; in
+    // the DIF form the difference is then taken the other way round, which
+    // needs the other operand canonical.
+    template <int E>
+    __device__ __forceinline__ static void dit_p2(uint64_t& a, uint64_t& b) {
+        if constexpr (E == 0) {
+            bf1(a, b);
+        } else {
+            const uint64_t t = canon_lz(mul_pow2_lz<E % 96>(b)), x = a;
+            if constexpr (E < 96) {
+                a = add_lz(x, t);
+                b = sub_lz(x, t);
+            } else {
+                a = sub_lz(x, t);
+                b = add_lz(x, t);
+            }
+        }
+    }
+    template <int E>
+    __device__ __forceinline__ static void dif_p2(uint64_t& a, uint64_t& b) {
+        if constexpr (E == 0) {
+            bf1(a, b);
+        } else {
+            const uint64_t y = canon_lz(b), x = a;
+            a = add_lz(x, y);
+            if constexpr (E < 96) b = mul_pow2_lz<E>(sub_lz(x, y));
+            else b = mul_pow2_lz<E - 96>(sub_lz(y, canon_lz(x)));
+        }
     }
 
     // NTT butterflies (ntt.hip) on lazy words, results lazy; w canonical.

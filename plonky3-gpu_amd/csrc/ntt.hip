@@ -90,16 +90,55 @@ struct Tile {
     static constexpr uint32_t STRIDE = B + 1;
 };
 
+template <class Fn, int... I>
+__device__ __forceinline__ void static_for_impl(Fn& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, i>) for i = 0 .. N-1, each i a constant expression
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// The low round of an inner pass (index bits [0, R), a = 0) on a field whose
+// small roots of unity are powers of two: the stage-j twiddle of the pair
+// (m, m + 2^j) is omega_{2^(j+1)}^(m mod 2^j) = 2^E with
+// E = log2(omega_64) (m mod 2^j) 2^(5-j) mod 192, a constant of the unrolled
+// code, so every product is a shift and fold (F::dit_p2 / F::dif_p2) and the
+// twiddle table is not read.  DIT runs the forward roots, DIF the inverse
+// ones: the direction of the fused LDE pass's two halves.
+template <class F, int R, bool DIT>
+__device__ __forceinline__ void butterflies_p2(typename F::T* x) {
+    static_assert(R <= 6, "roots of order at most 64");
+    constexpr int M = 1 << R;
+    constexpr int LOG2W = DIT ? F::LOG2_ROOT64_FWD : F::LOG2_ROOT64_INV;
+    static_for<R>([&](auto jj) {
+        constexpr int j = DIT ? int(decltype(jj)::value) : R - 1 - int(decltype(jj)::value);
+        static_for<M>([&](auto mm) {
+            constexpr int m = decltype(mm)::value;
+            if constexpr ((m & (1 << j)) == 0) {
+                constexpr int E = LOG2W * (m & ((1 << j) - 1)) * (1 << (5 - j)) % 192;
+                if constexpr (DIT) F::template dit_p2<E>(x[m], x[m | (1 << j)]);
+                else F::template dif_p2<E>(x[m], x[m | (1 << j)]);
+            }
+        });
+    });
+}
+
 // Butterflies of one radix-2^R round on a register group x[0..2^R) holding
 // tile positions tb + m * 2^Q0; twb = tws + (tb mod 2^Q0).
 // A0: the pass starts at index bit 0 (a = 0, so lo0 = 0) and this is its low
 // round (Q0 = 0).  Then tws[0] = omega^0 and tws[1] = omega^0: the stage of
 // bit 0 and the even-m butterflies of bit 1 multiply by 1, and skip it.
-template <class F, int R, int Q0, bool DIT, bool A0 = false>
+// P2 (with A0): the twiddles are the forward roots for DIT and the inverse
+// roots for DIF, and a field with power-of-two roots runs butterflies_p2.
+template <class F, int R, int Q0, bool DIT, bool A0 = false, bool P2 = false>
 __device__ __forceinline__ void butterflies(typename F::T* x, const typename F::T* twb) {
     using T = typename F::T;
     constexpr uint32_t M = 1u << R;
-    if constexpr (DIT) {
+    if constexpr (P2 && A0 && Q0 == 0 && F::POW2_ROOTS) {
+        butterflies_p2<F, R, DIT>(x);
+    } else if constexpr (DIT) {
 #pragma unroll
         for (uint32_t j = 0; j < R; ++j) {
 #pragma unroll
@@ -416,7 +455,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         if (active) {
             if constexpr (R1 > 0) lio.template load<R0, 0>(g.c, g.tb, x);
             else gio.template load<R0, 0>(g.c, g.tb, x);
-            butterflies<F, R0, 0, false, true>(x, tws);
+            butterflies<F, R0, 0, false, true, true>(x, tws);
         }
         for (int r = int(A.n_regions) - 1; r >= 0; --r) {
             const T* twist = A.scale + uint64_t(r) * n + row0 + g.tb;
@@ -427,7 +466,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
                 T y[M];
 #pragma unroll
                 for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_ntt(x[m], twist[m]);
-                butterflies<F, R0, 0, true, true>(y, twf);
+                butterflies<F, R0, 0, true, true, true>(y, twf);
                 if constexpr (R1 > 0) lio.template store<R0, 0>(g.c, g.tb, y);
                 else gio_r.template store<R0, 0>(g.c, g.tb, y);
             }
