@@ -44,6 +44,8 @@ struct BabyBear {
     __device__ __forceinline__ static uint32_t mont(uint32_t a, uint32_t b) { return reduce2p(mont_lazy(a, b)); }
     // canonical x times a constant held in Montgomery form
     __device__ __forceinline__ static uint32_t mul_c(uint32_t x, uint32_t c_mont) { return mont(x, c_mont); }
+    // -c of a canonical value (or of its multiply-constant form)
+    __device__ __forceinline__ static uint32_t neg_c(uint32_t c) { return c ? P - c : 0u; }
     __device__ __forceinline__ static uint32_t to_mont(uint32_t x) { return mont(x, R2_MOD_P); }
     __device__ __forceinline__ static uint32_t from_mont(uint32_t x) { return mont(x, 1u); }
 
@@ -140,6 +142,8 @@ struct Goldilocks {
         return canon(t2);
     }
     __device__ __forceinline__ static uint64_t mul_c(uint64_t x, uint64_t c) { return mul(x, c); }
+    // -c of a canonical value
+    __device__ __forceinline__ static uint64_t neg_c(uint64_t c) { return c ? P - c : 0ull; }
 
     // ---- NTT words: lazily reduced (any value below 2^64 congruent mod p) ----
     // The carry chains are written in gfx950 assembly, one block per

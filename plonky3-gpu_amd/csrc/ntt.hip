@@ -83,6 +83,78 @@ __device__ __forceinline__ void build_twiddles(typename F::T* tws, const typenam
     }
 }
 
+// Rounds of an S-stage pass: low = index bits [0, R0), high = [R0, S).
+__host__ __device__ constexpr uint32_t low_round_bits(uint32_t s) { return s <= 5 ? s : (s + 1) / 2; }
+
+// Words of one direction's twiddle table in a pass's LDS.  Fields without
+// power-of-two roots keep the stage table (2^S - 1 words); fields with them
+// keep the twisted round table of build_twisted (2^R0 words for the low round,
+// 2^S for the high round).
+__host__ __device__ constexpr uint32_t tw_table_words(bool twisted, uint32_t s) {
+    return s == 0 ? 0u
+                  : !twisted ? (1u << s) - 1
+                             : (low_round_bits(s) < s ? (1u << s) + (1u << low_round_bits(s)) : (1u << s));
+}
+
+// Twisted rounds (fields whose 64th roots of unity are powers of two).  In a
+// radix-2^R round at position base u = tb mod 2^Q0, the stage-j twiddle of the
+// pair (m, m + 2^j) factors as beta_j * zeta_j^(m mod 2^j) with
+// zeta_j = omega_{2^(j+1)} and beta_j = gamma^(2^(R-1-j)),
+// gamma = omega_{2^(a+Q0+R)}^(u 2^a + lo0).  The round is then
+//   DIT: x[m] *= gamma^bitrev_R(m), then the plain radix-2^R DIT on the zetas;
+//   DIF: the plain radix-2^R DIF on the zetas, then x[m] *= gamma^bitrev_R(m),
+// (induction on R: the top stage's twiddles beta_j zeta_j^k are exactly what
+// the scaled inputs contribute), so a round costs 2^R - 1 general products and
+// R 2^(R-1) power-of-two products instead of R 2^(R-1) general products.
+// Table: the low round (Q0 = 0, u = 0) at [0, 2^R0), the high round
+// (Q0 = R0) at 2^R0 + m 2^R0 + u, so that lanes of neighbouring u read
+// neighbouring words.  gamma^k = omega_{2^(Q0+R)}^(k u) times
+// omega_{2^(a+Q0+R)}^(k lo0): the first from the stage table twc, the second
+// (outer passes only) from the size-n table tw; an exponent in the upper half
+// of a table's range is the negated entry of the lower half.
+template <class F, int S>
+__device__ __forceinline__ void build_twisted(typename F::T* tt, const typename F::T* twc,
+                                              const typename F::T* tw, uint32_t L, uint32_t a, uint32_t lo0) {
+    using T = typename F::T;
+    constexpr uint32_t R0 = low_round_bits(S), R1 = S - R0;
+    constexpr uint32_t count = tw_table_words(true, S);
+    const uint64_t half = 1ull << (L - 1);  // a pass with S >= 1 has L >= 1
+    for (uint32_t i = threadIdx.x; i < count; i += NTT_THREADS) {
+        uint32_t q0, r, u, m;
+        if (i < (1u << R0)) {
+            q0 = 0;
+            r = R0;
+            u = 0;
+            m = i;
+        } else {
+            const uint32_t j = i - (1u << R0);
+            q0 = R0;
+            r = R1;
+            m = j >> R0;
+            u = j & ((1u << R0) - 1);
+        }
+        const uint32_t k = bitrev(m, r);
+        const uint32_t q = q0 + r - 1;  // stage table of the roots of order 2^(q0 + r)
+        const uint32_t x = (k * u) & ((2u << q) - 1);
+        T v = x < (1u << q) ? twc[(1u << q) - 1 + x] : F::neg_c(twc[(1u << q) - 1 + x - (1u << q)]);
+        if (lo0) {
+            const uint64_t e = (uint64_t(k) * lo0) << (L - a - q0 - r);
+            v = F::mul_c(v, e < half ? tw[e] : F::neg_c(tw[e - half]));
+        }
+        tt[i] = v;
+    }
+}
+
+// The twiddles a round's butterflies read, for the group with block index b:
+// the stage table at b mod 2^Q0, or the twisted table at that group's u
+// (entry m at twb[m << Q0]).
+template <class F, int Q0>
+__device__ __forceinline__ const typename F::T* round_tw(const typename F::T* tws, uint32_t b) {
+    const uint32_t u = b & ((1u << Q0) - 1);
+    if constexpr (F::POW2_ROOTS) return tws + (Q0 == 0 ? 0u : (1u << Q0)) + u;
+    else return tws + u;
+}
+
 // LDS tile: element (t, c) at c * (2^S + 1) + t.
 template <int S>
 struct Tile {
@@ -126,18 +198,29 @@ __device__ __forceinline__ void butterflies_p2(typename F::T* x) {
 }
 
 // Butterflies of one radix-2^R round on a register group x[0..2^R) holding
-// tile positions tb + m * 2^Q0; twb = tws + (tb mod 2^Q0).
+// tile positions tb + m * 2^Q0; twb = round_tw(tws, b).
 // A0: the pass starts at index bit 0 (a = 0, so lo0 = 0) and this is its low
 // round (Q0 = 0).  Then tws[0] = omega^0 and tws[1] = omega^0: the stage of
 // bit 0 and the even-m butterflies of bit 1 multiply by 1, and skip it.
-// P2 (with A0): the twiddles are the forward roots for DIT and the inverse
-// roots for DIF, and a field with power-of-two roots runs butterflies_p2.
-template <class F, int R, int Q0, bool DIT, bool A0 = false, bool P2 = false>
+// A field with power-of-two roots runs the twisted form (build_twisted);
+// with A0 its gamma is 1 and only the power-of-two butterflies remain.  DIT
+// passes run the forward roots and DIF passes the inverse ones, the
+// directions of butterflies_p2's two forms.
+template <class F, int R, int Q0, bool DIT, bool A0 = false>
 __device__ __forceinline__ void butterflies(typename F::T* x, const typename F::T* twb) {
     using T = typename F::T;
     constexpr uint32_t M = 1u << R;
-    if constexpr (P2 && A0 && Q0 == 0 && F::POW2_ROOTS) {
+    if constexpr (F::POW2_ROOTS) {
+        constexpr bool unit = A0 && Q0 == 0;
+        if constexpr (DIT && !unit) {
+#pragma unroll
+            for (uint32_t m = 1; m < M; ++m) x[m] = F::mul_ntt(x[m], twb[m << Q0]);
+        }
         butterflies_p2<F, R, DIT>(x);
+        if constexpr (!DIT && !unit) {
+#pragma unroll
+            for (uint32_t m = 1; m < M; ++m) x[m] = F::mul_ntt(x[m], twb[m << Q0]);
+        }
     } else if constexpr (DIT) {
 #pragma unroll
         for (uint32_t j = 0; j < R; ++j) {
@@ -315,7 +398,7 @@ __device__ __forceinline__ void round_io(const typename F::T* tws, uint32_t logC
         Group<R, Q0> g(gi, logC);
         T x[M];
         in.template load<R, Q0>(g.c, g.tb, x);
-        butterflies<F, R, Q0, DIT, A0>(x, tws + (g.b & ((1u << Q0) - 1)));
+        butterflies<F, R, Q0, DIT, A0>(x, round_tw<F, Q0>(tws, g.b));
         out.template store<R, Q0>(g.c, g.tb, x);
     }
 }
@@ -354,7 +437,7 @@ __device__ __forceinline__ void outer_split(const typename F::T* tws, const IO& 
 #pragma unroll
     for (uint32_t g = 0; g < GI; ++g) {
         const Group<RI, QI> gr(threadIdx.x + g * NTT_THREADS, LOGC);
-        butterflies<F, RI, QI, DIT>(v + g * MI, tws + (gr.b & ((1u << QI) - 1)));
+        butterflies<F, RI, QI, DIT>(v + g * MI, round_tw<F, QI>(tws, gr.b));
     }
 #pragma unroll
     for (uint32_t p = 0; p < 2; ++p) {
@@ -378,7 +461,7 @@ __device__ __forceinline__ void outer_split(const typename F::T* tws, const IO& 
 #pragma unroll
     for (uint32_t g = 0; g < GO; ++g) {
         const Group<RO, QO> gr(threadIdx.x + g * NTT_THREADS, LOGC);
-        butterflies<F, RO, QO, DIT>(v + g * MO, tws + (gr.b & ((1u << QO) - 1)));
+        butterflies<F, RO, QO, DIT>(v + g * MO, round_tw<F, QO>(tws, gr.b));
         gio.template store<RO, QO>(gr.c, gr.tb, v + g * MO);
     }
 }
@@ -404,7 +487,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     const uint32_t a = A.a, logC = A.logC;
     T* lds = smem;
     T* tws = smem + (R1 > 0 ? (STRIDE << (SPLIT ? logC - 1 : logC)) : 0u);
-    T* twf = tws + ((1u << S) - 1);  // forward twiddles of the fused inner pass
+    T* twf = tws + tw_table_words(F::POW2_ROOTS, S);  // forward twiddles of the fused inner pass
 
     // block -> (column tile, low index, high index, region)
     uint32_t bid = block;
@@ -425,9 +508,14 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
     auto pos = [&](uint32_t t) -> uint64_t { return row0 | (uint64_t(t) << a); };
 
     if constexpr (S > 0) {
-        build_twiddles<F>(tws, A.twc, A.tw, A.L, a, S, lo0);
-        // the fused inner pass (a = 0, lo0 = 0): forward stage table after the inverse one
-        if (MODE == INNER_LDE) build_twiddles<F>(twf, A.twc + TWC_LEN, A.tw + (n >> 1), A.L, a, S, lo0);
+        // the fused inner pass (a = 0, lo0 = 0): forward table after the inverse one
+        if constexpr (F::POW2_ROOTS) {
+            build_twisted<F, S>(tws, A.twc, A.tw, A.L, a, lo0);
+            if (MODE == INNER_LDE) build_twisted<F, S>(twf, A.twc + TWC_LEN, A.tw + (n >> 1), A.L, a, lo0);
+        } else {
+            build_twiddles<F>(tws, A.twc, A.tw, A.L, a, S, lo0);
+            if (MODE == INNER_LDE) build_twiddles<F>(twf, A.twc + TWC_LEN, A.tw + (n >> 1), A.L, a, S, lo0);
+        }
         __syncthreads();
     }
 
@@ -455,7 +543,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
         if (active) {
             if constexpr (R1 > 0) lio.template load<R0, 0>(g.c, g.tb, x);
             else gio.template load<R0, 0>(g.c, g.tb, x);
-            butterflies<F, R0, 0, false, true, true>(x, tws);
+            butterflies<F, R0, 0, false, true>(x, tws);
         }
         for (int r = int(A.n_regions) - 1; r >= 0; --r) {
             const T* twist = A.scale + uint64_t(r) * n + row0 + g.tb;
@@ -466,7 +554,7 @@ __device__ __forceinline__ void ntt_pass_body(const NttArgs<typename F::T>& A, u
                 T y[M];
 #pragma unroll
                 for (uint32_t m = 0; m < M; ++m) y[m] = F::mul_ntt(x[m], twist[m]);
-                butterflies<F, R0, 0, true, true, true>(y, twf);
+                butterflies<F, R0, 0, true, true>(y, twf);
                 if constexpr (R1 > 0) lio.template store<R0, 0>(g.c, g.tb, y);
                 else gio_r.template store<R0, 0>(g.c, g.tb, y);
             }
@@ -713,15 +801,15 @@ uint64_t finalize_args(NttArgs<T>& A, int mode, uint32_t s, const NttTuning& tun
 
 // Dynamic LDS bytes of a pass of width s with 2^logC-column tiles (the layout
 // of ntt_pass_body).  At most 2^(tile_log) + 2^(tile_log - 4) tile words plus
-// 2 (2^10 - 1) twiddles: below 152 KiB for both fields, inside the 160 KiB of
+// 2 (2^10 + 2^5) twiddles: below 153 KiB for both fields, inside the 160 KiB of
 // one workgroup; tiles above 80 KiB leave one workgroup per CU.
-template <class T>
+template <class F>
 uint32_t pass_lds_bytes(int mode, uint32_t s, uint32_t logC) {
-    const uint32_t r0 = s <= 5 ? s : (s + 1) / 2;
+    const uint32_t r0 = low_round_bits(s);
     const bool split = mode == OUTER_DIF_INV_SPLIT || mode == OUTER_DIT_FWD_SPLIT;
     const uint64_t tile = r0 < s ? (((1ull << s) + 1) << (split ? logC - 1 : logC)) : 0;
-    const uint64_t tws = s ? ((1ull << s) - 1) * (mode == INNER_LDE ? 2 : 1) : 0;
-    return uint32_t((tile + tws) * sizeof(T));
+    const uint64_t tws = uint64_t(tw_table_words(F::POW2_ROOTS, s)) * (mode == INNER_LDE ? 2 : 1);
+    return uint32_t((tile + tws) * sizeof(typename F::T));
 }
 
 // The kernel variant of a planned pass: an outer two-round pass whose tile is
@@ -821,7 +909,7 @@ void launch_pass(NttArgs<typename F::T> A, uint32_t s, const NttTuning& tune, hi
     using T = typename F::T;
     const uint64_t blocks = finalize_args(A, MODE, s, tune);
     const int mode = pass_mode<T>(MODE, s, A);
-    launch_any<F>(mode, s, A, blocks, pass_lds_bytes<T>(mode, s, A.logC), nullptr, 0, st);
+    launch_any<F>(mode, s, A, blocks, pass_lds_bytes<F>(mode, s, A.logC), nullptr, 0, st);
 }
 
 // All passes of one LDE over columns [0, w) of row-stride-ld matrices.
@@ -1058,7 +1146,7 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
                 for (size_t i = 0; i < passes.size(); ++i) {
                     if (passes[i].slot != slot || passes[i].s != s || kmode[i] != mode) continue;
                     NttWork<T> wk = geo[i];
-                    const uint32_t lds = pass_lds_bytes<T>(mode, s, wk.args.logC);
+                    const uint32_t lds = pass_lds_bytes<F>(mode, s, wk.args.logC);
                     g.lds_bytes = lds > g.lds_bytes ? lds : g.lds_bytes;
                     wk.block_start = g.blocks;
                     g.blocks += uint32_t(nblk[i]);
