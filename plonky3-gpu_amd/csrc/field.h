@@ -27,12 +27,31 @@ struct BabyBear {
     static constexpr bool POW2_ROOTS = false;
     static constexpr int LOG2_ROOT64_FWD = 0, LOG2_ROOT64_INV = 0;
 
-    // [0, 2p) -> [0, p): for x < p, x - p wraps above 2^32 - p > x.
-    __device__ __forceinline__ static uint32_t reduce2p(uint32_t x) { return min(x, x - P); }
+    // [0, 2p) -> [0, p): x - p, or x itself when that subtraction borrows.
+    // The borrow and the select go through VCC (v_sub_co_u32, v_cndmask_b32):
+    // on gfx950 that pair issues at about half the cycles of the
+    // v_sub_u32 + v_min_u32 form min(x, x - p), whose v_min_u32 issues at the
+    // rate of a multiply.
+    __device__ __forceinline__ static uint32_t reduce2p(uint32_t x) {
+        uint32_t t;
+        asm("v_sub_co_u32 %0, vcc, %1, %2\n\t"
+            "v_cndmask_b32 %0, %0, %1, vcc"
+            : "=&v"(t)
+            : "v"(x), "v"(P)
+            : "vcc");
+        return t;
+    }
     __device__ __forceinline__ static uint32_t add(uint32_t a, uint32_t b) { return reduce2p(a + b); }
+    // a - b, plus p when the subtraction borrows
     __device__ __forceinline__ static uint32_t sub(uint32_t a, uint32_t b) {
-        uint32_t d = a - b;
-        return min(d, d + P);
+        uint32_t d, e;
+        asm("v_sub_co_u32 %0, vcc, %2, %3\n\t"
+            "v_add_u32 %1, %0, %4\n\t"
+            "v_cndmask_b32 %0, %0, %1, vcc"
+            : "=&v"(d), "=&v"(e)
+            : "v"(a), "v"(b), "v"(P)
+            : "vcc");
+        return d;
     }
     // a * b * 2^-32 mod p in [0, 2p).  Valid when a * b < 2^32 * p, which holds
     // for a < 2p and b < p (t + m*p then stays below 2^64).
