@@ -323,7 +323,8 @@ def main():
                    "rows": n, "width": width, "log_blowup": lb, "lde_rows": N},
         "phases_ms": {"lde": lde_ms, "leaf_hash": hash_ms, "compress": comp_ms},
         "roofline": roofline,
-        "roofline_lde": {"bound": "hbm", "achieved": lde_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline_lde": {"bound": "valu" if lde_valu_floor_ms > lde_bytes / (HBM_PEAK_GBS * 1e9) * 1e3 else "hbm",
+                         "achieved": lde_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": lde_gbs / HBM_PEAK_GBS, "traffic": traffic.get("lde_phase"),
                          "algorithmic_bytes": lde_bytes, "avg_phase_ms": lde_ms,
                          "hbm_floor_ms": lde_bytes / (HBM_PEAK_GBS * 1e9) * 1e3,
@@ -332,8 +333,8 @@ def main():
                                   "ceiling_butterflies_per_s": bf_rate,
                                   "frac": lde_butterflies / (lde_ms / 1e3) / bf_rate,
                                   "floor_ms": lde_valu_floor_ms,
-                                  "ceiling_from": "p3h_probe_butterflies: the passes' own butterflies on registers, "
-                                                  "no memory traffic, 16384 x 256 threads",
+                                  "ceiling_from": "p3h_probe_butterflies: the passes' own radix-16 register rounds "
+                                                  "on registers, no memory traffic, 16384 x 256 threads",
                                   "probe_ms": bf_probe_ms},
                          "limiting_floor": "valu" if lde_valu_floor_ms > lde_bytes / (HBM_PEAK_GBS * 1e9) * 1e3
                          else "hbm"},
@@ -352,7 +353,8 @@ def main():
         xargs.steps, xargs.warmup = min(args.steps, 5), min(max(args.warmup, 1), 2)
         b = bench_batch(p3hip, xargs)
         out["batched"] = {k: b[k] for k in ("value", "ms_per_step", "plain_loop", "batched_over_plain", "config",
-                                             "cap_equals_oracle_fixture", "autotuned")}
+                                             "lde_launches_per_step", "queue_lde", "cap_equals_oracle_fixture",
+                                             "autotuned")}
         out["batched_over_plain"] = b["batched_over_plain"]
         sv = bench_serve(p3hip, xargs)
         out["serving"] = sv["serving"]
@@ -406,9 +408,11 @@ def calibrate_copy(nbytes: int = 1 << 30, reps: int = 3):
 
 def bench_batch(p3hip, args):
     """64 BabyBear traces (heights 2^12..2^20, widths 8..128, blowup 2 or 4): one
-    multi-height MMCS over 64 LDEs, planned and replayed as a
-    HIP graph, against the plain per-trace loop (LDE + single-matrix commit +
-    root per trace) on the same GPU and inputs."""
+    multi-height MMCS over 64 LDEs, planned and replayed as a HIP graph (LDE
+    passes as launches grouped by pass shape), against the plain per-trace
+    loop (LDE + single-matrix commit + root per trace) on the same GPU and
+    inputs; the same plan with the LDE as one work-queue launch is timed
+    beside it."""
     ctx = p3hip.Context(0)
     ctx.set_poseidon2(p3hip.BABYBEAR, p2_constants("bb", 13), 8, 13)
     ctx.set_hash_engine(args.hash_engine)
@@ -443,15 +447,22 @@ def bench_batch(p3hip, args):
     ctx.sync()
     plain_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
-    plan = ctx.batch_plan(mats, ks, 31, cap_height=cap_height, use_graph=True)
-    caps = [plan.run() for _ in range(max(1, args.warmup))]
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        caps.append(plan.run())
-    ctx.sync()
-    batch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    assert all((c == caps[0]).all() for c in caps), "non-deterministic batched root"
+    def timed_plan(lde_queue):
+        plan = ctx.batch_plan(mats, ks, 31, cap_height=cap_height, use_graph=True, lde_queue=lde_queue)
+        caps = [plan.run() for _ in range(max(1, args.warmup))]
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            caps.append(plan.run())
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        launches = plan.lde_launches()
+        plan.free()
+        return ms, caps, launches
+
+    queue_ms, caps_q, queue_launches = timed_plan(True)
+    batch_ms, caps, grouped_launches = timed_plan(False)
+    assert all((c == caps[0]).all() for c in caps + caps_q), "non-deterministic batched root"
     pinned_cap = fixture("batch64/cap")
     if pinned_cap is not None:
         assert (caps[0] == pinned_cap).all(), "batched cap differs from the oracle's full-size fixture"
@@ -467,11 +478,14 @@ def bench_batch(p3hip, args):
                                f"single-matrix commit + root", "committed_elems": elems},
         "plain_loop": {"value": elems / (plain_ms / 1e3), "ms_per_step": plain_ms},
         "batched_over_plain": plain_ms / batch_ms,
+        "lde_launches_per_step": grouped_launches,
+        "queue_lde": {"value": elems / (queue_ms / 1e3), "ms_per_step": queue_ms,
+                      "lde_launches_per_step": queue_launches,
+                      "note": "the LDE passes of all 64 traces as one work-queue launch"},
         "autotuned": {f"2^{h}x{w}_b{k}": v for (h, w, k), v in tuned.items()},
         "cap0": [int(x) for x in caps[0][0]],
         "cap_equals_oracle_fixture": None if pinned_cap is None else True,
     }
-    plan.free()
     for m in mats:
         m.free()
     ctx.close()

@@ -314,14 +314,24 @@ int p3h_batch_commit(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t*
                      uint32_t n_traces, uint64_t shift, uint32_t cap_height, p3h_tree** out);
 
 /* Planned batched commit: every buffer is allocated and every table uploaded
- * once; the LDE passes of all traces run as work-list launches grouped by pass
- * shape, and with use_graph != 0 the whole commit (LDE passes, leaf hashing,
- * tree levels with injection) is captured as one HIP graph and replayed by
- * p3h_batch_run on the traces' current contents.  No counterpart in the
- * reference.  cap_out (nullable): 2^cap_height digests. */
+ * once.  The LDE passes of all traces run as work-list launches grouped by
+ * pass shape; with P3H_BATCH_QUEUE_LDE they run instead as one launch whose
+ * workgroups take (trace, pass, column tile, row-block chunk) items from a
+ * device work list in dependency order (when every pass shape is carried by
+ * that kernel; else the grouped launches run).  With P3H_BATCH_GRAPH the whole
+ * commit (LDE passes, leaf hashing, tree levels with injection) is captured
+ * as one HIP graph and replayed by p3h_batch_run on the traces' current
+ * contents.  No counterpart in the reference.  flags: a bitwise or of the
+ * P3H_BATCH_* values (the earlier use_graph argument: 1 = graph).  cap_out
+ * (nullable): 2^cap_height digests. */
+#define P3H_BATCH_GRAPH 1
+#define P3H_BATCH_QUEUE_LDE 2
 int p3h_batch_plan(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n_traces,
-                   uint64_t shift, uint32_t cap_height, int use_graph, p3h_batch** out);
+                   uint64_t shift, uint32_t cap_height, int flags, p3h_batch** out);
 int p3h_batch_run(p3h_batch* b, void* cap_out);
+/* kernel launches of the LDE phase per run: 1 for the queue launch, else the
+ * number of pass-shape groups */
+int p3h_batch_lde_launches(const p3h_batch* b, uint32_t* launches);
 /* tree of the last run (borrowed; overwritten by the next run) */
 int p3h_batch_tree(const p3h_batch* b, const p3h_tree** out);
 int p3h_batch_free(p3h_batch* b);
@@ -362,11 +372,14 @@ int p3h_serve_destroy(p3h_serve* sv);
 
 /* ----------------------------------------------------------- measurement */
 /* The VALU ceiling of the NTT arithmetic: `blocks` x 256 threads each take 16
- * words through iters x 32 of the NTT passes' own radix-2 butterflies (a
- * modular product, sum and difference) in registers, with no memory traffic
- * in the timed loop.  *ms: the kernel time (HIP events on the context's
- * stream); *butterflies: the butterflies it ran.  No counterpart in the
- * reference; the benchmark reports the rate as the passes' VALU roofline. */
+ * words through iters radix-16 rounds of the NTT passes' own register-round
+ * code (BabyBear: 32 butterflies of a modular product, sum and difference;
+ * Goldilocks: the twisted round, 15 products then 32 power-of-two
+ * butterflies), twiddles in registers, with no memory traffic in the timed
+ * loop.  *ms: the kernel time (HIP events on the context's stream);
+ * *butterflies: the radix-2 butterflies those rounds stand for (32 per round).
+ * No counterpart in the reference; the benchmark reports the rate as the
+ * passes' VALU roofline. */
 int p3h_probe_butterflies(p3h_ctx* ctx, int field, uint32_t blocks, uint32_t iters, double* ms,
                           double* butterflies);
 

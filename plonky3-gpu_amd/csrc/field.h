@@ -68,7 +68,7 @@ struct BabyBear {
     __device__ __forceinline__ static uint32_t to_mont(uint32_t x) { return mont(x, R2_MOD_P); }
     __device__ __forceinline__ static uint32_t from_mont(uint32_t x) { return mont(x, 1u); }
 
-    // NTT butterflies (ntt.hip) on canonical words; w in multiply-constant form.
+    // NTT butterflies (ntt_impl.h) on canonical words; w in multiply-constant form.
     // dit: (a, b) -> (a + w b, a - w b); dif: (a, b) -> (a + b, (a - b) w);
     // the *1 forms are the same with w = 1.
     __device__ __forceinline__ static void dit(uint32_t& a, uint32_t& b, uint32_t w) {
@@ -372,7 +372,7 @@ struct Goldilocks {
         }
     }
 
-    // NTT butterflies (ntt.hip) on lazy words, results lazy; w canonical.
+    // NTT butterflies (ntt_impl.h) on lazy words, results lazy; w canonical.
     // The operand added to or subtracted from a is made canonical first, as
     // add_lz / sub_lz require.  Words are made canonical when stored to global
     // memory (store_form).

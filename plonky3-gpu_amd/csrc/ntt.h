@@ -1,4 +1,5 @@
-// Host-side launchers for the NTT / LDE kernels (ntt.hip).
+// Host-side launchers for the NTT / LDE kernels (ntt_impl.h; built per field by
+// ntt_bb.hip and ntt_gl.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,7 +29,7 @@ inline const T* twc_of(const T* tw_inv_fwd, uint32_t L, bool forward) {
 }
 inline uint64_t twiddle_buffer_len(uint32_t L) { return (L ? (uint64_t(1) << L) : 0) + 2 * TWC_LEN; }
 
-// Arguments of one NTT pass over a row-major matrix (see ntt.hip).
+// Arguments of one NTT pass over a row-major matrix (see ntt_impl.h).
 template <class T>
 struct NttArgs {
     const T* src;
@@ -80,9 +81,59 @@ struct LdePassGroup {
 template <class F>
 std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob<typename F::T>>& jobs);
 
+// One pass of one LDE in the queue launch (launch_lde_queue): its blocks are
+// taken `chunk` at a time, one ticket per chunk, tickets [first, first +
+// ceil(blocks / chunk)); it reads what item `dep` (the same LDE's previous
+// pass, -1 for none) writes.
+template <class T>
+struct NttQueueItem {
+    NttArgs<T> args;
+    uint32_t first;
+    uint32_t blocks;
+    uint32_t chunk;
+    int32_t dep;
+    uint16_t mode;  // kernel variant (after the split-form choice)
+    uint16_t s;     // pass width in index bits
+};
+
+// Every pass of many LDEs as one work list, in ticket order: slot by slot
+// (the inverse outer passes farthest first, the fused inner pass, then the
+// forward outer passes), within a slot LDE by LDE, so every dependency has
+// lower tickets than its dependents.
+template <class T>
+struct LdeQueuePlan {
+    std::vector<NttQueueItem<T>> items;  // ascending `first`
+    uint32_t total = 0;                  // tickets (chunks of every pass; the launch's workgroups)
+    uint32_t lds_bytes = 0;              // dynamic LDS: the largest pass's
+};
+
+// Plan the queue launch of many LDEs.  Returns false when a pass has a shape
+// the queue kernel does not carry (a fused inner pass of 0, 4 or 5 stages: a
+// transform of 1, 16 or 32 rows, or a tuning capped at 4 or 5 stages), when
+// a pass's LDS tile would leave one workgroup per CU (a full tile that is not
+// split, as on regions of 4 GiB and more), or when the blocks exceed one
+// dispatch (2^22 workgroups); the grouped launches then run the batch.
+template <class F>
+bool plan_lde_queue(const std::vector<LdeJob<typename F::T>>& jobs, LdeQueuePlan<typename F::T>& plan);
+
+// Words of the queue launch's counter block: the ticket counter, the error
+// word (set when a wait exceeded its spin limit) and one completion count per
+// item.  launch_lde_queue zeroes them with a kernel of its own first, so the
+// pair can be captured into a HIP graph and replayed.
+inline size_t lde_queue_counter_words(size_t items) { return 2 + items; }
+
+// The queue launch: plan.total workgroups each take a ticket in order from the
+// counter block, wait for the pass their ticket's pass reads, run the chunk's
+// blocks and count them done.  d_items holds plan.items in device memory.
+template <class F>
+int launch_lde_queue(const LdeQueuePlan<typename F::T>& plan, const NttQueueItem<typename F::T>* d_items,
+                     uint32_t* d_counters, hipStream_t st);
+
 // Once per device, before the first pass launch (the context does it at
 // creation): lets every pass kernel take up to 160 KiB of dynamic LDS.
 int ntt_prepare_device();
+int ntt_prepare_gl();        // the Goldilocks passes (ntt_gl.hip)
+int ntt_prepare_queue_bb();  // the BabyBear queue kernel (ntt_queue.hip)
 
 // Launch one group; d_work holds g.work in device memory.
 template <class F>

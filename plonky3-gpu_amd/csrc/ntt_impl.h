@@ -1,4 +1,6 @@
-// Column-batched NTT / coset LDE kernels for gfx950.
+// Column-batched NTT / coset LDE kernels for gfx950: the kernels and the
+// host-side planners and launchers, included by one translation unit per field
+// (ntt_bb.hip, ntt_gl.hip), so each field's kernels compile on their own.
 //
 // The matrix is row-major (height x width, the reference's RowMajorMatrix),
 // so every column is an independent transform whose elements sit `width`
@@ -28,6 +30,8 @@
 // the cosets one after another ("coset-major": stored row r * n + i); readers
 // (leaf hashing, openings, downloads) map LDE row j to stored row
 // (j mod 2^k) * n + j / 2^k.  Zero padding is never materialised.
+
+#pragma once
 
 #include <utility>
 
@@ -1029,10 +1033,9 @@ int dft_device(int kind, const typename F::T* in, typename F::T* out, typename F
     return int(hipGetLastError());
 }
 
-int ntt_prepare_device() {
-    hipError_t e = allow_lds_field<BabyBear>();
-    if (e == hipSuccess) e = allow_lds_field<Goldilocks>();
-    return int(e);
+template <class F>
+int ntt_prepare_field() {
+    return int(allow_lds_field<F>());
 }
 
 template <class F>
@@ -1063,18 +1066,25 @@ int coset_to_natural_device(const T* in, T* out, uint32_t L, uint32_t k, uint32_
 }
 
 
+// Every pass of every LDE of a batch, with its slot: the inverse outer passes
+// at negative slots (farthest first), the fused inner pass at 0, the forward
+// outer passes at positive slots.  Passes of one LDE run in slot order.
+template <class T>
+struct BatchPass {
+    int mode;
+    uint32_t s;
+    int slot;
+    uint32_t job;
+    NttArgs<T> A;
+    NttTuning tune;
+};
+
 template <class F>
-std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob<typename F::T>>& jobs) {
+std::vector<BatchPass<typename F::T>> batch_passes(const std::vector<LdeJob<typename F::T>>& jobs) {
     using T = typename F::T;
-    struct Pass {
-        int mode;
-        uint32_t s;
-        int slot;  // order: DIF at negative slots (farthest first), inner 0, DIT positive
-        NttArgs<T> A;
-        NttTuning tune;
-    };
-    std::vector<Pass> passes;
-    for (const auto& j : jobs) {
+    std::vector<BatchPass<T>> passes;
+    for (uint32_t ji = 0; ji < jobs.size(); ++ji) {
+        const auto& j = jobs[ji];
         Plan p = make_plan<F>(j.L, j.tune);
         const uint64_t n = 1ull << j.L;
         NttArgs<T> A{};
@@ -1090,7 +1100,7 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
             A.a = top - p.outer[i];
             A.src = cur;
             A.dst = j.out;
-            passes.push_back({OUTER_DIF_INV, p.outer[i], -int(p.n_outer - i), A, j.tune});
+            passes.push_back({OUTER_DIF_INV, p.outer[i], -int(p.n_outer - i), ji, A, j.tune});
             cur = j.out;
             top = A.a;
         }
@@ -1100,7 +1110,7 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
         A.scale = j.twist;
         A.n_regions = 1u << j.k;
         A.dst_region_rows = n;
-        passes.push_back({INNER_LDE, p.s_inner, 0, A, j.tune});
+        passes.push_back({INNER_LDE, p.s_inner, 0, ji, A, j.tune});
         A.scale = nullptr;
         A.tw = j.tw + (n >> 1);
         A.twc = twc_of(j.tw, j.L, true);
@@ -1110,10 +1120,17 @@ std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob
         uint32_t bot = p.s_inner;
         for (int i = int(p.n_outer) - 1; i >= 0; --i) {
             A.a = bot;
-            passes.push_back({OUTER_DIT_FWD, p.outer[i], int(p.n_outer) - i, A, j.tune});
+            passes.push_back({OUTER_DIT_FWD, p.outer[i], int(p.n_outer) - i, ji, A, j.tune});
             bot += p.outer[i];
         }
     }
+    return passes;
+}
+
+template <class F>
+std::vector<LdePassGroup<typename F::T>> plan_lde_batch(const std::vector<LdeJob<typename F::T>>& jobs) {
+    using T = typename F::T;
+    const std::vector<BatchPass<T>> passes = batch_passes<F>(jobs);
     std::vector<LdePassGroup<T>> groups;
     int lo = 0, hi = 0;
     for (const auto& ps : passes) {
@@ -1164,37 +1181,229 @@ int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typenam
     return int(hipGetLastError());
 }
 
+// ---- the queue launch of a batch's LDE passes ----
+//
+// One launch runs every pass of every LDE of a batch.  A pass's blocks are
+// cut into chunks of consecutive blocks (one block for passes of at most 512,
+// up to 8 for large ones, so the per-ticket costs below are shared), and the grid has
+// one workgroup per chunk; each workgroup takes a ticket from a counter, and
+// ticket t is chunk t - first of the pass (item) whose range holds it.  Before
+// its chunk the workgroup waits until every block of the pass that item reads
+// (the same LDE's previous pass) is counted done, and after it counts its own
+// blocks done.  Tickets are handed out in increasing order to
+// running workgroups and an item's dependency has lower tickets than the
+// item, so every awaited block was taken by a workgroup that is already
+// running and waits, if at all, only for lower tickets: the launch completes
+// whatever the number of resident workgroups, and the hardware dispatcher
+// refills a CU as soon as one of its workgroups ends.  Results pass between
+// workgroups on different XCDs (each with its own L2), so a block's stores are
+// released at agent scope (written back past the XCD's L2, once per block)
+// before its completion count, and a waiting workgroup acquires at agent
+// scope (once, after its wait) before reading.
+//
+// Every branch at the level of the queue is uniform across a wave (the ticket
+// and the item are scalar, the wave index is read through readfirstlane):
+// a region taken by one lane of a wave inside code that holds barriers lets
+// the compiler split the wave's lanes between loop nests, which livelocks
+// them.  A workgroup runs one block, so no loop holds the pass bodies.
+//
+// The kernel carries the pass variants of batches above 2^5 rows: outer
+// passes of 1..10 stages, fused inner passes of 1..3 and 6..10, split outer
+// passes of 6..10 (the split form exists only above 5 stages).  It is held to
+// 128 VGPRs (two 512-thread workgroups per CU); the single-round fused passes
+// of 4 and 5 stages need more (they keep 16 or 32 coefficients and a coset's
+// values per lane) and are left to the grouped launches.
+
+constexpr uint32_t QUEUE_SPIN_LIMIT = 1u << 25;  // sleeps of 256 cycles: seconds, far above any pass
+
+__host__ __device__ constexpr bool queue_carries(int mode, uint32_t s) {
+    return (mode == OUTER_DIF_INV || mode == OUTER_DIT_FWD) ? (s >= 1 && s <= 10)
+           : mode == INNER_LDE ? ((s >= 1 && s <= 3) || (s >= 6 && s <= 10))
+           : (mode == OUTER_DIF_INV_SPLIT || mode == OUTER_DIT_FWD_SPLIT) ? (s >= 6 && s <= 10)
+                                                                          : false;
+}
+
+namespace {
+
+template <class F, int MODE, int... S>
+__device__ __forceinline__ void queue_pass_s(uint32_t s, const NttArgs<typename F::T>& A, uint32_t block,
+                                             typename F::T* smem, std::integer_sequence<int, S...>) {
+    ((s == uint32_t(S) ? (ntt_pass_body<F, MODE, S>(A, block, smem), 0) : 0), ...);
+}
+
+template <class F>
+__device__ __forceinline__ void queue_pass(uint32_t mode, uint32_t s, const NttArgs<typename F::T>& A,
+                                           uint32_t block, typename F::T* smem) {
+    using Outer = std::integer_sequence<int, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10>;
+    using Inner = std::integer_sequence<int, 1, 2, 3, 6, 7, 8, 9, 10>;
+    using Wide = std::integer_sequence<int, 6, 7, 8, 9, 10>;
+    switch (mode) {
+        case OUTER_DIF_INV: queue_pass_s<F, OUTER_DIF_INV>(s, A, block, smem, Outer{}); break;
+        case OUTER_DIT_FWD: queue_pass_s<F, OUTER_DIT_FWD>(s, A, block, smem, Outer{}); break;
+        case INNER_LDE: queue_pass_s<F, INNER_LDE>(s, A, block, smem, Inner{}); break;
+        case OUTER_DIF_INV_SPLIT: queue_pass_s<F, OUTER_DIF_INV_SPLIT>(s, A, block, smem, Wide{}); break;
+        case OUTER_DIT_FWD_SPLIT: queue_pass_s<F, OUTER_DIT_FWD_SPLIT>(s, A, block, smem, Wide{}); break;
+        default: break;
+    }
+}
+
+// ctr: [0] ticket counter, [1] error word, [2 + i] blocks of item i done.
+template <class F>
+__global__ __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+void ntt_lde_queue(const NttQueueItem<typename F::T>* __restrict__ items, uint32_t n_items, uint32_t total,
+                   uint32_t* ctr) {
+    using T = typename F::T;
+    extern __shared__ __align__(16) unsigned char ntt_smem[];
+    __shared__ uint32_t s_ticket;
+    uint32_t* const done = ctr + 2;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (wave == 0) {
+        // one increment per workgroup: lane 0 adds 1, the other lanes 0, and
+        // lane 0's result is the value before its increment
+        const uint32_t v = __hip_atomic_fetch_add(ctr, lane == 0 ? 1u : 0u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        s_ticket = __builtin_amdgcn_readfirstlane(v);
+    }
+    __syncthreads();
+    const uint32_t t = __builtin_amdgcn_readfirstlane(s_ticket);
+    if (t >= total) return;  // the grid has exactly `total` workgroups
+    uint32_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (items[mid].first <= t) lo = mid;
+        else hi = mid;
+    }
+    const int32_t dep = items[lo].dep;
+    if (dep >= 0) {
+        // wave 0 polls (relaxed: no cache invalidation per poll), then one
+        // acquire invalidates this CU's L1 and the XCD's L2 for every wave of
+        // the workgroup, which read only after the barrier
+        if (wave == 0) {
+            const uint32_t need = items[dep].blocks;
+            uint32_t spins = 0;
+            while (__builtin_amdgcn_readfirstlane(
+                       __hip_atomic_load(done + dep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+                __builtin_amdgcn_s_sleep(4);
+                if (++spins == QUEUE_SPIN_LIMIT) {
+                    __hip_atomic_store(ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+    }
+    const NttArgs<T> A = items[lo].args;
+    const uint32_t mode = items[lo].mode, s = items[lo].s, chunk = items[lo].chunk, blocks = items[lo].blocks;
+    const uint32_t b0 = (t - items[lo].first) * chunk;
+    const uint32_t b1 = b0 + chunk < blocks ? b0 + chunk : blocks;
+    for (uint32_t b = b0; b < b1; ++b) {
+        if (b != b0) __syncthreads();  // the previous block is done with the LDS tile
+        queue_pass<F>(mode, s, A, b, reinterpret_cast<T*>(ntt_smem));
+    }
+    // every wave's stores have reached the XCD's L2 (vmcnt 0) before the
+    // barrier; then one release by wave 0 writes the L2 back for all of them
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (wave == 0)
+        __hip_atomic_fetch_add(done + lo, lane == 0 ? b1 - b0 : 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class U>
+__global__ void fill_kernel(U* p, uint32_t n, U v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+template <class F>
+bool plan_lde_queue(const std::vector<LdeJob<typename F::T>>& jobs, LdeQueuePlan<typename F::T>& plan) {
+    using T = typename F::T;
+    const std::vector<BatchPass<T>> passes = batch_passes<F>(jobs);
+    int lo = 0, hi = 0;
+    for (const auto& ps : passes) {
+        lo = ps.slot < lo ? ps.slot : lo;
+        hi = ps.slot > hi ? ps.slot : hi;
+    }
+    plan = LdeQueuePlan<T>{};
+    std::vector<int32_t> last(jobs.size(), -1);  // each LDE's latest planned pass
+    uint64_t total = 0;
+    for (int slot = lo; slot <= hi; ++slot)
+        for (const auto& ps : passes) {
+            if (ps.slot != slot) continue;
+            NttQueueItem<T> it{};
+            it.args = ps.A;
+            const uint64_t blocks = finalize_args(it.args, ps.mode, ps.s, ps.tune);
+            const int mode = pass_mode<T>(ps.mode, ps.s, it.args);
+            if (!queue_carries(mode, ps.s)) return false;
+            it.first = uint32_t(total);
+            it.blocks = uint32_t(blocks);
+            // chunks of consecutive blocks: at least 512 chunks per pass (two
+            // per resident workgroup) before any chunk holds two blocks
+            it.chunk = uint32_t(blocks >= 512 * 8 ? 8 : (blocks + 511) / 512);
+            it.dep = last[ps.job];
+            it.mode = uint16_t(mode);
+            it.s = uint16_t(ps.s);
+            total += (blocks + it.chunk - 1) / it.chunk;
+            if (total > MAX_GRID) return false;  // one workgroup per chunk in one dispatch
+            const uint32_t lds = pass_lds_bytes<F>(mode, ps.s, it.args.logC);
+            // one launch takes the largest pass's LDS for every workgroup: past half
+            // a CU's LDS that would hold every pass to one workgroup per CU
+            if (lds > uint32_t(MAX_DYN_LDS) / 2 - 1024) return false;
+            plan.lds_bytes = lds > plan.lds_bytes ? lds : plan.lds_bytes;
+            last[ps.job] = int32_t(plan.items.size());
+            plan.items.push_back(it);
+        }
+    plan.total = uint32_t(total);
+    return true;
+}
+
+template <class F>
+int launch_lde_queue(const LdeQueuePlan<typename F::T>& plan, const NttQueueItem<typename F::T>* d_items,
+                     uint32_t* d_counters, hipStream_t st) {
+    const uint32_t words = uint32_t(lde_queue_counter_words(plan.items.size()));
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3((words + 255) / 256), dim3(256), 0, st, d_counters, words, 0u);
+    if (plan.total == 0) return int(hipGetLastError());
+    hipLaunchKernelGGL((ntt_lde_queue<F>), dim3(plan.total), dim3(NTT_THREADS), plan.lds_bytes, st, d_items,
+                       uint32_t(plan.items.size()), plan.total, d_counters);
+    return int(hipGetLastError());
+}
+
+// The queue kernel's dynamic LDS: the planner keeps it below half a CU's (the
+// kernel also has a few bytes of static LDS, the ticket it shares).
+template <class F>
+int ntt_prepare_queue() {
+    return int(hipFuncSetAttribute(reinterpret_cast<const void*>(&ntt_lde_queue<F>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, MAX_DYN_LDS / 2));
+}
+
 namespace {
 
 // VALU ceiling of the NTT arithmetic: each thread takes 16 words through the
-// passes' own DIT butterfly (F::dit) in radix-2^4 register rounds -- 4 stages
-// of 8 butterflies, twiddles held in registers -- `iters` times, with no
-// memory traffic inside the loop; the words are xor-folded into one store so
-// the work cannot be dropped.
+// passes' own register-round code -- butterflies<F, 4, 1, DIT>, the round of a
+// pass's high bits: 4 stages of 8 butterflies on the stage twiddles for
+// BabyBear, the twisted round (15 scalings, then 32 power-of-two butterflies)
+// for Goldilocks -- `iters` times, twiddles held in registers (32 per lane,
+// lane-dependent like the rows of the LDS table a pass reads), with no memory
+// traffic inside the loop; the words are xor-folded into one store so the work
+// cannot be dropped.  Its rate is counted in radix-2 butterflies (32 per round).
 template <class F>
 __global__ __launch_bounds__(256) void butterfly_probe_kernel(typename F::T* out, const typename F::T* tw,
                                                               uint32_t iters) {
     using T = typename F::T;
     constexpr int R = 4, M = 1 << R;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    T x[M], w[M];
+    T x[M], w[2 * M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-        x[m] = T(tid * 31u + uint32_t(m)) % T(F::P);
-        w[m] = tw[m];
-    }
-    for (uint32_t it = 0; it < iters; ++it) {
+    for (int m = 0; m < M; ++m) x[m] = T(tid * 31u + uint32_t(m)) % T(F::P);
 #pragma unroll
-        for (int j = 0; j < R; ++j)
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                if (m & (1 << j)) continue;
-                F::dit(x[m], x[m | (1 << j)], w[(m + j) & (M - 1)]);
-            }
-    }
+    for (int i = 0; i < 2 * M; ++i) w[i] = tw[(uint32_t(i) + threadIdx.x) & (2 * M - 1)];
+    for (uint32_t it = 0; it < iters; ++it) butterflies<F, R, 1, true>(x, w);
     T acc = 0;
 #pragma unroll
-    for (int m = 0; m < M; ++m) acc ^= x[m];
+    for (int m = 0; m < M; ++m) acc ^= ntt_store_form(x[m]);
     out[tid] = acc;
 }
 
@@ -1206,31 +1415,5 @@ int butterfly_probe_device(typename F::T* out, const typename F::T* tw, uint32_t
     hipLaunchKernelGGL((butterfly_probe_kernel<F>), dim3(blocks), dim3(256), 0, st, out, tw, iters);
     return int(hipGetLastError());
 }
-template int butterfly_probe_device<BabyBear>(uint32_t*, const uint32_t*, uint32_t, uint32_t, hipStream_t);
-template int butterfly_probe_device<Goldilocks>(uint64_t*, const uint64_t*, uint32_t, uint32_t, hipStream_t);
-
-template std::vector<LdePassGroup<uint32_t>> plan_lde_batch<BabyBear>(const std::vector<LdeJob<uint32_t>>&);
-template std::vector<LdePassGroup<uint64_t>> plan_lde_batch<Goldilocks>(const std::vector<LdeJob<uint64_t>>&);
-template int launch_lde_group<BabyBear>(const LdePassGroup<uint32_t>&, const NttWork<uint32_t>*, hipStream_t);
-template int launch_lde_group<Goldilocks>(const LdePassGroup<uint64_t>&, const NttWork<uint64_t>*, hipStream_t);
-
-template int lde_device<BabyBear>(const uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t, const uint32_t*,
-                                  const uint32_t*, const NttTuning&, hipStream_t);
-template int lde_device<Goldilocks>(const uint64_t*, uint64_t*, uint32_t, uint32_t, uint32_t, const uint64_t*,
-                                    const uint64_t*, const NttTuning&, hipStream_t);
-template int dft_device<BabyBear>(int, const uint32_t*, uint32_t*, uint32_t*, uint32_t, uint32_t,
-                                  const uint32_t*, const uint32_t*, const NttTuning&, hipStream_t);
-template int dft_device<Goldilocks>(int, const uint64_t*, uint64_t*, uint64_t*, uint32_t, uint32_t,
-                                    const uint64_t*, const uint64_t*, const NttTuning&, hipStream_t);
-template int powers_device<BabyBear>(uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t, hipStream_t);
-template int powers_device<Goldilocks>(uint64_t*, uint64_t, uint64_t, uint64_t, uint64_t, hipStream_t);
-template int lde_twist_device<BabyBear>(uint32_t*, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-                                        hipStream_t);
-template int lde_twist_device<Goldilocks>(uint64_t*, uint32_t, uint32_t, const uint64_t*, uint64_t, uint64_t,
-                                          hipStream_t);
-template int coset_to_natural_device<uint32_t>(const uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t,
-                                               hipStream_t);
-template int coset_to_natural_device<uint64_t>(const uint64_t*, uint64_t*, uint32_t, uint32_t, uint32_t,
-                                               hipStream_t);
 
 }  // namespace p3h

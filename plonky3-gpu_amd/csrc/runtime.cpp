@@ -1124,27 +1124,47 @@ p3h_mat* view_of(p3h_mat* base, uint64_t h, uint32_t w) {
 }
 
 // Batched commit of many BabyBear traces: every buffer allocated and every
-// table uploaded at planning time; the LDE passes run as work-list launches
-// grouped by (mode, pass width) and the tree as leaf hashing + one launch per
-// level.  With use_graph the whole sequence is captured once as a HIP graph.
+// table uploaded at planning time; the LDE passes of all traces run as
+// work-list launches grouped by (mode, pass width) or, with
+// P3H_BATCH_QUEUE_LDE, as one work-queue launch, and the tree as leaf hashing
+// + one launch per level.  With P3H_BATCH_GRAPH the whole sequence is
+// captured once as a HIP graph.
 struct BatchPlan {
     p3h_ctx* ctx = nullptr;
     std::vector<p3h_mat*> traces;  // referenced
     std::vector<p3h_mat*> ldes;    // referenced (the tree holds its own)
     std::vector<LdePassGroup<uint32_t>> groups;
     std::vector<DevPtr> work_tables;
+    bool queue = false;
+    LdeQueuePlan<uint32_t> qplan;
+    DevPtr q_items, q_ctr;
     p3h_tree* tree = nullptr;
     LaunchList tree_launches;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
     void launch_all() {
-        for (size_t g = 0; g < groups.size(); ++g) {
+        if (queue) {
             Phase ph(ctx, "lde");
-            launch_check(launch_lde_group<BabyBear>(groups[g], dptr<NttWork<uint32_t>>(work_tables[g]), ctx->stream),
-                         "batched lde");
+            launch_check(launch_lde_queue<BabyBear>(qplan, dptr<NttQueueItem<uint32_t>>(q_items),
+                                                    dptr<uint32_t>(q_ctr), ctx->stream),
+                         "batched lde queue");
+        } else {
+            for (size_t g = 0; g < groups.size(); ++g) {
+                Phase ph(ctx, "lde");
+                launch_check(
+                    launch_lde_group<BabyBear>(groups[g], dptr<NttWork<uint32_t>>(work_tables[g]), ctx->stream),
+                    "batched lde");
+            }
         }
         run_launches(ctx, tree_launches);
+    }
+    // after a run: the queue's error word (a wait that exceeded its spin limit)
+    void check_queue() {
+        if (!queue) return;
+        uint32_t err = 0;
+        d2h(ctx, &err, dptr<uint32_t>(q_ctr) + 1, sizeof(err));
+        if (err) throw Err{P3H_EDEVICE, "batched LDE queue: a pass waited beyond its spin limit"};
     }
     ~BatchPlan() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1156,7 +1176,8 @@ struct BatchPlan {
 };
 
 BatchPlan* make_batch_plan(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n,
-                           uint64_t shift, uint32_t cap_height, bool use_graph) {
+                           uint64_t shift, uint32_t cap_height, int flags) {
+    const bool use_graph = (flags & P3H_BATCH_GRAPH) != 0;
     if (!traces || !added_bits || n == 0) throw Err{P3H_EINVAL, "null argument"};
     if (shift >= BB_P) throw Err{P3H_EINVAL, "shift is not a canonical field element"};
     auto* bp = new BatchPlan{};
@@ -1180,7 +1201,15 @@ BatchPlan* make_batch_plan(p3h_ctx* ctx, const p3h_mat* const* traces, const uin
                                             twiddles<HBB>(ctx, L), lde_twist<HBB>(ctx, L, k, uint32_t(shift)),
                                             tuning_for(ctx, P3H_BABYBEAR, L, tr->width, k)});
         }
-        bp->groups = plan_lde_batch<BabyBear>(jobs);
+        if ((flags & P3H_BATCH_QUEUE_LDE) && plan_lde_queue<BabyBear>(jobs, bp->qplan)) {
+            bp->queue = true;
+            const size_t ib = bp->qplan.items.size() * sizeof(NttQueueItem<uint32_t>);
+            bp->q_items = dalloc(ctx, ib ? ib : 1);
+            if (ib) h2d(ctx, bp->q_items->ptr, bp->qplan.items.data(), ib);
+            bp->q_ctr = dalloc(ctx, lde_queue_counter_words(bp->qplan.items.size()) * sizeof(uint32_t));
+        } else {
+            bp->groups = plan_lde_batch<BabyBear>(jobs);
+        }
         for (const auto& g : bp->groups) {
             DevPtr d = dalloc(ctx, g.work.size() * sizeof(NttWork<uint32_t>));
             h2d(ctx, d->ptr, g.work.data(), g.work.size() * sizeof(NttWork<uint32_t>));
@@ -1632,12 +1661,12 @@ int p3h_probe_butterflies(p3h_ctx* ctx, int field, uint32_t blocks, uint32_t ite
         if (!ms || !butterflies || blocks == 0 || iters == 0) throw Err{P3H_EINVAL, "bad probe arguments"};
         if (field != P3H_BABYBEAR && field != P3H_GOLDILOCKS) throw Err{P3H_EINVAL, "unknown field"};
         const size_t eb = field == P3H_BABYBEAR ? 4 : 8;
-        DevPtr out = dalloc(ctx, size_t(blocks) * 256 * eb), tw = dalloc(ctx, 16 * eb);
-        uint64_t host[16];
-        for (int i = 0; i < 16; ++i) host[i] = 1000003ull * uint64_t(i + 1) + 17;
+        DevPtr out = dalloc(ctx, size_t(blocks) * 256 * eb), tw = dalloc(ctx, 32 * eb);
+        uint64_t host[32];
+        for (int i = 0; i < 32; ++i) host[i] = 1000003ull * uint64_t(i + 1) + 17;
         if (field == P3H_BABYBEAR) {
-            uint32_t h32[16];
-            for (int i = 0; i < 16; ++i) h32[i] = bb_mont(uint32_t(host[i] % BB_P));
+            uint32_t h32[32];
+            for (int i = 0; i < 32; ++i) h32[i] = bb_mont(uint32_t(host[i] % BB_P));
             h2d(ctx, tw->ptr, h32, sizeof(h32));
         } else {
             h2d(ctx, tw->ptr, host, sizeof(host));
@@ -2449,9 +2478,10 @@ int p3h_batch_commit(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t*
                      uint64_t shift, uint32_t cap_height, p3h_tree** out) {
     return api(ctx, [&] {
         if (!out) throw Err{P3H_EINVAL, "null argument"};
-        BatchPlan* bp = make_batch_plan(ctx, traces, added_bits, n, shift, cap_height, false);
+        BatchPlan* bp = make_batch_plan(ctx, traces, added_bits, n, shift, cap_height, 0);
         try {
             run_batch_plan(bp);
+            bp->check_queue();
         } catch (...) {
             delete bp;
             throw;
@@ -2467,12 +2497,19 @@ struct p3h_batch {
 };
 
 int p3h_batch_plan(p3h_ctx* ctx, const p3h_mat* const* traces, const uint32_t* added_bits, uint32_t n, uint64_t shift,
-                   uint32_t cap_height, int use_graph, p3h_batch** out) {
+                   uint32_t cap_height, int flags, p3h_batch** out) {
     return api(ctx, [&] {
         if (!out) throw Err{P3H_EINVAL, "null argument"};
-        BatchPlan* bp = make_batch_plan(ctx, traces, added_bits, n, shift, cap_height, use_graph != 0);
+        if (flags & ~(P3H_BATCH_GRAPH | P3H_BATCH_QUEUE_LDE)) throw Err{P3H_EINVAL, "unknown batch flags"};
+        BatchPlan* bp = make_batch_plan(ctx, traces, added_bits, n, shift, cap_height, flags);
         *out = new p3h_batch{bp};
     });
+}
+
+int p3h_batch_lde_launches(const p3h_batch* b, uint32_t* launches) {
+    if (!b || !launches) return P3H_EINVAL;
+    *launches = b->plan->queue ? 1u : uint32_t(b->plan->groups.size());
+    return P3H_OK;
 }
 
 int p3h_batch_run(p3h_batch* b, void* cap_out) {
@@ -2480,6 +2517,7 @@ int p3h_batch_run(p3h_batch* b, void* cap_out) {
     p3h_ctx* ctx = b->plan->ctx;
     return api(ctx, [&] {
         run_batch_plan(b->plan);
+        b->plan->check_queue();
         if (cap_out) d2h(ctx, cap_out, b->plan->tree->layers.back()->ptr, b->plan->tree->counts.back() * 32);
     });
 }

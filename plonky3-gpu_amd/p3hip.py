@@ -31,6 +31,9 @@ import numpy as np
 
 BABYBEAR = 0
 GOLDILOCKS = 1
+# p3h_batch_plan flags
+BATCH_GRAPH = 1
+BATCH_QUEUE_LDE = 2
 BB_P = 0x78000001
 GL_P = 0xFFFFFFFF00000001
 DTYPE = {BABYBEAR: np.uint32, GOLDILOCKS: np.uint64}
@@ -114,6 +117,7 @@ _SIGS = {
     "p3h_batch_plan": (_i32, [_vp, C.POINTER(_vp), _vp, _u32, _u64, _u32, _i32, C.POINTER(_vp)]),
     "p3h_batch_run": (_i32, [_vp, _vp]),
     "p3h_batch_tree": (_i32, [_vp, C.POINTER(_vp)]),
+    "p3h_batch_lde_launches": (_i32, [_vp, C.POINTER(_u32)]),
     "p3h_batch_free": (_i32, [_vp]),
     "p3h_autotune_lde": (_i32, [_vp, _i32, _u32, _u32, _u32, _u32, C.POINTER(_u32), C.POINTER(_u32),
                                 C.POINTER(_u32), C.POINTER(C.c_double)]),
@@ -413,12 +417,17 @@ class Context:
         self._check(self.lib.p3h_batch_commit(self.h, arr, _ptr(ab), len(traces), shift, cap_height, C.byref(h)))
         return Tree(self, h, BABYBEAR)
 
-    def batch_plan(self, traces, added_bits, shift: int, cap_height: int = 0, use_graph: bool = True) -> "BatchPlan":
+    def batch_plan(self, traces, added_bits, shift: int, cap_height: int = 0, use_graph: bool = True,
+                   lde_queue: bool = False) -> "BatchPlan":
+        """Planned batched commit (p3h_batch_plan).  lde_queue: the LDE passes of
+        all traces as one work-queue launch instead of launches grouped by pass
+        shape."""
         arr = (_vp * len(traces))(*[m.h for m in traces])
         ab = np.ascontiguousarray(added_bits, dtype=np.uint32)
         h = _vp()
-        self._check(self.lib.p3h_batch_plan(self.h, arr, _ptr(ab), len(traces), shift, cap_height,
-                                            1 if use_graph else 0, C.byref(h)))
+        flags = (BATCH_GRAPH if use_graph else 0) | (BATCH_QUEUE_LDE if lde_queue else 0)
+        self._check(self.lib.p3h_batch_plan(self.h, arr, _ptr(ab), len(traces), shift, cap_height, flags,
+                                            C.byref(h)))
         return BatchPlan(self, h, traces)
 
     def autotune_lde(self, field: int, log_n: int, width: int, added_bits: int, iters: int = 3):
@@ -607,6 +616,12 @@ class BatchPlan:
         out = np.empty((n.value, 8), dtype=np.uint32)
         self.ctx._check(self.ctx.lib.p3h_batch_run(self.h, _ptr(out)))
         return out
+
+    def lde_launches(self) -> int:
+        """Kernel launches of the LDE phase per run (1: the queue launch)."""
+        n = _u32()
+        self.ctx._check(self.ctx.lib.p3h_batch_lde_launches(self.h, C.byref(n)))
+        return n.value
 
     def tree(self) -> "Tree":
         tr = _vp()

@@ -121,11 +121,18 @@ def test_batch_config_full_size(wctx, fx):
     assert (t.cap() == want).all()
     _check_batch_openings(t, fx)
     t.free()
-    say("batch64: planned HIP-graph replays")
+    say("batch64: planned HIP-graph replays, LDE as launches grouped by pass shape")
     plan = wctx.batch_plan(mats, ks, W.SHIFT["bb"], cap_height=ch, use_graph=True)
+    assert plan.lde_launches() > 1
     for _ in range(2):
         assert (plan.run() == want).all()
     _check_batch_openings(plan.tree(), fx)
+    plan.free()
+    say("batch64: planned HIP-graph replays, LDE as one work-queue launch")
+    plan = wctx.batch_plan(mats, ks, W.SHIFT["bb"], cap_height=ch, use_graph=True, lde_queue=True)
+    assert plan.lde_launches() == 1
+    for _ in range(2):
+        assert (plan.run() == want).all()
     plan.free()
     say("batch64: unbatched LDE calls + one MMCS commit")
     ldes = [wctx.coset_lde(m, k, W.SHIFT["bb"]) for m, k in zip(mats, ks)]

@@ -539,11 +539,19 @@ def _oracle_batch(orc, golden, traces, ks, cap):
     return orc.mmcs("bb", golden["consts_bb"], RF, 13, ldes, cap, 3)
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_plan_replays_match_oracle(ctx, golden, orc, use_graph):
-    traces, ks = _batch_inputs(21, 7)
+@pytest.mark.parametrize("use_graph,lde_queue,hmin,hmax", [(False, True, 6, 10), (True, True, 6, 10),
+                                                        (True, False, 3, 10), (False, True, 11, 14),
+                                                        (True, True, 11, 14), (True, False, 11, 14)])
+def test_batch_plan_replays_match_oracle(ctx, golden, orc, use_graph, lde_queue, hmin, hmax):
+    """The planned batch (LDE as one work-queue launch or grouped launches,
+    eager or replayed as a HIP graph) against the oracle's per-trace LDEs and
+    multi-height MMCS.  Heights up to 2^10 are one fused inner pass per trace;
+    2^11..2^14 add outer passes, so queue items wait on their trace's previous
+    pass.  (Transforms of 2^4 and 2^5 rows keep the grouped launches.)"""
+    traces, ks = _batch_inputs(21 if hmax <= 10 else 23, 7, hmin, hmax)
     mats = [ctx.upload(t) for t in traces]
-    plan = ctx.batch_plan(mats, ks, 31, cap_height=1, use_graph=use_graph)
+    plan = ctx.batch_plan(mats, ks, 31, cap_height=1, use_graph=use_graph, lde_queue=lde_queue)
+    assert (plan.lde_launches() == 1) == lde_queue
     c, vals, proof = _oracle_batch(orc, golden, traces, ks, 1)
     for rep in range(3):
         cap = plan.run()

@@ -3,7 +3,7 @@ libp3hip under AddressSanitizer + UndefinedBehaviorSanitizer.
 
 * tests/cpp/oracle_sanitize.cpp drives every oracle entry point on small and
   edge-case inputs (g++ -fsanitize=address,undefined);
-* tests/cpp/planner_sanitize.cpp drives plan_lde_batch (csrc/ntt.hip) and the
+* tests/cpp/planner_sanitize.cpp drives plan_lde_batch (csrc/ntt_impl.h) and the
   serving footprint and admission rule (csrc/host_plan.h); hipcc applies the
   sanitizers to host code only, and the device code, built at -O0 to keep the
   build short, is never launched.
@@ -44,5 +44,7 @@ def test_host_planners_under_asan_ubsan(tmp_path):
                     "-packed-fp32-ops", *host_san, "-I" + os.path.join(ROOT, "include"),
                     "-I" + os.path.join(ROOT, "plonky3-gpu_amd", "csrc"),
                     os.path.join(ROOT, "tests", "cpp", "planner_sanitize.cpp"),
-                    os.path.join(ROOT, "plonky3-gpu_amd", "csrc", "ntt.hip"), "-o", exe], check=True)
+                    os.path.join(ROOT, "plonky3-gpu_amd", "csrc", "ntt_bb.hip"),
+                    os.path.join(ROOT, "plonky3-gpu_amd", "csrc", "ntt_gl.hip"),
+                    os.path.join(ROOT, "plonky3-gpu_amd", "csrc", "ntt_queue.hip"), "-o", exe], check=True)
     _run(exe)

@@ -1,4 +1,4 @@
-"""The Goldilocks NTT's twisted register rounds (csrc/ntt.hip, build_twisted and
+"""The Goldilocks NTT's twisted register rounds (csrc/ntt_impl.h, build_twisted and
 the POW2_ROOTS branch of butterflies), checked on the CPU in exact integers.
 
 A radix-2^R round whose stage-j twiddle for the pair (m, m + 2^j) is
