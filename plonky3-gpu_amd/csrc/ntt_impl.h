@@ -1197,7 +1197,7 @@ int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typenam
 // whatever the number of resident workgroups, and the hardware dispatcher
 // refills a CU as soon as one of its workgroups ends.  Results pass between
 // workgroups on different XCDs (each with its own L2), so a block's stores are
-// released at agent scope (written back past the XCD's L2, once per block)
+// released at agent scope (written back past the XCD's L2, once per chunk)
 // before its completion count, and a waiting workgroup acquires at agent
 // scope (once, after its wait) before reading.
 //
@@ -1205,7 +1205,8 @@ int launch_lde_group(const LdePassGroup<typename F::T>& g, const NttWork<typenam
 // and the item are scalar, the wave index is read through readfirstlane):
 // a region taken by one lane of a wave inside code that holds barriers lets
 // the compiler split the wave's lanes between loop nests, which livelocks
-// them.  A workgroup runs one block, so no loop holds the pass bodies.
+// them.  A workgroup takes one ticket, so the only loop around the pass
+// bodies runs over the chunk's blocks, with scalar bounds.
 //
 // The kernel carries the pass variants of batches above 2^5 rows: outer
 // passes of 1..10 stages, fused inner passes of 1..3 and 6..10, split outer
