@@ -50,6 +50,14 @@ P2_F64_FLOPS_PER_DIGEST = 8 * 5
 GL_HASH_VALU_PER_PERM = 13467
 SIMD_COUNT, CLOCK_HZ = 1024, 2.4e9
 VALU_ISSUE_PEAK = SIMD_COUNT * CLOCK_HZ / 2  # wave64 instructions per second
+# The binary64 leaf hash issues 5.373e9 wave64 VALU instructions per launch at
+# 2^20 x 256 (2^26 lane-permutations; SQ_INSTS_VALU,
+# profiles/ntt_counters_bb_2^20x256_b1.md), 5124 per permutation and lane, of
+# which 4941 are FP64.  Every FP64 instruction issues in about 4.1-4.3 cycles
+# per wave64 (profiles/valu_issue_rates.md), so its issue ceiling is one
+# wave64 instruction per SIMD every 4 cycles.
+BB_F64_HASH_VALU_PER_PERM = 5124
+FP64_ISSUE_PEAK = SIMD_COUNT * CLOCK_HZ / 4
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 FIXTURE = os.path.join(ROOT, "tests", "golden", "configs.npz")
@@ -281,11 +289,18 @@ def main():
     if field == "bb" and args.hash_engine == 1:
         flops = perms * P2_F64_FLOPS_PER_PERM + N * P2_F64_FLOPS_PER_DIGEST
         ach = flops / (hash_ms / 1e3) / 1e12
+        issued = perms * BB_F64_HASH_VALU_PER_PERM / 64 / (hash_ms / 1e3)  # wave64 instructions per second
         roofline = {"bound": "valu", "kernel": "hash_rows_dense_f64_kernel (Poseidon2 leaf sponge, binary64)",
                     "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                     "traffic": traffic.get("hash_rows_dense_f64_kernel"), "algorithmic_flops": flops,
                     "algorithmic_bytes": hash_bytes, "hbm_GBps": hash_bytes / (hash_ms / 1e3) / 1e9,
-                    "avg_launch_ms": hash_ms}
+                    "avg_launch_ms": hash_ms,
+                    "issue": {"unit": "G wave64 VALU instructions/s", "achieved": issued / 1e9,
+                              "peak": FP64_ISSUE_PEAK / 1e9, "frac": issued / FP64_ISSUE_PEAK,
+                              "valu_instructions_per_perm": BB_F64_HASH_VALU_PER_PERM,
+                              "peak_from": "one wave64 FP64 instruction per SIMD every 4 cycles "
+                                           "(profiles/valu_issue_rates.md); the FLOP fraction counts adds and "
+                                           "multiplies as one flop against the peak's fused multiply-adds"}}
     elif field == "gl":
         issued = perms * GL_HASH_VALU_PER_PERM / 64 / (hash_ms / 1e3)  # wave64 instructions per second
         roofline = {"bound": "valu", "kernel": "hash_rows_dense_kernel<GLDom> (Poseidon2 W8 leaf sponge, integer)",
