@@ -251,7 +251,8 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_dense_f64_kernel(RowSe
         s[5] = double(b.y);
         s[6] = double(b.z);
         s[7] = double(b.w);
-        Poseidon2F64::permute(s, p);
+        // lanes 0..7 are overwritten by the next chunk or canonicalised below
+        Poseidon2F64::permute<false>(s, p);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);

@@ -16,17 +16,27 @@
 // Product a*b (|a*b| < 2^81): h = a*b rounded, l = fma(a, b, -h) exact (two-
 // product), then reduce(h) + l.  |l| <= ulp(h)/2 <= 2^28, so the result has
 // magnitude < p/2 + 2^29 < 2^31.
+// Product with a shared multiplier (mul_q, |a*b| < 2^79): with bp = b/p
+// rounded, t = fma(a, bp, M) = M + q, q = round(a b / p) up to a rounding of
+// |a b / p| 2^-52; -q (p - 1) = fma(t, -(p - 1), M (p - 1)) is exact because
+// p - 1 = 15 * 2^27 and 15 q < 2^53; r = fma(a, b, -q (p - 1)) = a b - q p + q
+// is exact (below 2^53); (r - t) + (M + c) = a b - q p + c, with every step
+// exact.  |a b - q p| <= p (1/2 + |a b / p| 2^-52) < 0.57 p.  Five FP64
+// instructions instead of six, bp is computed once per multiplier, and the
+// last addition carries a constant c for free.
 //
 // Bounds through one permutation (the largest magnitudes, all < 2^53):
 //   inputs (canonical or reduced)           <= p
-//   S-box output                             <  2^31
-//   MDS-light output (coefficient sum 35)    <  35 * 2^31 < 2^37.2
-//   S-box input (MDS output + constant)      <  2^37.3, square < 2^74.6 < 2^81
-//   internal layer: lanes enter <= 2^37.2, 2^15 * lane + sum < 2^52.3; which
+//   S-box output (plus a folded constant)    <  2^31 (0.51 p + p/2)
+//   MDS-light output (coefficient sum 35)    <  35 * 2^31 < 2^36.2
+//   full-round S-box input (MDS output, or   <  2^36.2, square < 2^72.4 < 2^79
+//     a reduced value plus a constant)
+//   internal layer: lanes enter < 2^36.2, 2^15 * lane + sum < 2^52.3; which
 //   lanes are reduced after each partial round is fixed per engine below, with
 //   its own bound argument
-//   end of the permutation: every lane reduced, so a sponge's carried lanes
-//   start the next permutation below 2^31.
+//   end of the permutation: every lane reduced (lanes 0..7 optionally left
+//   for the caller), so a sponge's carried lanes start the next permutation
+//   below 2^31.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,6 +48,17 @@ struct P2ParamsF64 {
     double init[4][16];
     double term[4][16];
     double internal[32];
+    // Folded round constants of the one-lane-per-state engine: the S-box of
+    // full round r adds c with mc = MAGIC + c, where c is the balanced residue
+    // of MDS-light^-1 applied to the constants of the round after it (init
+    // round r + 1; after the last init round, the first partial round's
+    // constant on lane 0; term round r + 1; nothing after the last term
+    // round), so those rounds need no constant addition of their own.
+    double mc_init[4][16];
+    double mc_term[4][16];
+    // 2^i, the internal layer's diagonal: read from the argument block, so its
+    // fmas take it from a scalar register and update their lane in place
+    double diag[16];
     uint32_t half_full;
     uint32_t partial;
 };
@@ -56,6 +77,30 @@ struct BBF64 {
         double l = __builtin_fma(a, b, -h);
         return reduce(h) + l;
     }
+    static constexpr double PM1 = 2013265920.0;          // p - 1 = 15 * 2^27
+    static constexpr double MAGIC_PM1 = MAGIC * PM1;     // 45 * 2^78, exact
+    // The fma that forms -q (p - 1) takes two constants.  A gfx950 VOP3
+    // instruction reads at most one scalar operand and no literal, so one
+    // constant sits in a VGPR pair (magic_pm1) and the other in an SGPR pair
+    // (neg_pm1); both are opaque to the compiler, which otherwise copies the
+    // VGPR constant into a fresh pair for an accumulating v_fmac_f64 with the
+    // other as a literal, one extra move per product.
+    struct QConsts {
+        double mpm1, npm1;
+    };
+    __device__ __forceinline__ static QConsts qconsts() {
+        QConsts c{MAGIC_PM1, -PM1};
+        asm("" : "+v"(c.mpm1));
+        asm("" : "+s"(c.npm1));
+        return c;
+    }
+    // a b - q p + c for mc = MAGIC + c, bp = b * PINV; |a b| < 2^79, |c| < 2^51
+    __device__ __forceinline__ static double mul_q(double a, double b, double bp, double mc, const QConsts& k) {
+        const double t = __builtin_fma(a, bp, MAGIC);     // MAGIC + q
+        const double nq = __builtin_fma(t, k.npm1, k.mpm1);  // -q (p - 1), exact
+        const double r = __builtin_fma(a, b, nq);         // a b - q p + q, exact
+        return (r - t) + mc;
+    }
     // v with |v| < 2^51 * p -> canonical [0, p)
     __device__ __forceinline__ static uint32_t canonical(double v) {
         double r = reduce(v);
@@ -70,6 +115,29 @@ struct Poseidon2F64 {
         double x3 = BBF64::mul(x2, x);
         double x4 = BBF64::mul(x2, x2);
         return BBF64::mul(x3, x4);
+    }
+    // x^7 + c (mc = MAGIC + c) for |x| < 2^39.5, every product through mul_q:
+    // x2 = x x, x3 = x2 x, x6 = x3 x3, x7 = x6 x, so x / p serves three of the
+    // four products; 22 FP64 instructions against 24 for sbox() plus a
+    // constant addition.  Products: x^2 < 2^79, x2 x and x6 x < 2^70.
+    __device__ __forceinline__ static double sbox_q(double x, double mc, const BBF64::QConsts& k) {
+        const double xp = x * BBF64::PINV;
+        const double x2 = BBF64::mul_q(x, x, xp, BBF64::MAGIC, k);
+        const double x3 = BBF64::mul_q(x2, x, xp, BBF64::MAGIC, k);
+        const double x3p = x3 * BBF64::PINV;
+        const double x6 = BBF64::mul_q(x3, x3, x3p, BBF64::MAGIC, k);
+        return BBF64::mul_q(x6, x, xp, mc, k);
+    }
+    // x^7 for a partial-round lane 0 (|x| < 2^41, so x^2 may pass 2^79): the
+    // square in the two-product form, the rest through mul_q (x2 x and x6 x
+    // below 2^72); 23 instructions.
+    __device__ __forceinline__ static double sbox_p(double x, const BBF64::QConsts& k) {
+        const double x2 = BBF64::mul(x, x);
+        const double xp = x * BBF64::PINV;
+        const double x3 = BBF64::mul_q(x2, x, xp, BBF64::MAGIC, k);
+        const double x3p = x3 * BBF64::PINV;
+        const double x6 = BBF64::mul_q(x3, x3, x3p, BBF64::MAGIC, k);
+        return BBF64::mul_q(x6, x, xp, BBF64::MAGIC, k);
     }
     // [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] x, exact integer arithmetic
     __device__ __forceinline__ static void mat4(double* x) {
@@ -96,9 +164,10 @@ struct Poseidon2F64 {
 #pragma unroll
         for (int i = 0; i < 16; ++i) s[i] = s[i] + sums[i & 3];
     }
-    __device__ __forceinline__ static void full_round(double* s, const double* rc) {
+    // S-box plus the next round's folded constants, then MDS-light
+    __device__ __forceinline__ static void full_round_q(double* s, const double* mc, const BBF64::QConsts& k) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) s[i] = sbox(s[i] + rc[i]);
+        for (int i = 0; i < 16; ++i) s[i] = sbox_q(s[i], mc[i], k);
         mds_light(s);
     }
     // s_i <- 2^i s_i + sum(s), then every lane reduced
@@ -123,39 +192,55 @@ struct Poseidon2F64 {
     static constexpr uint16_t LAZY13[13] = {0xff80, 0xfc71, 0xff81, 0xffff, 0xfc00, 0xf3c0, 0xfc01,
                                             0xfff1, 0xffff, 0xfc00, 0xf3c0, 0xfc01, 0xffff};
     template <uint32_t MASK>
-    __device__ __forceinline__ static void internal_layer_lazy(double* s) {
+    __device__ __forceinline__ static void internal_layer_lazy(double* s, const double* diag) {
         double sum = 0.0;
 #pragma unroll
         for (int i = 0; i < 16; ++i) sum = sum + s[i];
         s[0] = (MASK & 1u) ? BBF64::reduce(s[0] + sum) : s[0] + sum;
 #pragma unroll
         for (int i = 1; i < 16; ++i) {
-            const double v = __builtin_fma(s[i], double(1u << i), sum);
+            const double v = __builtin_fma(s[i], diag[i], sum);
             s[i] = ((MASK >> i) & 1u) ? BBF64::reduce(v) : v;
         }
     }
+    // Round 0's constant is folded into the last init round (half_full >= 1).
     template <int R>
-    __device__ __forceinline__ static void partial_rounds13(double* s, const P2ParamsF64& p) {
+    __device__ __forceinline__ static void partial_rounds13(double* s, const P2ParamsF64& p, const BBF64::QConsts& k) {
         if constexpr (R < 13) {
-            s[0] = sbox(s[0] + p.internal[R]);
-            internal_layer_lazy<LAZY13[R]>(s);
-            partial_rounds13<R + 1>(s, p);
+            s[0] = sbox_p(R == 0 ? s[0] : s[0] + p.internal[R], k);
+            internal_layer_lazy<LAZY13[R]>(s, p.diag);
+            partial_rounds13<R + 1>(s, p, k);
         }
     }
+    // Constants: init round 0 and term round 0 add theirs; every other full
+    // round's, and partial round 0's when there are full rounds, arrive folded
+    // into the S-boxes of the round before (P2ParamsF64::mc_*).
+    // REDUCE_RATE = false leaves lanes 0..7 unreduced (below 2^36.2), for a
+    // sponge that overwrites all of them next or canonicalises them.
+    template <bool REDUCE_RATE = true>
     __device__ __forceinline__ static void permute(double* s, const P2ParamsF64& p) {
+        const BBF64::QConsts k = BBF64::qconsts();
         mds_light(s);
-        for (uint32_t r = 0; r < p.half_full; ++r) full_round(s, p.init[r]);
-        if (p.partial == 13) {
-            partial_rounds13<0>(s, p);
+        if (p.half_full) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) s[i] = s[i] + p.init[0][i];
+            for (uint32_t r = 0; r < p.half_full; ++r) full_round_q(s, p.mc_init[r], k);
+        }
+        if (p.partial == 13 && p.half_full) {
+            partial_rounds13<0>(s, p, k);
         } else {
             for (uint32_t r = 0; r < p.partial; ++r) {
-                s[0] = sbox(s[0] + p.internal[r]);
+                s[0] = sbox_p(r == 0 && p.half_full ? s[0] : s[0] + p.internal[r], k);
                 internal_layer(s);
             }
         }
-        for (uint32_t r = 0; r < p.half_full; ++r) full_round(s, p.term[r]);
+        if (p.half_full) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) s[i] = BBF64::reduce(s[i]);
+            for (int i = 0; i < 16; ++i) s[i] = s[i] + p.term[0][i];
+            for (uint32_t r = 0; r < p.half_full; ++r) full_round_q(s, p.mc_term[r], k);
+        }
+#pragma unroll
+        for (int i = REDUCE_RATE ? 0 : 8; i < 16; ++i) s[i] = BBF64::reduce(s[i]);
     }
 };
 

@@ -1741,6 +1741,91 @@ int p3h_ctx_set_hash_engine(p3h_ctx* ctx, int engine) {
     });
 }
 
+// MDS-light of the width-16 BabyBear Poseidon2 (4x4 circulant blocks, then
+// each lane plus the sum of its column over the blocks) applied to integers.
+static void bb_mds_light_int(const int64_t* x, int64_t* y) {
+    for (int b = 0; b < 16; b += 4) {
+        const int64_t* v = x + b;
+        const int64_t t01 = v[0] + v[1], t23 = v[2] + v[3], t0123 = t01 + t23;
+        y[b] = t0123 + v[1] + t01;
+        y[b + 1] = t0123 + v[1] + 2 * v[2];
+        y[b + 2] = t0123 + v[3] + t23;
+        y[b + 3] = t0123 + v[3] + 2 * v[0];
+    }
+    int64_t sums[4];
+    for (int k = 0; k < 4; ++k) sums[k] = y[k] + y[k + 4] + y[k + 8] + y[k + 12];
+    for (int i = 0; i < 16; ++i) y[i] += sums[i & 3];
+}
+
+// x with MDS-light(x) = v (mod p), canonical, by Gauss-Jordan elimination over
+// F_p on the matrix whose column j is MDS-light of the j-th unit vector
+// (determinant 5^4 35^4, invertible mod p).
+static void bb_mds_light_solve(const uint32_t* v, uint32_t* x) {
+    uint32_t a[16][17];
+    for (int j = 0; j < 16; ++j) {
+        int64_t e[16] = {}, col[16];
+        e[j] = 1;
+        bb_mds_light_int(e, col);
+        for (int i = 0; i < 16; ++i) a[i][j] = uint32_t(col[i] % BB_P);
+    }
+    for (int i = 0; i < 16; ++i) a[i][16] = v[i] % BB_P;
+    for (int c = 0; c < 16; ++c) {
+        int piv = c;
+        while (a[piv][c] == 0) ++piv;
+        if (piv != c)
+            for (int k = 0; k < 17; ++k) std::swap(a[piv][k], a[c][k]);
+        const uint32_t inv = bb_inv(a[c][c]);
+        for (int k = 0; k < 17; ++k) a[c][k] = bb_mul(a[c][k], inv);
+        for (int i = 0; i < 16; ++i) {
+            if (i == c || a[i][c] == 0) continue;
+            const uint32_t f = a[i][c];
+            for (int k = 0; k < 17; ++k) a[i][k] = uint32_t((a[i][k] + uint64_t(BB_P - f) * a[c][k]) % BB_P);
+        }
+    }
+    for (int i = 0; i < 16; ++i) x[i] = a[i][16];
+}
+
+// The binary64 engine's constants from the reference layout
+// initial[hf][16], terminal[hf][16], internal[rp] (any representatives).
+// Besides the plain tables, mc_init[r] / mc_term[r] hold MAGIC + c with c the
+// balanced residue of MDS-light^-1 of the constants of the round after round r
+// (init r + 1, then partial round 0 on lane 0; term r + 1; none after the
+// last), which the engine adds in round r's S-boxes: MDS-light(y + c) = MDS-light(y)
+// + next constants (mod p).
+static P2ParamsF64 bb_f64_params(const uint32_t* c, uint32_t hf, uint32_t rp) {
+    P2ParamsF64 q{};
+    uint32_t init[P2_MAX_HALF_FULL][16], term[P2_MAX_HALF_FULL][16];
+    size_t k = 0;
+    for (uint32_t r = 0; r < hf; ++r)
+        for (int i = 0; i < 16; ++i) init[r][i] = c[k++] % BB_P;
+    for (uint32_t r = 0; r < hf; ++r)
+        for (int i = 0; i < 16; ++i) term[r][i] = c[k++] % BB_P;
+    for (uint32_t r = 0; r < hf; ++r)
+        for (int i = 0; i < 16; ++i) {
+            q.init[r][i] = double(init[r][i]);
+            q.term[r][i] = double(term[r][i]);
+        }
+    for (uint32_t r = 0; r < rp; ++r) q.internal[r] = double(c[k++] % BB_P);
+    auto fold = [](const uint32_t* next, double* mc) {
+        uint32_t x[16] = {};
+        if (next) bb_mds_light_solve(next, x);
+        for (int i = 0; i < 16; ++i) {
+            const int64_t bal = x[i] > BB_P / 2 ? int64_t(x[i]) - int64_t(BB_P) : int64_t(x[i]);
+            mc[i] = BBF64::MAGIC + double(bal);
+        }
+    };
+    uint32_t p0[16] = {};
+    if (rp) p0[0] = uint32_t(q.internal[0]);
+    for (uint32_t r = 0; r < hf; ++r) {
+        fold(r + 1 < hf ? init[r + 1] : (rp ? p0 : nullptr), q.mc_init[r]);
+        fold(r + 1 < hf ? term[r + 1] : nullptr, q.mc_term[r]);
+    }
+    for (int i = 0; i < 16; ++i) q.diag[i] = double(1u << i);
+    q.half_full = hf;
+    q.partial = rp;
+    return q;
+}
+
 int p3h_poseidon2_set(p3h_ctx* ctx, int field, uint32_t width, uint32_t rf, uint32_t rp, const void* consts) {
     return api(ctx, [&] {
         if (!consts) throw Err{P3H_EINVAL, "null constants"};
@@ -1760,16 +1845,7 @@ int p3h_poseidon2_set(p3h_ctx* ctx, int field, uint32_t width, uint32_t rf, uint
             p.half_full = rf / 2;
             p.partial = rp;
             ctx->bbp = p;
-            P2ParamsF64 q{};
-            k = 0;
-            for (uint32_t r = 0; r < rf / 2; ++r)
-                for (int i = 0; i < 16; ++i) q.init[r][i] = double(c[k++] % BB_P);
-            for (uint32_t r = 0; r < rf / 2; ++r)
-                for (int i = 0; i < 16; ++i) q.term[r][i] = double(c[k++] % BB_P);
-            for (uint32_t r = 0; r < rp; ++r) q.internal[r] = double(c[k++] % BB_P);
-            q.half_full = rf / 2;
-            q.partial = rp;
-            ctx->bbp64 = q;
+            ctx->bbp64 = bb_f64_params(c, rf / 2, rp);
             ctx->p2_set[P3H_BABYBEAR] = true;
         } else if (field == P3H_GOLDILOCKS) {
             if (width != 8) throw Err{P3H_EINVAL, "Goldilocks Poseidon2 width must be 8"};
