@@ -33,11 +33,14 @@ GL_P = 0xFFFFFFFF00000001
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # vector and matrix FP64 are the same figure on MI355X
 # The BabyBear leaf hash runs the exact binary64 Poseidon2 engine: one width-16
-# permutation (8 full + 13 partial rounds, 132 of the 208 partial-round lane
-# reductions kept) is 2142 add + 564 mul + 2255 fma binary64 operations =
-# 7216 flops in 4961 FP64 VALU instructions (counted from poseidon2_f64.h);
-# each digest lane's final canonical reduction adds 5 flops.
-P2_F64_FLOPS_PER_PERM = 7216
+# permutation of the dense leaf kernel (8 full + 13 partial rounds, 132 of the
+# 208 partial-round lane reductions kept, round constants folded into the
+# S-boxes, the rate lanes left for the next absorption) is 2024 add + 295 mul
+# + 2239 fma binary64 operations = 6797 flops in 4558 FP64 VALU instructions
+# (counted in the kernel's gfx950 assembly; SQ_INSTS_VALU_FLOPS_FP64 agrees,
+# profiles/valu_counters_bb_2^20x256_b1.md); each digest lane's final
+# canonical reduction adds 5 flops.
+P2_F64_FLOPS_PER_PERM = 6797
 P2_F64_FLOPS_PER_DIGEST = 8 * 5
 # The Goldilocks leaf hash (integer engine, width 8, 8 full + 22 partial
 # rounds) issues 5.647e10 wave64 VALU instructions for 2^24 leaves x 16
@@ -50,13 +53,13 @@ P2_F64_FLOPS_PER_DIGEST = 8 * 5
 GL_HASH_VALU_PER_PERM = 13467
 SIMD_COUNT, CLOCK_HZ = 1024, 2.4e9
 VALU_ISSUE_PEAK = SIMD_COUNT * CLOCK_HZ / 2  # wave64 instructions per second
-# The binary64 leaf hash issues 5.373e9 wave64 VALU instructions per launch at
+# The binary64 leaf hash issues 4.884e9 wave64 VALU instructions per launch at
 # 2^20 x 256 (2^26 lane-permutations; SQ_INSTS_VALU,
-# profiles/ntt_counters_bb_2^20x256_b1.md), 5124 per permutation and lane, of
-# which 4941 are FP64.  Every FP64 instruction issues in about 4.1-4.3 cycles
+# profiles/valu_counters_bb_2^20x256_b1.md), 4658 per permutation and lane, of
+# which 4559 are FP64.  Every FP64 instruction issues in about 4.1-4.3 cycles
 # per wave64 (profiles/valu_issue_rates.md), so its issue ceiling is one
 # wave64 instruction per SIMD every 4 cycles.
-BB_F64_HASH_VALU_PER_PERM = 5124
+BB_F64_HASH_VALU_PER_PERM = 4658
 FP64_ISSUE_PEAK = SIMD_COUNT * CLOCK_HZ / 4
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
