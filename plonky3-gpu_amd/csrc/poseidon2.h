@@ -79,7 +79,7 @@ struct GLDom {
 struct Poseidon2GL {
     static constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
-    // H * 2^32 + L (mod p) as a value below 2^64, for L, H < 2^56:
+    // H * 2^32 + L (mod p) as a value below 2^64, for H < 2^56 and any L < 2^64:
     // H 2^32 + L = h_hi 2^64 + (h_lo + l_hi) 2^32 + l_lo = k 2^64 + T with the
     // carry of h_lo + l_hi moved into k < 2^24 + 1; 2^64 == 2^32 - 1, so the
     // value is T + k (2^32 - 1), whose wrap-around (at most once: the wrapped
@@ -157,6 +157,48 @@ struct Poseidon2GL {
 #pragma unroll
         for (int i = 1; i < 8; ++i) s[i] = fold((lo[i] << i) + sl, (hi[i] << i) + sh);
     }
+    // The 22 partial rounds on split accumulators: lane i is a pair (L_i, H_i)
+    // of 64-bit words with value H_i 2^32 + L_i, so the internal layer is
+    // L_i <- 2^i L_i + sum(L), H_i <- 2^i H_i + sum(H) with no fold.  Lane 0 is
+    // folded before its S-box every round and split again after it; lane i >= 1
+    // is folded (and split) only after the rounds whose bit i is set in
+    // FOLD22, and every lane once at the end.  The schedule keeps every word
+    // below 2^64 and every folded H below 2^56 (fold's range) in the worst
+    // case, which tests/test_gl_fold_schedule.py checks by magnitude recursion
+    // over the table read from this header: 83 folds in the 22 rounds against
+    // 176 with a fold of every lane in every round.
+    static constexpr uint8_t FOLD22[22] = {0x00, 0x00, 0xc0, 0xbe, 0xc0, 0xfe, 0x00, 0x00, 0xc0, 0xbe, 0xc0,
+                                           0xfe, 0x00, 0x00, 0xc0, 0xbe, 0xc0, 0xfe, 0x00, 0x00, 0xc0, 0x00};
+    template <int R, class P>
+    __device__ __forceinline__ static void partial_rounds22(uint64_t* L, uint64_t* H, const P& p) {
+        if constexpr (R < 22) {
+            const uint64_t x = sbox(add_rc(fold(L[0], H[0]), p.internal[R]));
+            L[0] = x & EPS;
+            H[0] = x >> 32;
+            uint64_t sl = L[0], sh = H[0];
+#pragma unroll
+            for (int i = 1; i < 8; ++i) {
+                sl += L[i];
+                sh += H[i];
+            }
+            L[0] += sl;
+            H[0] += sh;
+#pragma unroll
+            for (int i = 1; i < 8; ++i) {
+                L[i] = (L[i] << i) + sl;
+                H[i] = (H[i] << i) + sh;
+            }
+#pragma unroll
+            for (int i = 1; i < 8; ++i) {
+                if ((FOLD22[R] >> i) & 1) {
+                    const uint64_t v = fold(L[i], H[i]);
+                    L[i] = v & EPS;
+                    H[i] = v >> 32;
+                }
+            }
+            partial_rounds22<R + 1>(L, H, p);
+        }
+    }
     template <class P>
     __device__ __forceinline__ static void permute(uint64_t* s, const P& p) {
         mds_light(s);
@@ -165,9 +207,21 @@ struct Poseidon2GL {
             for (int i = 0; i < 8; ++i) s[i] = sbox(add_rc(s[i], p.init[r][i]));
             mds_light(s);
         }
-        for (uint32_t r = 0; r < p.partial; ++r) {
-            s[0] = sbox(add_rc(s[0], p.internal[r]));
-            internal_layer(s);
+        if (p.partial == 22) {
+            uint64_t L[8], H[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                L[i] = s[i] & EPS;
+                H[i] = s[i] >> 32;
+            }
+            partial_rounds22<0>(L, H, p);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s[i] = fold(L[i], H[i]);
+        } else {
+            for (uint32_t r = 0; r < p.partial; ++r) {
+                s[0] = sbox(add_rc(s[0], p.internal[r]));
+                internal_layer(s);
+            }
         }
         for (uint32_t r = 0; r < p.half_full; ++r) {
 #pragma unroll
