@@ -21,6 +21,9 @@ namespace {
 
 constexpr uint32_t BB_PRIME = 0x78000001u;
 
+static_assert(sizeof(ChalProgram) + sizeof(P2ParamsF64) + sizeof(void*) <= 4096,
+              "transcript kernel arguments past 4 KiB");
+
 __global__ __launch_bounds__(64) void chal_run_kernel(DevChal* st, ChalProgram prog, P2ParamsF64 p) {
     __shared__ uint32_t inb[8], outb[8];
     const uint32_t lane = threadIdx.x;

@@ -430,6 +430,11 @@ struct MultiTable {
     uint32_t n;
 };
 
+// Kernel arguments are passed by value in the argument block: keep every
+// kernel's total within 4 KiB.
+static_assert(sizeof(MultiTable) + sizeof(P2ParamsF64) <= 4096, "multi-job kernel arguments past 4 KiB");
+static_assert(sizeof(TailLevels) + sizeof(P2ParamsF64) <= 4096, "tail kernel arguments past 4 KiB");
+
 __device__ __forceinline__ uint32_t job_of(const MultiTable& m, uint32_t b) {
     uint32_t t = 0;
     for (uint32_t k = 1; k < m.n; ++k) t += b >= m.block0[k] ? 1u : 0u;
