@@ -647,16 +647,22 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t g, uint32_t grid) {
 // Waves per SIMD the register allocation must leave room for: the BabyBear
 // split passes hold 64 words per lane and would otherwise settle at 3, one
 // 8-wave workgroup per CU; 4 lets two workgroups (and their two LDS half
-// tiles) share a CU.  1 = no bound.
-template <class F, int MODE>
+// tiles) share a CU.  The Goldilocks fused LDE pass of 8 stages (the coset
+// coefficients of a 16-word low group held across the cosets) takes 140
+// registers unbounded, one workgroup per CU; bounded to 128 it spills 52
+// bytes per lane and two workgroups share a CU, which took the 2^22 x 64
+// blowup-4 LDE from 16.5 to 15.4 ms.  (At 9 and 10 stages the bound would
+// spill over 300 bytes per lane; 6 and 7 stages fit without it.)  1 = no bound.
+template <class F, int MODE, int S>
 constexpr int pass_min_waves() {
-    return (MODE == OUTER_DIF_INV_SPLIT || MODE == OUTER_DIT_FWD_SPLIT) && sizeof(typename F::T) == 4 ? 4 : 1;
+    if constexpr (sizeof(typename F::T) == 8) return MODE == INNER_LDE && S >= 6 && S <= 8 ? 4 : 1;
+    return (MODE == OUTER_DIF_INV_SPLIT || MODE == OUTER_DIT_FWD_SPLIT) ? 4 : 1;
 }
 
 // Workgroups stride over the pass's blocks when there are more blocks than the
 // grid holds (a dispatch counts at most 2^32 - 1 work-items).
 template <class F, int MODE, int S>
-__global__ __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(pass_min_waves<F, MODE>())))
+__global__ __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(pass_min_waves<F, MODE, S>())))
 void ntt_pass(NttArgs<typename F::T> A, uint32_t total) {
     extern __shared__ __align__(16) unsigned char ntt_smem[];
     const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x);
@@ -670,7 +676,7 @@ void ntt_pass(NttArgs<typename F::T> A, uint32_t total) {
 // run in one grid; block_start[i] is the first block of descriptor i
 // (ascending, block_start[count] = total).
 template <class F, int MODE, int S>
-__global__ __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(pass_min_waves<F, MODE>())))
+__global__ __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(pass_min_waves<F, MODE, S>())))
 void ntt_pass_list(const NttWork<typename F::T>* work, uint32_t count,
                                                              uint32_t total) {
     extern __shared__ __align__(16) unsigned char ntt_smem[];
