@@ -81,10 +81,10 @@ struct BBF64 {
     static constexpr double MAGIC_PM1 = MAGIC * PM1;     // 45 * 2^78, exact
     // The fma that forms -q (p - 1) takes two constants.  A gfx950 VOP3
     // instruction reads at most one scalar operand and no literal, so one
-    // constant sits in a VGPR pair (magic_pm1) and the other in an SGPR pair
-    // (neg_pm1); both are opaque to the compiler, which otherwise copies the
-    // VGPR constant into a fresh pair for an accumulating v_fmac_f64 with the
-    // other as a literal, one extra move per product.
+    // constant sits in a VGPR pair (QConsts::mpm1) and the other in an SGPR
+    // pair (QConsts::npm1); both are opaque to the compiler, which otherwise
+    // copies the VGPR constant into a fresh pair for an accumulating
+    // v_fmac_f64 with the other as a literal, one extra move per product.
     struct QConsts {
         double mpm1, npm1;
     };
