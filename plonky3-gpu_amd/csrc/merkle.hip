@@ -191,28 +191,58 @@ __global__ void open_rows_kernel(const RowSeg<T>* segs, uint32_t nsegs, uint32_t
 // ---------------------------------------------------------------- FP64 ----
 // The same sponge / compression with the BabyBear permutation evaluated in
 // binary64 (poseidon2_f64.h).  Inputs and outputs are canonical uint32.
+// Reads the next n <= 8 words of row j of the concatenated segments into v,
+// moving the cursor (segment, offset) forward; a segment without data reads
+// as zeros.
+struct SegCursor {
+    uint32_t seg, off;
+    RowSeg<uint32_t> cur;
+    const uint32_t* rowp;
+};
+__device__ __forceinline__ void seg_read8(const RowSeg<uint32_t>* segs, uint64_t j, SegCursor& c, uint32_t n,
+                                          uint32_t* v) {
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) {
+        if (q < n) {
+            while (c.off >= c.cur.w) {
+                c.off = 0;
+                ++c.seg;
+                c.cur = segs[c.seg];
+                c.rowp = c.cur.ptr + stored_row(c.cur, j) * c.cur.w;
+            }
+            v[q] = c.cur.ptr ? c.rowp[c.off] : 0u;
+            ++c.off;
+        }
+    }
+}
+
+// The overwrite sponge over row j of the concatenated segments.  The next
+// chunk's words are read before the current chunk's permutation, and the
+// rate lanes are left unreduced by a permutation whose successor overwrites
+// all of them; on return every lane is reduced (callers use the state as a
+// compression input).
 __device__ __forceinline__ void sponge_row_f64(double* s, const RowSeg<uint32_t>* segs, uint32_t total_w, uint64_t j,
                                                const P2ParamsF64& p) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) s[i] = 0.0;
-    uint32_t seg = 0, off = 0;
-    RowSeg<uint32_t> cur = segs[0];
-    const uint32_t* rowp = cur.ptr + stored_row(cur, j) * cur.w;
+    SegCursor c{0, 0, segs[0], nullptr};
+    c.rowp = c.cur.ptr + stored_row(c.cur, j) * c.cur.w;
+    uint32_t nx[8];
+    seg_read8(segs, j, c, total_w < 8 ? total_w : 8u, nx);
     for (uint32_t start = 0; start < total_w; start += 8) {
+        uint32_t v[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            if (start + q < total_w) {
-                while (off >= cur.w) {
-                    off = 0;
-                    ++seg;
-                    cur = segs[seg];
-                    rowp = cur.ptr + stored_row(cur, j) * cur.w;
-                }
-                s[q] = cur.ptr ? double(rowp[off]) : 0.0;
-                ++off;
-            }
+        for (int q = 0; q < 8; ++q) v[q] = nx[q];
+        const uint32_t n = total_w - start;  // words left, this chunk takes min(n, 8)
+        if (n > 8) seg_read8(segs, j, c, n - 8 < 8 ? n - 8 : 8u, nx);
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q)
+            if (q < n) s[q] = double(v[q]);
+        Poseidon2F64::permute<false>(s, p);
+        if (n < 16) {  // the next chunk is partial or there is none: its untouched rate lanes carry over
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s[q] = BBF64::reduce(s[q]);
         }
-        Poseidon2F64::permute(s, p);
     }
 }
 
@@ -460,7 +490,8 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_multi_f64_kernel(MultiTable
         s[5] = double(b.y);
         s[6] = double(b.z);
         s[7] = double(b.w);
-        Poseidon2F64::permute(s, p);
+        // lanes 0..7 are overwritten by the next chunk or canonicalised below
+        Poseidon2F64::permute<false>(s, p);
     }
     uint32_t* o = m.out[t] + j * 8;
 #pragma unroll
