@@ -51,6 +51,43 @@ def test_permutation_both_binary64_layouts(ctx, golden, orc):
         assert (ctx.permute(BB, st) == orc.permute("bb", golden["consts_bb"], RF, 13, st)).all(), n
 
 
+@pytest.mark.parametrize("kind", ["max", "zero", "random"])
+def test_permutation_under_extreme_round_constants(kind, orc):
+    """Both engines' constant handling against the oracle: the binary64
+    engine's constants folded through MDS-light^-1 into the previous round
+    (balanced residues computed on the host) and the Goldilocks partial rounds
+    on split accumulators, with every round constant p - 1, every one zero,
+    and random; states include all-(p-1) and all-zero rows, and the sponge
+    and the dense leaf kernel (whose rate lanes stay unreduced between
+    chunks) run on the same constants."""
+    def consts(f, n, p):
+        if kind == "max":
+            return np.full(n, p - 1, dtype=np.uint32 if f == "bb" else np.uint64)
+        if kind == "zero":
+            return np.zeros(n, dtype=np.uint32 if f == "bb" else np.uint64)
+        return rnd(77, n, f)
+    cb, cg = consts("bb", RF * 16 + 13, BB_P), consts("gl", RF * 8 + 22, GL_P)
+    with p3hip.Context(0) as c:
+        c.set_poseidon2(BB, cb, RF, 13)
+        c.set_poseidon2(GL, cg, RF, 22)
+        st = rnd(5, 16 * 4100)
+        st[:16] = BB_P - 1
+        st[16:32] = 0
+        assert (c.permute(BB, st) == orc.permute("bb", cb, RF, 13, st)).all()
+        stg = rnd(6, 8 * 1100, "gl")
+        stg[:8] = GL_P - 1
+        stg[8:16] = 0
+        assert (c.permute(GL, stg) == orc.permute("gl", cg, RF, 22, stg)).all()
+        rows = rnd(8, (1 << 16) * 24).reshape(1 << 16, 24)  # past 2^15 leaves: the dense leaf kernel
+        rows[0] = BB_P - 1
+        tree = c.mmcs_commit([c.upload(rows)], 0)
+        assert (tree.root() == orc.flat_commit("bb", cb, RF, 13, rows)).all()
+        rows_gl = rnd(9, 2048 * 12, "gl").reshape(2048, 12)
+        rows_gl[0] = GL_P - 1
+        tree_gl = c.mmcs_commit([c.upload(rows_gl)], 0)
+        assert (tree_gl.root() == orc.flat_commit("gl", cg, RF, 22, rows_gl)).all()
+
+
 # ------------------------------------------------------------------- DFT --
 def test_dft_golden(ctx, golden):
     dft = p3hip.Radix2DitHip(ctx)
