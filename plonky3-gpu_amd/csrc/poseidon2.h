@@ -120,8 +120,11 @@ struct Poseidon2GL {
         x[2] = y2;
         x[3] = y3;
     }
-    // mat4 on both halves of the state, then s_i + (s_i + s_{i+-4})
-    __device__ __forceinline__ static void mds_light(uint64_t* s) {
+    // mat4 on both halves of the state, then s_i + (s_i + s_{i+-4}).  With
+    // rc, the next round's constants are added as their 32-bit halves to the
+    // halves before the fold (each half then stays below 22 * 2^32), in place
+    // of a carry-checked 64-bit addition per lane.
+    __device__ __forceinline__ static void mds_light(uint64_t* s, const uint64_t* rc = nullptr) {
         uint64_t lo[8], hi[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -135,8 +138,15 @@ struct Poseidon2GL {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint64_t sl = lo[k] + lo[k + 4], sh = hi[k] + hi[k + 4];
-            s[k] = fold(lo[k] + sl, hi[k] + sh);
-            s[k + 4] = fold(lo[k + 4] + sl, hi[k + 4] + sh);
+            uint64_t l0 = lo[k] + sl, h0 = hi[k] + sh, l1 = lo[k + 4] + sl, h1 = hi[k + 4] + sh;
+            if (rc) {
+                l0 += rc[k] & EPS;
+                h0 += rc[k] >> 32;
+                l1 += rc[k + 4] & EPS;
+                h1 += rc[k + 4] >> 32;
+            }
+            s[k] = fold(l0, h0);
+            s[k + 4] = fold(l1, h1);
         }
     }
     // s_0 <- s_0 + sum(s), s_i <- 2^i s_i + sum(s)
@@ -166,13 +176,15 @@ struct Poseidon2GL {
     // below 2^64 and every folded H below 2^56 (fold's range) in the worst
     // case, which tests/test_gl_fold_schedule.py checks by magnitude recursion
     // over the table read from this header: 83 folds in the 22 rounds against
-    // 176 with a fold of every lane in every round.
+    // 176 with a fold of every lane in every round.  Each round's constant is
+    // added to lane 0 as its 32-bit halves just before the fold.
     static constexpr uint8_t FOLD22[22] = {0x00, 0x00, 0xc0, 0xbe, 0xc0, 0xfe, 0x00, 0x00, 0xc0, 0xbe, 0xc0,
                                            0xfe, 0x00, 0x00, 0xc0, 0xbe, 0xc0, 0xfe, 0x00, 0x00, 0xc0, 0x00};
     template <int R, class P>
     __device__ __forceinline__ static void partial_rounds22(uint64_t* L, uint64_t* H, const P& p) {
         if constexpr (R < 22) {
-            const uint64_t x = sbox(add_rc(fold(L[0], H[0]), p.internal[R]));
+            const uint64_t rc = p.internal[R];
+            const uint64_t x = sbox(fold(L[0] + (rc & EPS), H[0] + (rc >> 32)));
             L[0] = x & EPS;
             H[0] = x >> 32;
             uint64_t sl = L[0], sh = H[0];
@@ -199,13 +211,17 @@ struct Poseidon2GL {
             partial_rounds22<R + 1>(L, H, p);
         }
     }
+    // A full round's constants are added inside the MDS-light before it (the
+    // initial one for init round 0), and term round 0's in the folds that end
+    // the 22 partial rounds.
     template <class P>
     __device__ __forceinline__ static void permute(uint64_t* s, const P& p) {
-        mds_light(s);
-        for (uint32_t r = 0; r < p.half_full; ++r) {
+        const uint32_t hf = p.half_full;
+        mds_light(s, hf ? p.init[0] : nullptr);
+        for (uint32_t r = 0; r < hf; ++r) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) s[i] = sbox(add_rc(s[i], p.init[r][i]));
-            mds_light(s);
+            for (int i = 0; i < 8; ++i) s[i] = sbox(s[i]);
+            mds_light(s, r + 1 < hf ? p.init[r + 1] : nullptr);
         }
         if (p.partial == 22) {
             uint64_t L[8], H[8];
@@ -215,18 +231,27 @@ struct Poseidon2GL {
                 H[i] = s[i] >> 32;
             }
             partial_rounds22<0>(L, H, p);
+            if (hf) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) s[i] = fold(L[i], H[i]);
+                for (int i = 0; i < 8; ++i) s[i] = fold(L[i] + (p.term[0][i] & EPS), H[i] + (p.term[0][i] >> 32));
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) s[i] = fold(L[i], H[i]);
+            }
         } else {
             for (uint32_t r = 0; r < p.partial; ++r) {
                 s[0] = sbox(add_rc(s[0], p.internal[r]));
                 internal_layer(s);
             }
-        }
-        for (uint32_t r = 0; r < p.half_full; ++r) {
+            if (hf) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) s[i] = sbox(add_rc(s[i], p.term[r][i]));
-            mds_light(s);
+                for (int i = 0; i < 8; ++i) s[i] = add_rc(s[i], p.term[0][i]);
+            }
+        }
+        for (uint32_t r = 0; r < hf; ++r) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s[i] = sbox(s[i]);
+            mds_light(s, r + 1 < hf ? p.term[r + 1] : nullptr);
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) s[i] = Goldilocks::canon(s[i]);

@@ -5,7 +5,9 @@ the width-8 permutation on split accumulators: lane i is a pair of 64-bit
 words (L_i, H_i) with value H_i 2^32 + L_i, the internal layer updates
 L_i <- 2^i L_i + sum(L) and H_i <- 2^i H_i + sum(H), and a lane is folded
 back to one 64-bit word (and split again) only before its S-box (lane 0) or
-after the rounds whose bit is set in FOLD22.  The arithmetic is exact while
+after the rounds whose bit is set in FOLD22; a round constant joins lane 0
+as its 32-bit halves just before that lane's fold, and term round 0's
+constants join every lane at the final folds.  The arithmetic is exact while
 every word stays below 2^64 and every folded H below 2^56, the range of
 fold().  Random GPU inputs never reach the worst case, so the schedule read
 from the header is checked here by worst-case magnitude recursion, and the
@@ -37,7 +39,9 @@ def check(masks):
     bl, bh = [HALF] * 8, [HALF] * 8
     folds = 0
     for r, m in enumerate(masks):
-        assert bh[0] < FOLD_H, f"round {r}: lane 0 folded with H 2^{math.log2(bh[0]):.1f}"
+        # the round constant's high half joins lane 0's H just before its fold
+        assert bh[0] + HALF < FOLD_H, f"round {r}: lane 0 folded with H 2^{math.log2(bh[0]):.1f}"
+        assert bl[0] + HALF < WORD
         bl[0] = bh[0] = HALF  # the S-box output, split
         sl, sh = sum(bl), sum(bh)
         bl = [bl[0] + sl] + [(bl[i] << i) + sl for i in range(1, 8)]
@@ -49,8 +53,9 @@ def check(masks):
                 assert bh[i] < FOLD_H, f"round {r}: lane {i} folded with H 2^{math.log2(bh[i]):.1f}"
                 bl[i] = bh[i] = HALF
                 folds += 1
-    for i in range(8):
-        assert bh[i] < FOLD_H, f"final fold of lane {i}: H 2^{math.log2(bh[i]):.1f}"
+    for i in range(8):  # the final folds take term round 0's constants as halves
+        assert bh[i] + HALF < FOLD_H, f"final fold of lane {i}: H 2^{math.log2(bh[i]):.1f}"
+        assert bl[i] + HALF < WORD
     return folds
 
 
