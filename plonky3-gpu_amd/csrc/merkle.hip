@@ -221,6 +221,7 @@ __device__ __forceinline__ void seg_read8(const RowSeg<uint32_t>* segs, uint64_t
 // rate lanes are left unreduced by a permutation whose successor overwrites
 // all of them; on return every lane is reduced (callers use the state as a
 // compression input).
+template <bool RELOAD>
 __device__ __forceinline__ void sponge_row_f64(double* s, const RowSeg<uint32_t>* segs, uint32_t total_w, uint64_t j,
                                                const P2ParamsF64& p) {
 #pragma unroll
@@ -238,7 +239,7 @@ __device__ __forceinline__ void sponge_row_f64(double* s, const RowSeg<uint32_t>
 #pragma unroll
         for (uint32_t q = 0; q < 8; ++q)
             if (q < n) s[q] = double(v[q]);
-        Poseidon2F64::permute<false>(s, p);
+        Poseidon2F64::permute<false, RELOAD>(s, p);
         if (n < 16) {  // the next chunk is partial or there is none: its untouched rate lanes carry over
 #pragma unroll
             for (int q = 0; q < 8; ++q) s[q] = BBF64::reduce(s[q]);
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_f64_kernel(const RowSe
     uint64_t j = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
     if (j >= rows) return;
     double s[16];
-    sponge_row_f64(s, segs, total_w, j, p);
+    sponge_row_f64<true>(s, segs, total_w, j, p);
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);
 }
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_dense_f64_kernel(RowSe
         s[6] = double(b.z);
         s[7] = double(b.w);
         // lanes 0..7 are overwritten by the next chunk or canonicalised below
-        Poseidon2F64::permute<false>(s, p);
+        Poseidon2F64::permute<false, true>(s, p);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);
@@ -316,7 +317,7 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_ragged_f64_kernel(RowS
 #pragma unroll
         for (uint32_t q = 0; q < 8; ++q)
             if (q < n) s[q] = double(cur[q]);
-        Poseidon2F64::permute(s, p);
+        Poseidon2F64::permute<true, true>(s, p);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[j * 8 + i] = BBF64::canonical(s[i]);
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(HASH_THREADS) void compress_f64_kernel(const uint32
     Poseidon2F64::permute(s, p);
     if (inj) {
         double h[16];
-        sponge_row_f64(h, inj, inj_w, i, p);
+        sponge_row_f64<false>(h, inj, inj_w, i, p);
 #pragma unroll
         for (int q = 0; q < 8; ++q) s[8 + q] = h[q];
         Poseidon2F64::permute(s, p);
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_multi_f64_kernel(MultiTable
         s[6] = double(b.z);
         s[7] = double(b.w);
         // lanes 0..7 are overwritten by the next chunk or canonicalised below
-        Poseidon2F64::permute<false>(s, p);
+        Poseidon2F64::permute<false, true>(s, p);
     }
     uint32_t* o = m.out[t] + j * 8;
 #pragma unroll

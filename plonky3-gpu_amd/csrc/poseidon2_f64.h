@@ -205,11 +205,12 @@ struct Poseidon2F64 {
     }
     // Round 0's constant is folded into the last init round (half_full >= 1).
     template <int R>
-    __device__ __forceinline__ static void partial_rounds13(double* s, const P2ParamsF64& p, const BBF64::QConsts& k) {
+    __device__ __forceinline__ static void partial_rounds13(double* s, const double* internal, const double* diag,
+                                                            const BBF64::QConsts& k) {
         if constexpr (R < 13) {
-            s[0] = sbox_p(R == 0 ? s[0] : s[0] + p.internal[R], k);
-            internal_layer_lazy<LAZY13[R]>(s, p.diag);
-            partial_rounds13<R + 1>(s, p, k);
+            s[0] = sbox_p(R == 0 ? s[0] : s[0] + internal[R], k);
+            internal_layer_lazy<LAZY13[R]>(s, diag);
+            partial_rounds13<R + 1>(s, internal, diag, k);
         }
     }
     // Constants: init round 0 and term round 0 add theirs; every other full
@@ -217,17 +218,28 @@ struct Poseidon2F64 {
     // into the S-boxes of the round before (P2ParamsF64::mc_*).
     // REDUCE_RATE = false leaves lanes 0..7 unreduced (below 2^36.2), for a
     // sponge that overwrites all of them next or canonicalises them.
-    template <bool REDUCE_RATE = true>
+    // RELOAD = true reads the constants from the argument block at an offset
+    // z that is zero but opaque to the compiler, once per call.  A caller that
+    // loops over permutations then re-reads them with scalar loads, off the
+    // VALU; otherwise the compiler keeps them in scalar registers across the
+    // loop (more than the 102 it has), spills them to VGPR lanes and reads
+    // them back with one v_readlane_b32 per 32-bit half, in every permutation.
+    // Kernels that run two permutations and a sponge per thread (the injecting
+    // compression) keep RELOAD = false: there the opaque offset makes the
+    // compiler copy the argument block to scratch.
+    template <bool REDUCE_RATE = true, bool RELOAD = false>
     __device__ __forceinline__ static void permute(double* s, const P2ParamsF64& p) {
         const BBF64::QConsts k = BBF64::qconsts();
+        uint32_t z = 0;
+        if constexpr (RELOAD) asm volatile("" : "+s"(z));
         mds_light(s);
         if (p.half_full) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i) s[i] = s[i] + p.init[0][i];
+            for (int i = 0; i < 16; ++i) s[i] = s[i] + p.init[z][i];
             for (uint32_t r = 0; r < p.half_full; ++r) full_round_q(s, p.mc_init[r], k);
         }
         if (p.partial == 13 && p.half_full) {
-            partial_rounds13<0>(s, p, k);
+            partial_rounds13<0>(s, p.internal + z, p.diag + z, k);
         } else {
             for (uint32_t r = 0; r < p.partial; ++r) {
                 s[0] = sbox_p(r == 0 && p.half_full ? s[0] : s[0] + p.internal[r], k);
@@ -236,7 +248,7 @@ struct Poseidon2F64 {
         }
         if (p.half_full) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i) s[i] = s[i] + p.term[0][i];
+            for (int i = 0; i < 16; ++i) s[i] = s[i] + p.term[z][i];
             for (uint32_t r = 0; r < p.half_full; ++r) full_round_q(s, p.mc_term[r], k);
         }
 #pragma unroll
