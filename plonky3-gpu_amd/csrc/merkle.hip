@@ -28,7 +28,8 @@ __device__ __forceinline__ uint64_t stored_row(const RowSeg<T>& s, uint64_t j) {
 }
 
 // Absorb the concatenated row j of `segs` into state s (domain form).
-template <class D>
+// RELOAD: see Poseidon2GL::permute.
+template <class D, bool RELOAD>
 __device__ __forceinline__ void sponge_row(typename D::T* s, const RowSeg<typename D::T>* segs, uint32_t nsegs,
                                            uint32_t total_w, uint64_t j, const P2Params<typename D::T, D::W>& p) {
     using T = typename D::T;
@@ -51,7 +52,7 @@ __device__ __forceinline__ void sponge_row(typename D::T* s, const RowSeg<typena
                 ++off;
             }
         }
-        Poseidon2<D>::permute(s, p);
+        Poseidon2<D>::template permute<RELOAD>(s, p);
     }
     (void)nsegs;
 }
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_kernel(const RowSeg<ty
     uint64_t j = uint64_t(blockIdx.x) * HASH_THREADS + threadIdx.x;
     if (j >= rows) return;
     T s[D::W];
-    sponge_row<D>(s, segs, nsegs, total_w, j, p);
+    sponge_row<D, true>(s, segs, nsegs, total_w, j, p);
 #pragma unroll
     for (int i = 0; i < D::DIG; ++i) out[j * D::DIG + i] = D::from_dom(s[i]);
 }
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(HASH_THREADS) void hash_rows_dense_kernel(RowSeg<ty
 #pragma unroll
             for (int e = 0; e < PER16; ++e) s[q * PER16 + e] = D::to_dom(tmp[e]);
         }
-        Poseidon2<D>::permute(s, p);
+        Poseidon2<D>::template permute<true>(s, p);
     }
 #pragma unroll
     for (int i = 0; i < D::DIG; ++i) out[j * D::DIG + i] = D::from_dom(s[i]);
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(HASH_THREADS) void compress_kernel(const typename D
     Poseidon2<D>::permute(s, p);
     if (ninj) {
         T h[D::W];
-        sponge_row<D>(h, inj, ninj, inj_w, i, p);
+        sponge_row<D, false>(h, inj, ninj, inj_w, i, p);
 #pragma unroll
         for (int q = 0; q < D::DIG; ++q) s[D::DIG + q] = h[q];
 #pragma unroll

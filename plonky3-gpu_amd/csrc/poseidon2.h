@@ -180,25 +180,75 @@ struct Poseidon2GL {
     // added to lane 0 as its 32-bit halves just before the fold.
     static constexpr uint8_t FOLD22[22] = {0x00, 0x00, 0xc0, 0xbe, 0xc0, 0xfe, 0x00, 0x00, 0xc0, 0xbe, 0xc0,
                                            0xfe, 0x00, 0x00, 0xc0, 0xbe, 0xc0, 0xfe, 0x00, 0x00, 0xc0, 0x00};
-    template <int R, class P>
-    __device__ __forceinline__ static void partial_rounds22(uint64_t* L, uint64_t* H, const P& p) {
-        if constexpr (R < 22) {
-            const uint64_t rc = p.internal[R];
-            const uint64_t x = sbox(fold(L[0] + (rc & EPS), H[0] + (rc >> 32)));
-            L[0] = x & EPS;
-            H[0] = x >> 32;
-            uint64_t sl = L[0], sh = H[0];
+    // a b + c from a 32-bit a: one v_mad_u64_u32.  b is one of the opaque
+    // multipliers below, so the compiler cannot turn a product by a power of
+    // two into a shift of a zero-extended word, which costs two moves to build
+    // the (a, 0) register pair and a 64-bit shift-add.
+    __device__ __forceinline__ static uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) {
+        return uint64_t(a) * uint64_t(b) + c;
+    }
+    // 2^i, i < 8, in VGPRs the compiler cannot see through
+    struct Pow2 {
+        uint32_t m[8];
+        __device__ __forceinline__ Pow2() {
 #pragma unroll
-            for (int i = 1; i < 8; ++i) {
-                sl += L[i];
-                sh += H[i];
+            for (int i = 0; i < 8; ++i) {
+                m[i] = 1u << i;
+                asm("" : "+v"(m[i]));
             }
-            L[0] += sl;
-            H[0] += sh;
+        }
+    };
+    // Lanes whose (L, H) are 32-bit halves on entry to partial round R: every
+    // lane in round 0 (the split of the MDS-light output), lane 0 in every
+    // round (the split of its S-box output), and the lanes folded at the end
+    // of round R - 1.  Their terms go through mad32.
+    template <int R>
+    static constexpr uint8_t fresh_lanes() { return R == 0 ? 0xFF : uint8_t(FOLD22[R - 1] | 1u); }
+
+    template <int R>
+    __device__ __forceinline__ static void partial_rounds22(uint64_t* L, uint64_t* H, const uint64_t* internal,
+                                                            const Pow2& m) {
+        if constexpr (R < 22) {
+            constexpr uint8_t FRESH = fresh_lanes<R>();
+            const uint64_t rc = internal[R];
+            const uint64_t x = sbox(fold(L[0] + (rc & EPS), H[0] + (rc >> 32)));
+            const uint32_t x_lo = uint32_t(x), x_hi = uint32_t(x >> 32);
+            // sums of the 8 lanes: 64-bit accumulators first, then the halves
+            uint64_t sl = 0, sh = 0;
+            bool started = false;
 #pragma unroll
             for (int i = 1; i < 8; ++i) {
-                L[i] = (L[i] << i) + sl;
-                H[i] = (H[i] << i) + sh;
+                if (!((FRESH >> i) & 1)) {
+                    sl = started ? sl + L[i] : L[i];
+                    sh = started ? sh + H[i] : H[i];
+                    started = true;
+                }
+            }
+            if (!started) {
+                sl = x_lo;
+                sh = x_hi;
+            } else {
+                sl = mad32(x_lo, m.m[0], sl);
+                sh = mad32(x_hi, m.m[0], sh);
+            }
+#pragma unroll
+            for (int i = 1; i < 8; ++i) {
+                if ((FRESH >> i) & 1) {
+                    sl = mad32(uint32_t(L[i]), m.m[0], sl);
+                    sh = mad32(uint32_t(H[i]), m.m[0], sh);
+                }
+            }
+            L[0] = mad32(x_lo, m.m[0], sl);
+            H[0] = mad32(x_hi, m.m[0], sh);
+#pragma unroll
+            for (int i = 1; i < 8; ++i) {
+                if ((FRESH >> i) & 1) {
+                    L[i] = mad32(uint32_t(L[i]), m.m[i], sl);
+                    H[i] = mad32(uint32_t(H[i]), m.m[i], sh);
+                } else {
+                    L[i] = (L[i] << i) + sl;
+                    H[i] = (H[i] << i) + sh;
+                }
             }
 #pragma unroll
             for (int i = 1; i < 8; ++i) {
@@ -208,16 +258,22 @@ struct Poseidon2GL {
                     H[i] = v >> 32;
                 }
             }
-            partial_rounds22<R + 1>(L, H, p);
+            partial_rounds22<R + 1>(L, H, internal, m);
         }
     }
     // A full round's constants are added inside the MDS-light before it (the
     // initial one for init round 0), and term round 0's in the folds that end
-    // the 22 partial rounds.
-    template <class P>
+    // the 22 partial rounds.  RELOAD = true reads the constants at an offset
+    // z that is zero but opaque to the compiler, once per call, so that a
+    // caller looping over permutations re-reads them with scalar loads
+    // instead of keeping them in scalar registers across the loop (more than
+    // it has), spilled to VGPR lanes and read back with v_readlane_b32.
+    template <bool RELOAD = false, class P>
     __device__ __forceinline__ static void permute(uint64_t* s, const P& p) {
         const uint32_t hf = p.half_full;
-        mds_light(s, hf ? p.init[0] : nullptr);
+        uint32_t z = 0;
+        if constexpr (RELOAD) asm volatile("" : "+s"(z));
+        mds_light(s, hf ? p.init[z] : nullptr);
         for (uint32_t r = 0; r < hf; ++r) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) s[i] = sbox(s[i]);
@@ -230,10 +286,10 @@ struct Poseidon2GL {
                 L[i] = s[i] & EPS;
                 H[i] = s[i] >> 32;
             }
-            partial_rounds22<0>(L, H, p);
+            partial_rounds22<0>(L, H, p.internal + z, Pow2());
             if (hf) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) s[i] = fold(L[i] + (p.term[0][i] & EPS), H[i] + (p.term[0][i] >> 32));
+                for (int i = 0; i < 8; ++i) s[i] = fold(L[i] + (p.term[z][i] & EPS), H[i] + (p.term[z][i] >> 32));
             } else {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) s[i] = fold(L[i], H[i]);
@@ -311,9 +367,11 @@ struct Poseidon2 {
 #pragma unroll
         for (int i = 1; i < W; ++i) s[i] = D::add(D::mul_diag(s[i], uint32_t(i), diag[i]), sum);
     }
+    // RELOAD: see Poseidon2GL::permute (the Goldilocks engine only)
+    template <bool RELOAD = false>
     __device__ __forceinline__ static void permute(T* s, const P2Params<T, W>& p) {
         if constexpr (std::is_same<D, GLDom>::value) {
-            Poseidon2GL::permute(s, p);
+            Poseidon2GL::permute<RELOAD>(s, p);
             return;
         }
         mds_light(s);
