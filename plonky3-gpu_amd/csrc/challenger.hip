@@ -101,10 +101,11 @@ __device__ __forceinline__ uint32_t grind_sample(const DevChal* st, uint64_t w, 
         const uint32_t v = i < ell ? inp : (i == ell ? lo : (i == ell + 1 && ell < 7 ? hi : st->sponge[i]));
         s[i] = double(v);
     }
-    Poseidon2F64::permute(s, p);
+    // the grid loops over witness chunks: constants re-read per permutation
+    Poseidon2F64::permute<true, true>(s, p);
     if (ell == 7) {
         s[0] = double(hi);
-        Poseidon2F64::permute(s, p);
+        Poseidon2F64::permute<true, true>(s, p);
     }
     return BBF64::canonical(s[7]);
 }
