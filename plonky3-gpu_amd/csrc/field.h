@@ -250,6 +250,40 @@ struct Goldilocks {
         return pack(tl, th);
     }
 
+    // (hi, lo) = 128-bit value r3 r2 r1 r0 (32-bit limbs): x = (r1 r0) - r3 +
+    // r2 (2^32 - 1), lazy, with the borrow and carry corrections of mul_lz
+    // taken from VCC.  The same value as reduce128_lazy, without its 64-bit
+    // compares (each needs r3 moved into an (r3, 0) register pair) or the
+    // move that packs lo; u = r2 (2^32 - 1) is one v_mad_u64_u32.
+    __device__ __forceinline__ static uint64_t reduce128_lz(uint64_t hi, uint32_t r0, uint32_t r1) {
+        const uint64_t u = uint64_t(uint32_t(hi)) * 0xFFFFFFFFull;
+        uint32_t tl, th, m;
+        asm("v_sub_co_u32 %0, vcc, %3, %5\n\t"          // t = (r1 r0) - r3
+            "v_subbrev_co_u32 %1, vcc, 0, %4, vcc\n\t"
+            "v_cndmask_b32_e64 %2, 0, -1, vcc\n\t"       // borrow: t += 0xFFFFFFFF00000001
+            "v_addc_co_u32 %0, vcc, 0, %0, vcc\n\t"
+            "v_addc_co_u32 %1, vcc, %2, %1, vcc\n\t"
+            "v_add_co_u32 %0, vcc, %0, %6\n\t"           // t += u
+            "v_addc_co_u32 %1, vcc, %1, %7, vcc\n\t"
+            "v_cndmask_b32_e64 %2, 0, -1, vcc\n\t"       // carry: t += 2^32 - 1
+            "v_add_co_u32 %0, vcc, %0, %2\n\t"
+            "v_addc_co_u32 %1, vcc, 0, %1, vcc"
+            : "=&v"(tl), "=&v"(th), "=&v"(m)
+            : "v"(r0), "v"(r1), "v"(uint32_t(hi >> 32)), "v"(uint32_t(u)), "v"(uint32_t(u >> 32))
+            : "vcc");
+        return pack(tl, th);
+    }
+    // any a, b < 2^64 -> a b mod p below 2^64: mul128's partial products with
+    // the reduction of reduce128_lz
+    __device__ __forceinline__ static uint64_t mul_lazy_vcc(uint64_t a, uint64_t b) {
+        const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+        const uint64_t p00 = uint64_t(a0) * b0;
+        const uint64_t m0 = uint64_t(a0) * b1 + (p00 >> 32);
+        const uint64_t m1 = uint64_t(a1) * b0 + uint32_t(m0);
+        const uint64_t hi = uint64_t(a1) * b1 + (m0 >> 32) + (m1 >> 32);
+        return reduce128_lz(hi, uint32_t(p00), uint32_t(m1));
+    }
+
     // x lazy, 0 <= SH < 96: x * 2^SH mod p, lazy.  With x * 2^(SH mod 32) =
     // q2 q1 q0 (32-bit limbs; q2 < 2^31) and 2^64 = 2^32 - 1, 2^96 = -1:
     //   SH < 32: (q1 q0) + q2 (2^32 - 1)

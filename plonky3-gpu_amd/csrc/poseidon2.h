@@ -99,10 +99,10 @@ struct Poseidon2GL {
         return t < x ? t + EPS : t;
     }
     __device__ __forceinline__ static uint64_t sbox(uint64_t x) {
-        const uint64_t x2 = Goldilocks::mul_lazy(x, x);
-        const uint64_t x3 = Goldilocks::mul_lazy(x2, x);
-        const uint64_t x4 = Goldilocks::mul_lazy(x2, x2);
-        return Goldilocks::mul_lazy(x3, x4);
+        const uint64_t x2 = Goldilocks::mul_lazy_vcc(x, x);
+        const uint64_t x3 = Goldilocks::mul_lazy_vcc(x2, x);
+        const uint64_t x4 = Goldilocks::mul_lazy_vcc(x2, x2);
+        return Goldilocks::mul_lazy_vcc(x3, x4);
     }
     // [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] x over the integers
     __device__ __forceinline__ static void mat4(uint64_t* x) {
