@@ -43,14 +43,14 @@ FP64_PEAK_TFLOPS = 78.6  # vector and matrix FP64 are the same figure on MI355X
 P2_F64_FLOPS_PER_PERM = 6797
 P2_F64_FLOPS_PER_DIGEST = 8 * 5
 # The Goldilocks leaf hash (integer engine, width 8, 8 full + 22 partial
-# rounds) issues 4.759e10 wave64 VALU instructions for 2^24 leaves x 16
+# rounds) issues 4.634e10 wave64 VALU instructions for 2^24 leaves x 16
 # permutations (SQ_INSTS_VALU, profiles/valu_counters_gl_2^22x64_b2.md), i.e.
-# 11346 per permutation and lane.  Its VALU roofline is that instruction
+# 11048 per permutation and lane.  Its VALU roofline is that instruction
 # stream against one wave64 instruction per SIMD every 2 cycles (1024 SIMDs at
 # 2.4 GHz), the issue rate of MI355X's 157.3 TFLOP/s FP32 vector peak; its
 # 64-bit and multiply instructions take longer, so the fraction is a lower
 # bound on how busy the VALU is.
-GL_HASH_VALU_PER_PERM = 11346
+GL_HASH_VALU_PER_PERM = 11048
 SIMD_COUNT, CLOCK_HZ = 1024, 2.4e9
 VALU_ISSUE_PEAK = SIMD_COUNT * CLOCK_HZ / 2  # wave64 instructions per second
 # The binary64 leaf hash issues 4.812e9 wave64 VALU instructions per launch at
