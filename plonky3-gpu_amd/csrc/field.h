@@ -214,47 +214,15 @@ struct Goldilocks {
             : "vcc");
         return pack(l, h);
     }
-    // a, b any below 2^64: a * b mod p, lazy.  Four 32 x 32 -> 64 products
-    // give the 128-bit product r3 r2 r1 r0 (32-bit limbs); with 2^64 = 2^32 - 1
-    // and 2^96 = -1 (mod p), x = (r1 r0) - r3 + r2 (2^32 - 1).  A borrow of
-    // the subtraction adds 2^64 - (2^32 - 1) (the difference is then at least
-    // 2^64 - 2^32 + 1, so it cannot wrap); a carry of the addition adds
+    // (hi, lo) = 128-bit value r3 r2 r1 r0 (32-bit limbs): with 2^64 = 2^32 - 1
+    // and 2^96 = -1 (mod p), x = (r1 r0) - r3 + r2 (2^32 - 1), lazy.  A borrow
+    // of the subtraction adds 2^64 - (2^32 - 1) (the difference is then at
+    // least 2^64 - 2^32 + 1, so it cannot wrap); a carry of the addition adds
     // 2^32 - 1 (the wrapped sum is below 2^64 - 2^33 + 2, so it cannot wrap).
-    __device__ __forceinline__ static uint64_t mul_lz(uint64_t a, uint64_t b) {
-        const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
-        const uint64_t p00 = uint64_t(a0) * b0, p01 = uint64_t(a0) * b1, p10 = uint64_t(a1) * b0,
-                       p11 = uint64_t(a1) * b1;
-        uint32_t r1, r2, r3, tl, th, ul, uh, m;
-        asm("v_add_co_u32 %0, vcc, %8, %9\n\t"          // r1 = h00 + l01
-            "v_addc_co_u32 %1, vcc, %10, %11, vcc\n\t"   // r2 = h01 + l11 + c
-            "v_addc_co_u32 %2, vcc, 0, %12, vcc\n\t"     // r3 = h11 + c
-            "v_add_co_u32 %0, vcc, %0, %13\n\t"          // r1 += l10
-            "v_addc_co_u32 %1, vcc, %1, %14, vcc\n\t"    // r2 += h10 + c
-            "v_addc_co_u32 %2, vcc, 0, %2, vcc\n\t"      // r3 += c
-            "v_sub_co_u32 %3, vcc, %15, %2\n\t"          // t = (r1 r0) - r3
-            "v_subbrev_co_u32 %4, vcc, 0, %0, vcc\n\t"
-            "v_cndmask_b32_e64 %7, 0, -1, vcc\n\t"       // borrow: t += 0xFFFFFFFF00000001
-            "v_addc_co_u32 %3, vcc, 0, %3, vcc\n\t"
-            "v_addc_co_u32 %4, vcc, %7, %4, vcc\n\t"
-            "v_sub_co_u32 %5, vcc, 0, %1\n\t"            // u = r2 2^32 - r2
-            "v_subbrev_co_u32 %6, vcc, 0, %1, vcc\n\t"
-            "v_add_co_u32 %3, vcc, %3, %5\n\t"           // t += u
-            "v_addc_co_u32 %4, vcc, %4, %6, vcc\n\t"
-            "v_cndmask_b32_e64 %7, 0, -1, vcc\n\t"       // carry: t += 2^32 - 1
-            "v_add_co_u32 %3, vcc, %3, %7\n\t"
-            "v_addc_co_u32 %4, vcc, 0, %4, vcc"
-            : "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(tl), "=&v"(th), "=&v"(ul), "=&v"(uh), "=&v"(m)
-            : "v"(uint32_t(p00 >> 32)), "v"(uint32_t(p01)), "v"(uint32_t(p01 >> 32)), "v"(uint32_t(p11)),
-              "v"(uint32_t(p11 >> 32)), "v"(uint32_t(p10)), "v"(uint32_t(p10 >> 32)), "v"(uint32_t(p00))
-            : "vcc");
-        return pack(tl, th);
-    }
-
-    // (hi, lo) = 128-bit value r3 r2 r1 r0 (32-bit limbs): x = (r1 r0) - r3 +
-    // r2 (2^32 - 1), lazy, with the borrow and carry corrections of mul_lz
-    // taken from VCC.  The same value as reduce128_lazy, without its 64-bit
-    // compares (each needs r3 moved into an (r3, 0) register pair) or the
-    // move that packs lo; u = r2 (2^32 - 1) is one v_mad_u64_u32.
+    // Both corrections take their condition from VCC: the same value as
+    // reduce128_lazy without its 64-bit compares (each needs r3 moved into an
+    // (r3, 0) register pair) or the move that packs lo; u = r2 (2^32 - 1) is
+    // one v_mad_u64_u32.
     __device__ __forceinline__ static uint64_t reduce128_lz(uint64_t hi, uint32_t r0, uint32_t r1) {
         const uint64_t u = uint64_t(uint32_t(hi)) * 0xFFFFFFFFull;
         uint32_t tl, th, m;
@@ -273,8 +241,10 @@ struct Goldilocks {
             : "vcc");
         return pack(tl, th);
     }
-    // any a, b < 2^64 -> a b mod p below 2^64: mul128's partial products with
-    // the reduction of reduce128_lz
+    // any a, b < 2^64 -> a b mod p below 2^64 (lazy): mul128's partial
+    // products, whose carry addends the compiler places in v_mad_u64_u32, with
+    // the reduction of reduce128_lz.  Every 32-bit limb sum as an assembly
+    // carry chain instead (no addends) issued slower in the NTT and the hash.
     __device__ __forceinline__ static uint64_t mul_lazy_vcc(uint64_t a, uint64_t b) {
         const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
         const uint64_t p00 = uint64_t(a0) * b0;
@@ -411,21 +381,21 @@ struct Goldilocks {
     // add_lz / sub_lz require.  Words are made canonical when stored to global
     // memory (store_form).
     __device__ __forceinline__ static void dit(uint64_t& a, uint64_t& b, uint64_t w) {
-        const uint64_t t = canon_lz(mul_lz(b, w)), x = a;
+        const uint64_t t = canon_lz(mul_lazy_vcc(b, w)), x = a;
         a = add_lz(x, t);
         b = sub_lz(x, t);
     }
     __device__ __forceinline__ static void dif(uint64_t& a, uint64_t& b, uint64_t w) {
         const uint64_t y = canon_lz(b), x = a;
         a = add_lz(x, y);
-        b = mul_lz(sub_lz(x, y), w);
+        b = mul_lazy_vcc(sub_lz(x, y), w);
     }
     __device__ __forceinline__ static void bf1(uint64_t& a, uint64_t& b) {
         const uint64_t y = canon_lz(b), x = a;
         a = add_lz(x, y);
         b = sub_lz(x, y);
     }
-    __device__ __forceinline__ static uint64_t mul_ntt(uint64_t x, uint64_t c) { return mul_lz(x, c); }
+    __device__ __forceinline__ static uint64_t mul_ntt(uint64_t x, uint64_t c) { return mul_lazy_vcc(x, c); }
     __device__ __forceinline__ static uint64_t store_form(uint64_t x) { return canon_lz(x); }
 };
 
